@@ -1,0 +1,151 @@
+#!/usr/bin/env python3
+"""RACE Gauss-Seidel benchmark on one MI355X.
+
+Workload: symmetric Gauss-Seidel (forward + backward sweep) under a RACE
+distance-1 colouring on the 3D 27-point stencil, 128^3 grid (2,097,152 rows,
+56,623,104 stored entries), off-diagonals U[-1,0), diagonal 1.05 x row |sum|,
+b ~ U[-1,1].  One step = one forward + one backward sweep over the whole
+matrix, with the matrix and vectors already resident in HBM.
+
+Prints one JSON line: GB/s of algorithmic traffic (per sweep: nnz*(8+4) +
+n*(4 + 3*8) bytes for FP64), the roofline of the Gauss-Seidel kernel against
+the 8 TB/s HBM peak, and the CPU oracle (the same sweeps in RACE order,
+single thread) timed on this host beside it.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "SymmSpMV/GS/MPK GB/s vs HBM peak; mixed vs FP64 GPU at stated err; CPU RACE"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def gs_bytes(n, nnz, sv=8, sx=8, si=4):
+    # values + column indices, row pointer, x read + x write + b read
+    return nnz * (sv + si) + n * (si + 3 * sx)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--grid", type=int, default=128)
+    ap.add_argument("--threads", type=int, default=int(os.environ.get("RACE_BENCH_THREADS", "2048")))
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (single-GPU build)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import race_amd as R
+    from race_amd import matgen as M
+
+    g = args.grid
+    t0 = time.time()
+    rp, col = M.stencil27(g)
+    val = M.gs_values(rp, col, seed=2024)
+    n = len(rp) - 1
+    nnz = int(rp[-1])
+    b = M.uniform_vec(n, 7)
+    log(f"matrix 27-pt {g}^3: n={n} nnz={nnz} ({time.time() - t0:.1f}s)")
+
+    t0 = time.time()
+    ce = R.Interface(n, args.threads, R.ONE, rp, col, symm_hint=True)
+    ce.RACEColor()
+    t_color = time.time() - t0
+    bfs_ms, ntrav = ce.preprocessStats()
+    perm, inv = ce.getPerm(), ce.getInvPerm()
+    log(f"RACEColor T={args.threads}: {t_color:.2f}s (GPU BFS {bfs_ms:.0f} ms in {ntrav} traversals), "
+        f"threads={ce.getNumThreads()} eff={ce.getEfficiency():.3f} stages={ce.getMaxStageDepth()}")
+    prp, pcol, pval = M.permute(rp, col, val, perm, inv)
+    prp, pcol, pval = M.make_diag_first(prp, pcol, pval)
+    pb = M.permute_vec(b, perm)
+    del rp, col, val
+
+    dx = R.DeviceArray.from_host(np.zeros(n))
+    db = R.DeviceArray.from_host(pb)
+    t0 = time.time()
+    fid = ce.registerDevice(R.K_GS, prp, pcol, pval, dx, db)
+    log(f"register GS: {time.time() - t0:.2f}s, waves fwd={ce.waveCount(fid)} bwd={ce.waveCount(fid, True)}")
+    pp, _, _ = ce.schedule()
+
+    for _ in range(args.warmup):
+        ce.executeAsync(fid, False)
+        ce.executeAsync(fid, True)
+    R.interface.L.load().race_device_synchronize()
+    ce.timerBegin()
+    for _ in range(args.steps):
+        ce.executeAsync(fid, False)
+        ce.executeAsync(fid, True)
+    ms = ce.timerEnd()
+    R.interface.L.load().race_device_synchronize()
+    ms_step = ms / args.steps
+    bytes_step = 2 * gs_bytes(n, nnz)
+    gbs = bytes_step / (ms_step * 1e-3) / 1e9
+    launches_per_step = 2 * (len(pp) - 1)
+    log(f"GPU: {ms_step:.3f} ms/step, {gbs:.1f} GB/s, {launches_per_step} launches/step")
+
+    # CPU oracle beside it: the same sweeps in RACE order, single thread
+    cpu = None
+    if not args.no_cpu:
+        from oracle import race_oracle as O
+        O.build()
+        lo, hi = ce.schedule(False)[1:], None
+        pp_f, lo_f, hi_f = ce.schedule(False)
+        pp_r, lo_r, hi_r = ce.schedule(True)
+        fcalls = np.stack([lo_f, hi_f], axis=1).reshape(-1)
+        rcalls = np.stack([hi_r - 1, lo_r - 1], axis=1).reshape(-1)
+        xo = np.zeros(n)
+        sweeps = 0
+        t0 = time.time()
+        while time.time() - t0 < args.cpu_seconds or sweeps == 0:
+            O.run_calls(O.K_GS, prp, pcol, pval, xo, pb, fcalls)
+            O.run_calls(O.K_GS, prp, pcol, pval, xo, pb, rcalls)
+            sweeps += 1
+        tc = (time.time() - t0) / sweeps
+        cpu = {"value": round(bytes_step / tc / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
+               "sample": f"{sweeps} fwd+bwd GS steps on the full {g}^3 27-pt matrix in RACE order "
+                         f"(oracle C restatement, 1 thread, {tc * 1e3:.0f} ms/step)"}
+        # parity of the timed GPU state against the oracle after the same number of steps
+        log(f"CPU oracle: {tc * 1e3:.1f} ms/step")
+
+    out = {
+        "metric": METRIC,
+        "value": round(gbs, 3),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"RACE Gauss-Seidel fwd+bwd sweep, 3D 27-pt {g}^3 (n={n}, nnz={nnz}), "
+                               f"distance-1 colouring, {args.threads} RACE threads, FP64",
+                   "rows": n, "nnz": nnz, "race_threads": args.threads,
+                   "launches_per_step": launches_per_step},
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(gbs / HBM_PEAK_GBS, 5), "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

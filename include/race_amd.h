@@ -1,0 +1,168 @@
+/*
+ * race_amd.h -- C ABI of the MI355X RACE engine.
+ *
+ * Drop-in boundary for RRZE-HPC/RACE's RACE::Interface (include/interface.h).
+ * Every entry point returns an int status whose values equal the
+ * reference's RACE_error enum (include/error.h:29-41); none of them exits the
+ * process.  A handle is not thread-safe: use one host thread per handle.
+ * All pointers are plain host pointers unless documented as device pointers.
+ * The C++ facade include/race_amd_interface.hpp rebuilds RACE::Interface on
+ * top of these functions.
+ */
+#ifndef RACE_AMD_H
+#define RACE_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes == RACE_error (include/error.h:29-41) */
+enum race_status {
+    RACE_SUCCESS = 0,
+    RACE_ERR_INVALID_ARG = 1,
+    RACE_ERR_MATRIX_SYMM = 2,
+    RACE_ERR_D2_COLOR = 3,
+    RACE_ERR_D1_COLOR = 4,
+    RACE_ERR_GRAPH_TRAVERSAL = 5,
+    RACE_ERR_INCOMPATIBILITY = 6,
+    RACE_ERR_HWLOC = 7,
+    RACE_ERR_PIN = 8,
+    RACE_ERR_INTERNAL = 9,
+    RACE_ERR_NOT_IMPLEMENTED = 10
+};
+
+/* RACE::dist, d2Method, LBTarget, PinMethod (include/type.h:31-55) */
+enum race_dist { RACE_ONE = 1, RACE_TWO = 2, RACE_POWER = 3 };
+enum race_d2_method { RACE_TWO_BLOCK = 0, RACE_THREE_BLOCK = 1 };
+enum race_lb_target { RACE_LB_ROW = 0, RACE_LB_NNZ = 1 };
+enum race_pin_method { RACE_FILL = 0, RACE_SCATTER = 1 };
+
+typedef struct race_handle_s *race_handle;
+
+/* Interface::Interface (include/interface.h:127, src/interface.cpp:33-36).
+ * rowPtr/col are copied (the reference borrows them). */
+int race_create(race_handle *out, int nrow, int nthreads, int dist, const int *rowPtr,
+                const int *col, int symm_hint, int smt, int pin_method, const int *init_perm,
+                const int *init_inv_perm, int d2_method, int lb_target);
+/* Interface::~Interface (src/interface.cpp:38-60) */
+int race_destroy(race_handle h);
+
+/* Interface::RACEColor() -- colouring preprocessing (src/interface.cpp:260-301).
+ * RACE_EFFICIENCY and RACE_THREADS are read from the environment like the
+ * reference (src/level_recursion.cpp:62-70). */
+int race_color(race_handle h);
+/* Interface::RACEColor(highestPower, cacheSize, safetyFactor, mtxType,
+ * highestSubPower) -- MPK preprocessing (src/interface.cpp:205-257).
+ * cache_size is in bytes; mtx_type is 'N', 'L' or 'U'. */
+int race_color_power(race_handle h, int highest_power, double cache_size, double safety,
+                     char mtx_type, int highest_sub_power);
+
+/* Interface::getPerm / getInvPerm (src/interface.cpp:323-375): perm[new]=old,
+ * copied into caller buffers of length len (== nrow). */
+int race_get_perm(race_handle h, int *perm, int len);
+int race_get_inv_perm(race_handle h, int *inv_perm, int len);
+/* Interface::getNumThreads / getEfficiency / getMaxStageDepth /
+ * getHighestPower (src/interface.cpp:304-321, 377-380, 1054-1057) */
+int race_get_num_threads(race_handle h, int *out);
+int race_get_efficiency(race_handle h, double *out);
+int race_get_max_stage_depth(race_handle h, int *out);
+int race_get_highest_power(race_handle h, int *out);
+
+/* ---- host callbacks (drop-in for registerFunction with CPU kernels) ---- */
+typedef void (*race_color_fn)(int start, int end, void *args);
+typedef void (*race_power_fn)(int start, int end, int pow, int sub_pow, int numa, void *args);
+/* Interface::registerFunction(void(*)(int,int,void*), void*)
+ * (include/interface.h:158, src/interface.cpp:397-420).  start>end asks for
+ * reverse order, as in the reference. */
+int race_register_function(race_handle h, race_color_fn f, void *args, int *id);
+/* Interface::registerFunction(void(*)(int,int,int,int,int,void*), void*, power,
+ * subPower, numaSplit) (include/interface.h:163, src/interface.cpp:435-458) */
+int race_register_power_function(race_handle h, race_power_fn f, void *args, int power,
+                                 int sub_power, int numa_split, int *id);
+
+/* ---- device kernels (the GPU path) ---- */
+enum race_kernel_kind {
+    RACE_K_SYMMSPMV = 0, /* b += A x, upper triangle stored, diagonal first
+                            (SYMM_SPMV_KERNEL examples/src/helpers/kernels.cpp:403-427) */
+    RACE_K_GS = 1,       /* Gauss-Seidel sweep (GS_KERNEL kernels.cpp:174-199) */
+    RACE_K_KACZ = 2,     /* Kaczmarz sweep (KACZ_KERNEL kernels.cpp:285-316) */
+    RACE_K_SPMV = 3,     /* b = A x (SPMV_KERNEL kernels.cpp:5-25) */
+    RACE_K_MPK = 4       /* x_p = A x_{p-1}, p=1..power (MAT_SPMV_KERNEL kernels.cpp:749-777) */
+};
+enum race_precision { RACE_FP64 = 0, RACE_FP32 = 1, RACE_BF16 = 2 };
+
+typedef struct {
+    int kind;            /* race_kernel_kind */
+    int nrow;
+    const int *rowPtr;   /* host CRS of the permuted matrix, diagonal first in each row */
+    const int *col;
+    const double *val;
+    int val_precision;   /* storage precision of the matrix values on the device */
+    int vec_precision;   /* RACE_FP64 or RACE_FP32 for x and b */
+    int nrhs;            /* right-hand sides, vectors are column-major v[r + k*nrow] */
+    void *x;             /* device pointer */
+    void *b;             /* device pointer (unused for MPK) */
+    int power;           /* MPK: number of powers; x holds (power+1)*nrow entries */
+} race_kernel_desc;
+
+/* Typed device registration: the GPU counterpart of registerFunction; the
+ * matrix is uploaded and laid out once, the vectors stay caller-owned. */
+int race_register_device(race_handle h, const race_kernel_desc *desc, int *id);
+/* re-point a registered device function at other vectors */
+int race_set_vectors(race_handle h, int id, void *x, void *b);
+
+/* Interface::executeFunction (src/interface.cpp:541-572): synchronous. */
+int race_execute(race_handle h, int id, int rev);
+/* Asynchronous variant on a caller HIP stream (hipStream_t, may be NULL for
+ * the handle's own stream). */
+int race_execute_async(race_handle h, int id, int rev, void *stream);
+/* Interface::setPower/getPower (src/interface.cpp:574-592) */
+int race_set_power(race_handle h, int id, int power);
+int race_get_power(race_handle h, int id, int *power);
+/* Interface::tuneFunction (src/interface.cpp:605-647): best power by timing */
+int race_tune_function(race_handle h, int id, int rev, int *best_power);
+/* Interface::checkPowerCoverage (src/interface.cpp:1060-1095): returns
+ * RACE_SUCCESS when every (row, power) is computed exactly once. */
+int race_check_power_coverage(race_handle h);
+
+/* ---- introspection ---- */
+/* flattened colour schedule: phase_ptr[nphases+1], leaf_lo/hi[nleaves] */
+int race_get_schedule(race_handle h, int rev, int *nphases, int *nleaves, int *phase_ptr,
+                      int *leaf_lo, int *leaf_hi, int cap);
+/* zone tree as flat arrays (leaf i: valueZ[0], valueZ[1], parent, nthreads, stage, effRow) */
+int race_get_tree(race_handle h, int *nleaves, int *data6, int cap);
+/* MPK level groups: group_ptr[ngroups+1] (row offsets) */
+int race_get_mpk_groups(race_handle h, int *ngroups, int *group_ptr, int cap);
+/* dependency waves of a registered GS/KACZ function (critical path length) */
+int race_get_wave_count(race_handle h, int id, int rev, int *waves);
+/* HIP-event timer on the handle's stream (brackets work launched there) */
+int race_timer_begin(race_handle h);
+int race_timer_end(race_handle h, float *ms);
+/* seconds spent in GPU traversals during preprocessing */
+int race_get_preprocess_stats(race_handle h, double *bfs_ms, int *traversals);
+
+/* ---- host-only scheduling logic (no GPU needed) ---- */
+/* LB::splitZones on a level histogram (src/lb.cpp:188-667).  Outputs rows per
+ * zone (blocks), nested thread count per thread, level boundaries. */
+int race_lb_split(const int *level_row, const int *level_nnz, int nlevels, int nthreads,
+                  double efficiency, int dist, int d2_method, int lb_target, int cap,
+                  int *nblocks, int *zone_rows, int *level_ptr, int *nthreads_out, int *scale);
+
+/* ---- device memory helpers (for callers without a HIP toolchain) ---- */
+int race_device_malloc(void **p, size_t bytes);
+int race_device_free(void *p);
+int race_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int race_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int race_memset_device(void *dst, int value, size_t bytes);
+int race_device_synchronize(void);
+int race_device_count(int *count);
+
+/* error text, RACE_error_string (src/error.cpp:26-60) */
+const char *race_error_string(int status);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RACE_AMD_H */

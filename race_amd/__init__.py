@@ -1,0 +1,13 @@
+"""race_amd -- MI355X-native RACE execution engine.
+
+The product is librace_amd.so (C ABI in include/race_amd.h, HIP kernels for
+gfx950 under race_amd/csrc).  This package holds the Python mirror of
+RACE::Interface over that ABI, the synthetic matrix generators and the build
+script.
+"""
+from .interface import (BF16, FP32, FP64, K_GS, K_KACZ, K_MPK, K_SPMV, K_SYMMSPMV, LB_NNZ, LB_ROW, ONE,
+                        POWER, TWO, TWO_BLOCK, DeviceArray, Interface, RaceError, device_count, lb_split)
+
+__all__ = ["Interface", "DeviceArray", "RaceError", "lb_split", "device_count", "ONE", "TWO", "POWER",
+           "K_SYMMSPMV", "K_GS", "K_KACZ", "K_SPMV", "K_MPK", "FP64", "FP32", "BF16", "LB_NNZ", "LB_ROW",
+           "TWO_BLOCK"]

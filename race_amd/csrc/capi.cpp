@@ -1,0 +1,553 @@
+// capi.cpp -- C ABI of the MI355X RACE engine (include/race_amd.h).
+//
+// A handle mirrors RACE::Interface (include/interface.h:54-205,
+// src/interface.cpp): it owns the graph (copied to the device), the GPU
+// traverser, the zone tree or MPK partition and the registered functions.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "../../include/race_amd.h"
+#include "bfs.hpp"
+#include "engine.hpp"
+#include "exec.hpp"
+
+using namespace race;
+
+namespace {
+
+struct HostFn {
+    race_color_fn color = nullptr;
+    race_power_fn powerFn = nullptr;
+    void* args = nullptr;
+    int power = 1, subPower = 1;
+};
+
+struct Function {
+    std::unique_ptr<HostFn> host;
+    std::unique_ptr<DeviceFn> dev;
+};
+
+// Pattern symmetrisation: add (c,r) for every (r,c) without its pair
+// (RACE::makeSymmetricGraph, src/graph.cpp:47-142).  Added columns are
+// appended after the row's own entries.
+void symmetrizePattern(int n, const int* rp, const int* col, std::vector<int>& orp,
+                       std::vector<int>& ocol, bool* changed) {
+    std::vector<std::vector<int>> extra(n);
+    bool sym = true;
+    for (int r = 0; r < n; ++r)
+        for (int j = rp[r]; j < rp[r + 1]; ++j) {
+            const int c = col[j];
+            if (c == r || c < 0 || c >= n) continue;
+            bool has = false;
+            for (int k = rp[c]; k < rp[c + 1]; ++k)
+                if (col[k] == r) {
+                    has = true;
+                    break;
+                }
+            if (!has) {
+                extra[c].push_back(r);
+                sym = false;
+            }
+        }
+    *changed = !sym;
+    if (sym) return;
+    orp.assign(n + 1, 0);
+    for (int r = 0; r < n; ++r) orp[r + 1] = orp[r] + (rp[r + 1] - rp[r]) + (int)extra[r].size();
+    ocol.resize(orp[n]);
+    for (int r = 0; r < n; ++r) {
+        int o = orp[r];
+        for (int j = rp[r]; j < rp[r + 1]; ++j) ocol[o++] = col[j];
+        for (int c : extra[r]) ocol[o++] = c;
+    }
+}
+
+}  // namespace
+
+struct race_handle_s {
+    int nrow = 0, requested = 1, dist = DIST_ONE, d2 = TWO_BLOCK, lbTarget = LB_NNZ;
+    int smt = 1, pin = 1;
+    bool symmHint = false;
+    std::vector<int> rowPtr, col;  // graph pattern used for traversal (copied)
+    long nnzInput = 0;
+    std::vector<int> initPerm, initInv;
+    hipStream_t stream = nullptr;
+    std::unique_ptr<GpuTraverser> trav;
+    std::unique_ptr<ZoneTree> tree;
+    int availableThreads = -1;
+    bool colored = false;
+    Schedule fwd, rev;
+    MPKPartition mpk;
+    int highestPower = 1, highestSubPower = 1;
+    std::vector<int> perm, inv;
+    std::vector<Function> funcs;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    ~race_handle_s() {
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+    }
+};
+
+static int checkHandle(race_handle h) { return h ? OK : ERR_INVALID_ARG; }
+
+extern "C" {
+
+const char* race_error_string(int s) {
+    switch (s) {
+        case RACE_SUCCESS: return "RACE_SUCCESS";
+        case RACE_ERR_INVALID_ARG: return "RACE_ERR_INVALID_ARG";
+        case RACE_ERR_MATRIX_SYMM: return "RACE_ERR_MATRIX_SYMM";
+        case RACE_ERR_D2_COLOR: return "RACE_ERR_D2_COLOR";
+        case RACE_ERR_D1_COLOR: return "RACE_ERR_D1_COLOR";
+        case RACE_ERR_GRAPH_TRAVERSAL: return "RACE_ERR_GRAPH_TRAVERSAL";
+        case RACE_ERR_INCOMPATIBILITY: return "RACE_ERR_INCOMPATIBILITY";
+        case RACE_ERR_HWLOC: return "RACE_ERR_HWLOC";
+        case RACE_ERR_PIN: return "RACE_ERR_PIN";
+        case RACE_ERR_INTERNAL: return "RACE_ERR_INTERNAL";
+        case RACE_ERR_NOT_IMPLEMENTED: return "RACE_ERR_NOT_IMPLEMENTED";
+        default: return "RACE_ERR_UNKNOWN";
+    }
+}
+
+int race_create(race_handle* out, int nrow, int nthreads, int dist, const int* rowPtr, const int* col,
+                int symm_hint, int smt, int pin_method, const int* init_perm, const int* init_inv_perm,
+                int d2_method, int lb_target) {
+    if (!out || nrow <= 0 || !rowPtr || !col || nthreads <= 0) return ERR_INVALID_ARG;
+    if (dist != DIST_ONE && dist != DIST_TWO && dist != DIST_POWER) return ERR_INVALID_ARG;
+    if ((init_perm == nullptr) != (init_inv_perm == nullptr)) return ERR_INVALID_ARG;
+    auto h = std::make_unique<race_handle_s>();
+    h->nrow = nrow;
+    h->requested = nthreads;
+    h->dist = dist;
+    h->d2 = d2_method;
+    h->lbTarget = lb_target;
+    h->smt = smt;
+    h->pin = pin_method;
+    h->symmHint = symm_hint != 0;
+    h->nnzInput = rowPtr[nrow];
+    // RACE_MAX_RECURSION_STAGES=0 lets a POWER graph skip symmetrisation
+    // (src/graph_SoA.cpp:441-460)
+    int maxRec = 10;
+    if (dist == DIST_POWER) {
+        std::vector<int> v = envIntList("RACE_MAX_RECURSION_STAGES");
+        if (!v.empty()) maxRec = v[0];
+    }
+    bool changed = false;
+    if (maxRec > 0 && !h->symmHint) symmetrizePattern(nrow, rowPtr, col, h->rowPtr, h->col, &changed);
+    if (!changed) {
+        h->rowPtr.assign(rowPtr, rowPtr + nrow + 1);
+        h->col.assign(col, col + rowPtr[nrow]);
+    }
+    if (init_perm) {
+        h->initPerm.assign(init_perm, init_perm + nrow);
+        h->initInv.assign(init_inv_perm, init_inv_perm + nrow);
+    }
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) return ERR_INTERNAL;
+    *out = h.release();
+    return OK;
+}
+
+int race_destroy(race_handle h) {
+    if (!h) return ERR_INVALID_ARG;
+    hipStream_t s = h->stream;
+    if (s) (void)hipStreamSynchronize(s);
+    delete h;
+    if (s) (void)hipStreamDestroy(s);
+    return OK;
+}
+
+static int ensureTraverser(race_handle h) {
+    if (h->trav) return OK;
+    h->trav = std::make_unique<GpuTraverser>();
+    return h->trav->init(h->nrow, h->rowPtr.data(), h->col.data(),
+                         h->initPerm.empty() ? nullptr : h->initPerm.data(),
+                         h->initInv.empty() ? nullptr : h->initInv.data(), h->stream);
+}
+
+int race_color(race_handle h) {
+    if (checkHandle(h)) return ERR_INVALID_ARG;
+    if (h->dist == DIST_POWER) return ERR_INVALID_ARG;  // src/interface.cpp:263-267
+    int e = ensureTraverser(h);
+    if (e) return e;
+    std::vector<int> effI = envIntList("RACE_EFFICIENCY");
+    std::vector<double> eff(effI.begin(), effI.end());
+    std::vector<int> lvlThreads = envIntList("RACE_THREADS");
+    h->tree = std::make_unique<ZoneTree>(h->dist, h->d2, h->lbTarget);
+    LevelRecursion lr(h->trav.get(), h->nrow, h->requested, h->dist, h->d2, h->lbTarget, eff, lvlThreads);
+    e = lr.levelBalancing(h->tree.get());
+    if (e) return e;
+    h->availableThreads = lr.availableThreads;
+    h->perm = h->trav->perm();
+    h->inv = h->trav->invPerm();
+    h->fwd = flattenSchedule(*h->tree, false, h->nrow);
+    h->rev = flattenSchedule(*h->tree, true, h->nrow);
+    h->colored = true;
+    return OK;
+}
+
+int race_color_power(race_handle h, int highest_power, double cache_size, double safety,
+                     char mtx_type, int highest_sub_power) {
+    if (checkHandle(h)) return ERR_INVALID_ARG;
+    if (highest_sub_power < 1 || highest_power < 1) return ERR_INVALID_ARG;
+    if (h->dist != DIST_POWER) return ERR_INVALID_ARG;  // src/interface.cpp:220-224
+    if (mtx_type != 'N') return ERR_NOT_IMPLEMENTED;
+    h->highestPower = highest_power;
+    h->highestSubPower = highest_sub_power;
+    int e = ensureTraverser(h);
+    if (e) return e;
+    // cache-violation cut-off of stage 0 (src/matrixPowerRecursive.cpp:37-74,88-109)
+    std::vector<int> def = envIntList("RACE_CACHE_VIOLATION_CUTOFF_DEFAULT");
+    std::vector<int> cut{def.empty() ? 50 : def[0]};
+    for (int v : envIntList("RACE_CACHE_VIOLATION_CUTOFF")) cut.push_back(v);
+    std::vector<int> mr = envIntList("RACE_MAX_RECURSION_STAGES");
+    int maxRec = mr.empty() ? std::max(0, (int)cut.size()) : mr[0];
+    int cutoff0 = (0 >= maxRec) ? 0x7fffffff : (cut.size() > 1 ? cut[1] : cut[0]);
+    e = mpkPartition(h->trav.get(), h->nrow, h->nnzInput, highest_power * highest_sub_power,
+                     cache_size, safety, cutoff0, &h->mpk);
+    if (e) return e;
+    h->perm = h->trav->perm();
+    h->inv = h->trav->invPerm();
+    if (!h->mpk.hopeless.empty()) {
+        std::fprintf(stderr,
+                     "race_amd: %zu BFS levels exceed the cache by more than the cut-off; "
+                     "recursive (hopeless-region) MPK partitioning is not implemented\n",
+                     h->mpk.hopeless.size());
+        return ERR_NOT_IMPLEMENTED;
+    }
+    h->availableThreads = h->requested;
+    h->colored = true;
+    return OK;
+}
+
+int race_get_perm(race_handle h, int* perm, int len) {
+    if (checkHandle(h) || !perm || len != h->nrow || h->perm.empty()) return ERR_INVALID_ARG;
+    std::memcpy(perm, h->perm.data(), sizeof(int) * len);
+    return OK;
+}
+
+int race_get_inv_perm(race_handle h, int* inv, int len) {
+    if (checkHandle(h) || !inv || len != h->nrow || h->inv.empty()) return ERR_INVALID_ARG;
+    std::memcpy(inv, h->inv.data(), sizeof(int) * len);
+    return OK;
+}
+
+int race_get_num_threads(race_handle h, int* out) {
+    if (checkHandle(h) || !out) return ERR_INVALID_ARG;
+    *out = h->availableThreads;
+    return OK;
+}
+
+int race_get_efficiency(race_handle h, double* out) {
+    if (checkHandle(h) || !out || !h->colored) return ERR_INVALID_ARG;
+    if (h->dist == DIST_POWER) {
+        *out = 1.0;  // single device: every row is a main row
+    } else {
+        // (nrow / effRow) / requested threads (src/interface.cpp:304-321)
+        *out = ((double)h->nrow / (double)h->tree->tree[0].effRowZ) / h->requested;
+    }
+    return OK;
+}
+
+int race_get_max_stage_depth(race_handle h, int* out) {
+    if (checkHandle(h) || !out) return ERR_INVALID_ARG;
+    *out = h->tree ? h->tree->maxStage() : 1;
+    return OK;
+}
+
+int race_get_highest_power(race_handle h, int* out) {
+    if (checkHandle(h) || !out) return ERR_INVALID_ARG;
+    *out = h->highestPower;
+    return OK;
+}
+
+int race_register_function(race_handle h, race_color_fn f, void* args, int* id) {
+    if (checkHandle(h) || !f || !id) return ERR_INVALID_ARG;
+    if (h->dist == DIST_POWER || !h->colored) return ERR_INVALID_ARG;
+    Function fn;
+    fn.host = std::make_unique<HostFn>();
+    fn.host->color = f;
+    fn.host->args = args;
+    h->funcs.push_back(std::move(fn));
+    *id = (int)h->funcs.size() - 1;
+    return OK;
+}
+
+int race_register_power_function(race_handle h, race_power_fn f, void* args, int power,
+                                 int sub_power, int numa_split, int* id) {
+    (void)numa_split;
+    if (checkHandle(h) || !f || !id) return ERR_INVALID_ARG;
+    if (h->dist != DIST_POWER || !h->colored) return ERR_INVALID_ARG;
+    if (power < 1 || sub_power < 1) return ERR_INVALID_ARG;
+    Function fn;
+    fn.host = std::make_unique<HostFn>();
+    fn.host->powerFn = f;
+    fn.host->args = args;
+    fn.host->power = power;
+    fn.host->subPower = sub_power;
+    h->funcs.push_back(std::move(fn));
+    *id = (int)h->funcs.size() - 1;
+    return OK;
+}
+
+int race_register_device(race_handle h, const race_kernel_desc* d, int* id) {
+    if (checkHandle(h) || !d || !id) return ERR_INVALID_ARG;
+    if (d->nrow != h->nrow) return ERR_INVALID_ARG;
+    const bool power = d->kind == RACE_K_MPK;
+    if (power != (h->dist == DIST_POWER) && d->kind != RACE_K_SPMV) return ERR_INVALID_ARG;
+    if (d->kind != RACE_K_SPMV && !h->colored) return ERR_INVALID_ARG;
+    ExecContext ctx;
+    ctx.fwd = &h->fwd;
+    ctx.rev = &h->rev;
+    ctx.mpk = &h->mpk;
+    Function fn;
+    int e = makeDeviceFn(*d, ctx, &fn.dev);
+    if (e) return e;
+    h->funcs.push_back(std::move(fn));
+    *id = (int)h->funcs.size() - 1;
+    return OK;
+}
+
+int race_set_vectors(race_handle h, int id, void* x, void* b) {
+    if (checkHandle(h) || id < 0 || id >= (int)h->funcs.size() || !h->funcs[id].dev)
+        return ERR_INVALID_ARG;
+    h->funcs[id].dev->x = x;
+    h->funcs[id].dev->b = b;
+    return OK;
+}
+
+// Host colouring callbacks run phase by phase; leaves of one phase are
+// independent, so they are called in order on the calling thread.
+static void runHostColor(race_handle h, HostFn& f, bool rev) {
+    const Schedule& s = rev ? h->rev : h->fwd;
+    for (int p = 0; p < s.numPhases(); ++p)
+        for (int l = s.phasePtr[p]; l < s.phasePtr[p + 1]; ++l) {
+            if (rev)
+                f.color(s.leafHi[l] - 1, s.leafLo[l] - 1, f.args);
+            else
+                f.color(s.leafLo[l], s.leafHi[l], f.args);
+        }
+    // serial part (empty unless dense rows were split off)
+    if (rev)
+        f.color(h->nrow, h->nrow, f.args);
+    else
+        f.color(h->nrow, h->nrow, f.args);
+}
+
+// Host power callbacks in the wavefront order of the level groups.
+static void runHostPower(race_handle h, HostFn& f) {
+    const std::vector<int>& g = h->mpk.groupPtr;
+    const int G = (int)g.size() - 1;
+    const int k = f.power * f.subPower;
+    for (int L = 0; L < G + k - 1; ++L)
+        for (int p = 0; p < k; ++p) {
+            const int grp = L - p;
+            if (grp < 0 || grp >= G) continue;
+            const int pw = p / f.subPower + 1, sp = p % f.subPower + 1;
+            f.powerFn(g[grp], g[grp + 1], pw, sp, 0, f.args);
+        }
+}
+
+int race_execute_async(race_handle h, int id, int rev, void* stream) {
+    if (checkHandle(h) || id < 0 || id >= (int)h->funcs.size()) return ERR_INVALID_ARG;
+    Function& fn = h->funcs[id];
+    if (fn.host) {
+        if (fn.host->color)
+            runHostColor(h, *fn.host, rev != 0);
+        else
+            runHostPower(h, *fn.host);
+        return OK;
+    }
+    hipStream_t st = stream ? (hipStream_t)stream : h->stream;
+    return fn.dev->run(rev != 0, st);
+}
+
+int race_execute(race_handle h, int id, int rev) {
+    int e = race_execute_async(h, id, rev, nullptr);
+    if (e) return e;
+    if (hipStreamSynchronize(h->stream) != hipSuccess) return ERR_INTERNAL;
+    return OK;
+}
+
+int race_set_power(race_handle h, int id, int power) {
+    if (checkHandle(h) || id < 0 || id >= (int)h->funcs.size() || power < 1) return ERR_INVALID_ARG;
+    Function& fn = h->funcs[id];
+    if (fn.host)
+        fn.host->power = power;
+    else
+        fn.dev->power = power;
+    return OK;
+}
+
+int race_get_power(race_handle h, int id, int* power) {
+    if (checkHandle(h) || id < 0 || id >= (int)h->funcs.size() || !power) return ERR_INVALID_ARG;
+    Function& fn = h->funcs[id];
+    *power = fn.host ? fn.host->power : fn.dev->power;
+    return OK;
+}
+
+// time per power for p = 1..highestPower, keep the fastest
+// (Interface::tuneFunction src/interface.cpp:605-647)
+int race_tune_function(race_handle h, int id, int rev, int* best_power) {
+    if (checkHandle(h) || id < 0 || id >= (int)h->funcs.size() || !best_power) return ERR_INVALID_ARG;
+    double bestT = 1e300;
+    int best = 1;
+    for (int p = 1; p <= h->highestPower; ++p) {
+        race_set_power(h, id, p);
+        auto t0 = std::chrono::steady_clock::now();
+        double t = 0;
+        int it = 0;
+        while (t < 0.2) {
+            int e = race_execute(h, id, rev);
+            if (e) return e;
+            t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            ++it;
+        }
+        double per = t / (it * (double)p);
+        if (per < bestT) {
+            bestT = per;
+            best = p;
+        }
+    }
+    race_set_power(h, id, best);
+    *best_power = best;
+    return OK;
+}
+
+static void coverageFn(int start, int end, int pow, int, int, void* args) {
+    std::vector<int>* cnt = (std::vector<int>*)args;
+    for (int r = start; r < end; ++r) (*cnt)[r] += 1;
+    (void)pow;
+}
+
+int race_check_power_coverage(race_handle h) {
+    if (checkHandle(h) || h->dist != DIST_POWER || !h->colored) return ERR_INVALID_ARG;
+    std::vector<int> cnt(h->nrow, 0);
+    HostFn f;
+    f.powerFn = coverageFn;
+    f.args = &cnt;
+    f.power = h->highestPower;
+    runHostPower(h, f);
+    for (int r = 0; r < h->nrow; ++r)
+        if (cnt[r] != h->highestPower) return ERR_INTERNAL;
+    return OK;
+}
+
+int race_get_schedule(race_handle h, int rev, int* nphases, int* nleaves, int* phase_ptr,
+                      int* leaf_lo, int* leaf_hi, int cap) {
+    if (checkHandle(h) || !h->colored || h->dist == DIST_POWER) return ERR_INVALID_ARG;
+    const Schedule& s = rev ? h->rev : h->fwd;
+    if (nphases) *nphases = s.numPhases();
+    if (nleaves) *nleaves = (int)s.leafLo.size();
+    if (phase_ptr && (int)s.phasePtr.size() <= cap)
+        std::memcpy(phase_ptr, s.phasePtr.data(), sizeof(int) * s.phasePtr.size());
+    if (leaf_lo && leaf_hi && (int)s.leafLo.size() <= cap) {
+        std::memcpy(leaf_lo, s.leafLo.data(), sizeof(int) * s.leafLo.size());
+        std::memcpy(leaf_hi, s.leafHi.data(), sizeof(int) * s.leafHi.size());
+    }
+    return OK;
+}
+
+int race_get_tree(race_handle h, int* nleaves, int* data6, int cap) {
+    if (checkHandle(h) || !h->tree || !nleaves) return ERR_INVALID_ARG;
+    const auto& T = h->tree->tree;
+    *nleaves = (int)T.size();
+    if (data6 && (int)T.size() <= cap)
+        for (size_t i = 0; i < T.size(); ++i) {
+            data6[6 * i + 0] = T[i].valueZ.front();
+            data6[6 * i + 1] = T[i].valueZ.back();
+            data6[6 * i + 2] = T[i].parentZ;
+            data6[6 * i + 3] = T[i].nthreadsZ;
+            data6[6 * i + 4] = T[i].stage;
+            data6[6 * i + 5] = T[i].effRowZ;
+        }
+    return OK;
+}
+
+int race_get_mpk_groups(race_handle h, int* ngroups, int* group_ptr, int cap) {
+    if (checkHandle(h) || h->dist != DIST_POWER || !h->colored || !ngroups) return ERR_INVALID_ARG;
+    *ngroups = (int)h->mpk.groupPtr.size() - 1;
+    if (group_ptr && (int)h->mpk.groupPtr.size() <= cap)
+        std::memcpy(group_ptr, h->mpk.groupPtr.data(), sizeof(int) * h->mpk.groupPtr.size());
+    return OK;
+}
+
+int race_get_wave_count(race_handle h, int id, int rev, int* waves) {
+    if (checkHandle(h) || id < 0 || id >= (int)h->funcs.size() || !waves || !h->funcs[id].dev)
+        return ERR_INVALID_ARG;
+    *waves = h->funcs[id].dev->waves(rev != 0);
+    return OK;
+}
+
+int race_timer_begin(race_handle h) {
+    if (checkHandle(h)) return ERR_INVALID_ARG;
+    if (!h->ev0 && hipEventCreate(&h->ev0) != hipSuccess) return ERR_INTERNAL;
+    if (!h->ev1 && hipEventCreate(&h->ev1) != hipSuccess) return ERR_INTERNAL;
+    return hipEventRecord(h->ev0, h->stream) == hipSuccess ? OK : ERR_INTERNAL;
+}
+
+int race_timer_end(race_handle h, float* ms) {
+    if (checkHandle(h) || !ms || !h->ev0) return ERR_INVALID_ARG;
+    if (hipEventRecord(h->ev1, h->stream) != hipSuccess) return ERR_INTERNAL;
+    if (hipEventSynchronize(h->ev1) != hipSuccess) return ERR_INTERNAL;
+    return hipEventElapsedTime(ms, h->ev0, h->ev1) == hipSuccess ? OK : ERR_INTERNAL;
+}
+
+int race_get_preprocess_stats(race_handle h, double* bfs_ms, int* traversals) {
+    if (checkHandle(h) || !h->trav) return ERR_INVALID_ARG;
+    if (bfs_ms) *bfs_ms = h->trav->lastTraverseMs;
+    if (traversals) *traversals = (int)h->trav->traversals;
+    return OK;
+}
+
+int race_lb_split(const int* level_row, const int* level_nnz, int nlevels, int nthreads,
+                  double efficiency, int dist, int d2_method, int lb_target, int cap, int* nblocks,
+                  int* zone_rows, int* level_ptr, int* nthreads_out, int* scale) {
+    if (!level_row || !level_nnz || nlevels <= 0 || nthreads <= 0) return ERR_INVALID_ARG;
+    LevelData ld;
+    ld.levelRow.assign(level_row, level_row + nlevels);
+    ld.levelNnz.assign(level_nnz, level_nnz + nlevels);
+    LBResult r = splitZones(ld, nthreads, efficiency, dist, d2_method, lb_target);
+    if (nblocks) *nblocks = (int)r.zoneRows.size();
+    if (nthreads_out) *nthreads_out = r.threads();
+    if ((int)r.levelPtr.size() > cap) return ERR_INVALID_ARG;
+    if (zone_rows) std::memcpy(zone_rows, r.zoneRows.data(), sizeof(int) * r.zoneRows.size());
+    if (level_ptr) std::memcpy(level_ptr, r.levelPtr.data(), sizeof(int) * r.levelPtr.size());
+    if (scale) std::memcpy(scale, r.scale.data(), sizeof(int) * r.scale.size());
+    return OK;
+}
+
+int race_device_malloc(void** p, size_t bytes) {
+    if (!p) return ERR_INVALID_ARG;
+    return hipMalloc(p, bytes ? bytes : 1) == hipSuccess ? OK : ERR_INTERNAL;
+}
+
+int race_device_free(void* p) { return hipFree(p) == hipSuccess ? OK : ERR_INTERNAL; }
+
+int race_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+    return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess ? OK : ERR_INTERNAL;
+}
+
+int race_memcpy_d2h(void* dst, const void* src, size_t bytes) {
+    return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) == hipSuccess ? OK : ERR_INTERNAL;
+}
+
+int race_memset_device(void* dst, int value, size_t bytes) {
+    return hipMemset(dst, value, bytes) == hipSuccess ? OK : ERR_INTERNAL;
+}
+
+int race_device_synchronize(void) { return hipDeviceSynchronize() == hipSuccess ? OK : ERR_INTERNAL; }
+
+int race_device_count(int* count) {
+    if (!count) return ERR_INVALID_ARG;
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+    *count = c;
+    return OK;
+}
+
+}  // extern "C"

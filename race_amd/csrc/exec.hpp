@@ -1,0 +1,47 @@
+// exec.hpp -- colour-phase launcher and device kernel functions.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <vector>
+
+#include "../../include/race_amd.h"
+#include "engine.hpp"
+#include "hip_util.hpp"
+
+namespace race {
+
+// Dependency waves inside the leaves of a schedule.  RACE runs the rows of a
+// leaf sequentially; rows of one wave have no dependency on each other, and
+// running the waves in order reproduces the sequential result exactly.
+struct WaveTable {
+    std::vector<int> phaseLeafPtr;  // phase -> [first leaf, last leaf)
+    std::vector<int> leafWavePtr;   // leaf  -> [first wave, last wave)
+    std::vector<int> wavePtr;       // wave  -> [first row, last row) in rows
+    std::vector<int> rows;          // row ids, wave by wave
+    long critical = 0;              // sum over phases of the deepest leaf
+};
+// kind: RACE_K_GS (row writes its own x, reads its neighbours) or
+// RACE_K_KACZ (row reads and writes x at all its columns)
+WaveTable buildWaves(const Schedule& s, int kind, int n, const int* rowPtr, const int* col);
+
+class DeviceFn {
+  public:
+    virtual ~DeviceFn() {}
+    virtual int run(bool rev, hipStream_t st) = 0;
+    virtual int waves(bool /*rev*/) const { return 0; }
+    void* x = nullptr;
+    void* b = nullptr;
+    int power = 1;
+    int kind = 0;
+};
+
+struct ExecContext {
+    const Schedule* fwd = nullptr;
+    const Schedule* rev = nullptr;
+    const MPKPartition* mpk = nullptr;
+};
+
+int makeDeviceFn(const race_kernel_desc& d, const ExecContext& ctx, std::unique_ptr<DeviceFn>* out);
+
+}  // namespace race

@@ -1,0 +1,264 @@
+"""Python mirror of RACE::Interface over the C ABI (include/race_amd.h).
+
+Method names follow the reference (include/interface.h:54-205) so tests read
+like the reference's drivers:
+
+    ce = Interface(nrow, nthreads, TWO, rowPtr, col, symm_hint=True)
+    ce.RACEColor()
+    perm, inv = ce.getPerm(), ce.getInvPerm()
+    fid = ce.registerDevice(K_SYMMSPMV, rowPtr_u, col_u, val_u, x_dev, b_dev)
+    ce.executeFunction(fid)
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import (BF16, FP32, FP64, K_GS, K_KACZ, K_MPK, K_SPMV, K_SYMMSPMV, LB_NNZ, LB_ROW, ONE,
+                   POWER, SCATTER, TWO, TWO_BLOCK, RaceError)
+
+__all__ = ["Interface", "DeviceArray", "lb_split", "RaceError", "ONE", "TWO", "POWER", "K_SYMMSPMV",
+           "K_GS", "K_KACZ", "K_SPMV", "K_MPK", "FP64", "FP32", "BF16", "LB_NNZ", "LB_ROW",
+           "TWO_BLOCK", "device_count"]
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _ip(a):
+    return a.ctypes.data_as(L.i32p) if a is not None else None
+
+
+def device_count():
+    c = ctypes.c_int(0)
+    L.load().race_device_count(ctypes.byref(c))
+    return c.value
+
+
+class DeviceArray:
+    """Device allocation made through the C ABI (no torch needed)."""
+
+    def __init__(self, n, dtype=np.float64):
+        self.n = int(n)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = self.n * self.dtype.itemsize
+        p = ctypes.c_void_p()
+        L.check(L.load().race_device_malloc(ctypes.byref(p), self.nbytes), "race_device_malloc")
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, a, dtype=None):
+        a = np.ascontiguousarray(a, dtype=dtype or np.asarray(a).dtype)
+        d = cls(a.size, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes <= self.nbytes
+        L.check(L.load().race_memcpy_h2d(self.ptr, a.ctypes.data, a.nbytes), "race_memcpy_h2d")
+
+    def download(self):
+        out = np.empty(self.n, self.dtype)
+        L.check(L.load().race_memcpy_d2h(out.ctypes.data, self.ptr, self.nbytes), "race_memcpy_d2h")
+        return out
+
+    def zero(self):
+        L.check(L.load().race_memset_device(self.ptr, 0, self.nbytes), "race_memset_device")
+
+    def free(self):
+        if self.ptr:
+            L.load().race_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Interface:
+    """RACE::Interface(nrow, nthreads, dist, rowPtr, col, symm_hint, SMT,
+    pinMethod, initPerm, initInvPerm, d2Type, lbTarget)."""
+
+    def __init__(self, nrow, nthreads, dist, rowPtr, col, symm_hint=False, SMT=1, pinMethod=SCATTER,
+                 initPerm=None, initInvPerm=None, d2Type=TWO_BLOCK, lbTarget=LB_NNZ):
+        self._L = L.load()
+        self.nrow = int(nrow)
+        self._keep = []
+        rp, c = _i32(rowPtr), _i32(col)
+        ip = _i32(initPerm) if initPerm is not None else None
+        iip = _i32(initInvPerm) if initInvPerm is not None else None
+        h = ctypes.c_void_p()
+        L.check(self._L.race_create(ctypes.byref(h), self.nrow, int(nthreads), int(dist), _ip(rp), _ip(c),
+                                    int(bool(symm_hint)), int(SMT), int(pinMethod), _ip(ip), _ip(iip),
+                                    int(d2Type), int(lbTarget)), "race_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._L.race_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- preprocessing ----
+    def RACEColor(self, highestPower=None, cacheSize=None, safetyFactor=2.0, mtxType="N",
+                  highestSubPower=1):
+        if highestPower is None:
+            L.check(self._L.race_color(self.h), "RACEColor")
+        else:
+            L.check(self._L.race_color_power(self.h, int(highestPower), float(cacheSize),
+                                             float(safetyFactor), mtxType.encode()[:1],
+                                             int(highestSubPower)), "RACEColor(power)")
+
+    def getPerm(self):
+        out = np.empty(self.nrow, np.int32)
+        L.check(self._L.race_get_perm(self.h, _ip(out), self.nrow), "getPerm")
+        return out
+
+    def getInvPerm(self):
+        out = np.empty(self.nrow, np.int32)
+        L.check(self._L.race_get_inv_perm(self.h, _ip(out), self.nrow), "getInvPerm")
+        return out
+
+    def getNumThreads(self):
+        v = ctypes.c_int()
+        L.check(self._L.race_get_num_threads(self.h, ctypes.byref(v)), "getNumThreads")
+        return v.value
+
+    def getEfficiency(self):
+        v = ctypes.c_double()
+        L.check(self._L.race_get_efficiency(self.h, ctypes.byref(v)), "getEfficiency")
+        return v.value
+
+    def getMaxStageDepth(self):
+        v = ctypes.c_int()
+        L.check(self._L.race_get_max_stage_depth(self.h, ctypes.byref(v)), "getMaxStageDepth")
+        return v.value
+
+    def getHighestPower(self):
+        v = ctypes.c_int()
+        L.check(self._L.race_get_highest_power(self.h, ctypes.byref(v)), "getHighestPower")
+        return v.value
+
+    # ---- registration / execution ----
+    def registerFunction(self, f, args=None, power=None, subPower=1, numaSplit=0):
+        """Host callback: f(start, end, args) for colouring, or
+        f(start, end, pow, subPow, numa, args) for POWER."""
+        fid = ctypes.c_int()
+        if power is None:
+            cb = L.COLOR_FN(lambda s, e, a: f(s, e, args))
+            L.check(self._L.race_register_function(self.h, cb, None, ctypes.byref(fid)), "registerFunction")
+        else:
+            cb = L.POWER_FN(lambda s, e, p, sp, nu, a: f(s, e, p, sp, nu, args))
+            L.check(self._L.race_register_power_function(self.h, cb, None, int(power), int(subPower),
+                                                         int(numaSplit), ctypes.byref(fid)),
+                    "registerFunction(power)")
+        self._keep.append(cb)
+        return fid.value
+
+    def registerDevice(self, kind, rowPtr, col, val, x, b=None, nrhs=1, power=1,
+                       val_precision=FP64, vec_precision=FP64):
+        rp, c = _i32(rowPtr), _i32(col)
+        v = np.ascontiguousarray(val, dtype=np.float64)
+        d = L.KernelDesc(kind=int(kind), nrow=self.nrow, rowPtr=_ip(rp), col=_ip(c),
+                         val=v.ctypes.data_as(L.f64p), val_precision=int(val_precision),
+                         vec_precision=int(vec_precision), nrhs=int(nrhs),
+                         x=x.ptr if isinstance(x, DeviceArray) else x,
+                         b=(b.ptr if isinstance(b, DeviceArray) else b), power=int(power))
+        fid = ctypes.c_int()
+        L.check(self._L.race_register_device(self.h, ctypes.byref(d), ctypes.byref(fid)), "registerDevice")
+        return fid.value
+
+    def setVectors(self, fid, x, b=None):
+        L.check(self._L.race_set_vectors(self.h, fid, x.ptr if isinstance(x, DeviceArray) else x,
+                                         b.ptr if isinstance(b, DeviceArray) else b), "setVectors")
+
+    def executeFunction(self, fid, rev=False):
+        L.check(self._L.race_execute(self.h, int(fid), int(bool(rev))), "executeFunction")
+
+    def executeAsync(self, fid, rev=False, stream=None):
+        L.check(self._L.race_execute_async(self.h, int(fid), int(bool(rev)), stream), "executeAsync")
+
+    def setPower(self, fid, p):
+        L.check(self._L.race_set_power(self.h, fid, int(p)), "setPower")
+
+    def getPower(self, fid):
+        v = ctypes.c_int()
+        L.check(self._L.race_get_power(self.h, fid, ctypes.byref(v)), "getPower")
+        return v.value
+
+    def tuneFunction(self, fid, rev=False):
+        v = ctypes.c_int()
+        L.check(self._L.race_tune_function(self.h, fid, int(bool(rev)), ctypes.byref(v)), "tuneFunction")
+        return v.value
+
+    def checkPowerCoverage(self):
+        return self._L.race_check_power_coverage(self.h) == L.SUCCESS
+
+    # ---- introspection ----
+    def schedule(self, rev=False):
+        nph, nl = ctypes.c_int(), ctypes.c_int()
+        L.check(self._L.race_get_schedule(self.h, int(rev), ctypes.byref(nph), ctypes.byref(nl), None,
+                                          None, None, 0), "schedule")
+        cap = max(nph.value + 1, nl.value)
+        pp = np.empty(nph.value + 1, np.int32)
+        lo = np.empty(nl.value, np.int32)
+        hi = np.empty(nl.value, np.int32)
+        L.check(self._L.race_get_schedule(self.h, int(rev), ctypes.byref(nph), ctypes.byref(nl), _ip(pp),
+                                          _ip(lo), _ip(hi), cap), "schedule")
+        return pp, lo, hi
+
+    def tree(self):
+        n = ctypes.c_int()
+        L.check(self._L.race_get_tree(self.h, ctypes.byref(n), None, 0), "tree")
+        d = np.empty(6 * n.value, np.int32)
+        L.check(self._L.race_get_tree(self.h, ctypes.byref(n), _ip(d), n.value), "tree")
+        return d.reshape(n.value, 6)
+
+    def mpkGroups(self):
+        n = ctypes.c_int()
+        L.check(self._L.race_get_mpk_groups(self.h, ctypes.byref(n), None, 0), "mpkGroups")
+        g = np.empty(n.value + 1, np.int32)
+        L.check(self._L.race_get_mpk_groups(self.h, ctypes.byref(n), _ip(g), n.value + 1), "mpkGroups")
+        return g
+
+    def waveCount(self, fid, rev=False):
+        v = ctypes.c_int()
+        L.check(self._L.race_get_wave_count(self.h, fid, int(rev), ctypes.byref(v)), "waveCount")
+        return v.value
+
+    def timerBegin(self):
+        L.check(self._L.race_timer_begin(self.h), "timerBegin")
+
+    def timerEnd(self):
+        v = ctypes.c_float()
+        L.check(self._L.race_timer_end(self.h, ctypes.byref(v)), "timerEnd")
+        return float(v.value)
+
+    def preprocessStats(self):
+        ms, n = ctypes.c_double(), ctypes.c_int()
+        L.check(self._L.race_get_preprocess_stats(self.h, ctypes.byref(ms), ctypes.byref(n)), "stats")
+        return ms.value, n.value
+
+
+def lb_split(levelRow, levelNnz, nthreads, eff, dist, d2=TWO_BLOCK, lb_target=LB_NNZ):
+    """LB::splitZones on a level histogram through the C ABI (host only)."""
+    lr, ln = _i32(levelRow), _i32(levelNnz)
+    cap = 4 * len(lr) + 8
+    nb, nt = ctypes.c_int(), ctypes.c_int()
+    zr = np.zeros(cap, np.int32)
+    lp = np.zeros(cap, np.int32)
+    sc = np.zeros(cap, np.int32)
+    L.check(L.load().race_lb_split(_ip(lr), _ip(ln), len(lr), int(nthreads), float(eff), int(dist), int(d2),
+                                   int(lb_target), cap, ctypes.byref(nb), _ip(zr), _ip(lp), ctypes.byref(nt),
+                                   _ip(sc)), "race_lb_split")
+    return dict(zone_rows=zr[:nb.value].tolist(), levelPtr=lp[:nb.value + 1].tolist(),
+                scale=sc[:nt.value].tolist(), threads=nt.value)

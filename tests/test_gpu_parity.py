@@ -1,0 +1,205 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle.
+
+Integer schedule data (levels, permutations, zone trees, phases, MPK groups)
+must be bit-exact; FP64 kernels must agree to <= 1e-12 normwise relative
+error (summation order differs: atomics, lane-group reductions, FMA).
+"""
+import numpy as np
+import pytest
+
+import race_amd as R
+from race_amd import matgen as M
+
+from .conftest import relerr
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def graphs():
+    """(name, rowPtr, col) of small test graphs."""
+    out = []
+    rp, c = M.stencil7(12)
+    out.append(("7pt-12", rp, c))
+    rp, c = M.stencil27(10)
+    out.append(("27pt-10", rp, c))
+    # two disconnected 7-point blocks (island restart) with a dangling row
+    rp1, c1 = M.stencil7(6)
+    n1 = len(rp1) - 1
+    rp2 = np.concatenate([rp1, rp1[1:] + rp1[-1], [2 * rp1[-1] + 1]]).astype(np.int32)
+    c2 = np.concatenate([c1, c1 + n1, [2 * n1]]).astype(np.int32)
+    out.append(("islands", rp2, c2))
+    return out
+
+
+def _iface(rp, col, T, dist, symm=True):
+    return R.Interface(len(rp) - 1, T, dist, rp, col, symm_hint=symm)
+
+
+@pytest.mark.parametrize("dist", [R.ONE, R.TWO])
+def test_stage0_perm_matches_oracle(oracle, dist):
+    for name, rp, col in graphs():
+        print("case", name, dist, flush=True)
+        ce = _iface(rp, col, 1, dist)
+        ce.RACEColor()
+        g = oracle.Graph(rp, col, symm_hint=True)
+        g.traverse(dist, 0, g.n)
+        assert np.array_equal(ce.getPerm(), g.perm), name
+        assert np.array_equal(ce.getInvPerm(), g.inv), name
+
+
+def test_spin_symmetrized_perm(oracle, spin):
+    rp, col, _ = spin
+    for dist in (R.ONE, R.TWO):
+        ce = R.Interface(len(rp) - 1, 1, dist, rp, col, symm_hint=False)
+        ce.RACEColor()
+        g = oracle.Graph(rp, col, symm_hint=False)
+        g.traverse(dist, 0, g.n)
+        assert np.array_equal(ce.getPerm(), g.perm)
+
+
+@pytest.mark.parametrize("T", [2, 3, 8, 20, 64])
+@pytest.mark.parametrize("dist", [R.ONE, R.TWO])
+def test_recursion_tree_and_schedule(oracle, T, dist):
+    for name, rp, col in graphs()[:2]:
+        print("case", name, T, dist, flush=True)
+        ce = _iface(rp, col, T, dist)
+        ce.RACEColor()
+        g = oracle.Graph(rp, col, symm_hint=True)
+        zt = oracle.level_recursion(g, T, dist)
+        assert np.array_equal(ce.getPerm(), g.perm), name
+        assert ce.getNumThreads() == zt.available_threads
+        tr = ce.tree()
+        ref = np.array([[l.valueZ[0], l.valueZ[-1], l.parentZ, l.nthreadsZ, l.stage, l.effRowZ]
+                        for l in zt.tree], np.int32)
+        assert np.array_equal(tr, ref), name
+        for rev in (False, True):
+            pp, lo, hi = ce.schedule(rev)
+            ph = [[(a, b) for (a, b) in p if (b > a if not rev else a > b)] for p in oracle.phases(zt, rev)]
+            ph = [p for p in ph if p]
+            assert len(pp) - 1 == len(ph)
+            for k, p in enumerate(ph):
+                mine = list(zip(lo[pp[k]:pp[k + 1]], hi[pp[k]:pp[k + 1]]))
+                want = [(b + 1, a + 1) if rev else (a, b) for (a, b) in p]
+                assert mine == want, (name, rev, k)
+
+
+def _prepared(rp, col, val, T, dist, oracle):
+    """colour, permute (diagonal first), return the interface and oracle tree."""
+    n = len(rp) - 1
+    ce = R.Interface(n, T, dist, rp, col, symm_hint=True)
+    ce.RACEColor()
+    perm, inv = ce.getPerm(), ce.getInvPerm()
+    prp, pcol, pval = M.permute(rp, col, val, perm, inv)
+    prp, pcol, pval = M.make_diag_first(prp, pcol, pval)
+    g = oracle.Graph(rp, col, symm_hint=True)
+    zt = oracle.level_recursion(g, T, dist)
+    assert np.array_equal(g.perm, perm)
+    return ce, (prp, pcol, pval), zt
+
+
+@pytest.mark.parametrize("T", [1, 4, 16, 64])
+def test_gs_matches_race_order(oracle, T):
+    rp, col = M.stencil27(12)
+    val = M.gs_values(rp, col, seed=7)
+    n = len(rp) - 1
+    ce, (prp, pcol, pval), zt = _prepared(rp, col, val, T, R.ONE, oracle)
+    b = M.uniform_vec(n, 11)
+    x0 = M.uniform_vec(n, 12)
+    dx = R.DeviceArray.from_host(x0)
+    db = R.DeviceArray.from_host(b)
+    fid = ce.registerDevice(R.K_GS, prp, pcol, pval, dx, db)
+    xo = x0.copy()
+    for rev in (False, True, False):
+        print("sweep rev", rev, "T", T, flush=True)
+        ce.executeFunction(fid, rev)
+        oracle.run_calls(oracle.K_GS, prp, pcol, pval, xo, b, oracle.leaf_calls(zt, rev))
+        assert relerr(dx.download(), xo) <= TOL
+
+
+@pytest.mark.parametrize("T", [1, 8, 32])
+def test_kacz_matches_race_order(oracle, T):
+    rp, col = M.stencil7(14)
+    val = M.jittered_values(rp, col, 0.1, seed=3)
+    n = len(rp) - 1
+    ce, (prp, pcol, pval), zt = _prepared(rp, col, val, T, R.TWO, oracle)
+    b = M.uniform_vec(n, 5)
+    dx = R.DeviceArray.from_host(np.zeros(n))
+    db = R.DeviceArray.from_host(b)
+    fid = ce.registerDevice(R.K_KACZ, prp, pcol, pval, dx, db)
+    xo = np.zeros(n)
+    for rev in (False, True):
+        ce.executeFunction(fid, rev)
+        oracle.run_calls(oracle.K_KACZ, prp, pcol, pval, xo, b, oracle.leaf_calls(zt, rev))
+        assert relerr(dx.download(), xo) <= TOL
+
+
+@pytest.mark.parametrize("T", [1, 8, 64])
+def test_symmspmv(oracle, T):
+    rp, col = M.stencil7(16)
+    val = M.laplace7_values(rp, col)
+    n = len(rp) - 1
+    ce, (prp, pcol, pval), zt = _prepared(rp, col, val, T, R.TWO, oracle)
+    urp, ucol, uval = M.upper_triangle(prp, pcol, pval)
+    x = M.uniform_vec(n, 1234)
+    dx = R.DeviceArray.from_host(x)
+    db = R.DeviceArray.from_host(np.zeros(n))
+    fid = ce.registerDevice(R.K_SYMMSPMV, urp, ucol, uval, dx, db)
+    ce.executeFunction(fid)
+    bo = np.zeros(n)
+    oracle.run_calls(oracle.K_SYMMSPMV, urp, ucol, uval, x.copy(), bo, oracle.leaf_calls(zt))
+    full = np.zeros(n)
+    oracle.run_calls(oracle.K_SPMV, prp, pcol, pval, x.copy(), full, [(0, n)])
+    got = db.download()
+    assert relerr(got, bo) <= TOL
+    assert relerr(got, full) <= TOL
+
+
+def test_mpk_groups_and_powers(oracle):
+    rp, col = M.stencil7(16)
+    val = M.jittered_values(rp, col, 0.1, seed=9)
+    n = len(rp) - 1
+    k = 4
+    cache = 64 * 1024.0  # small cache so there are many level groups
+    ce = R.Interface(n, 1, R.POWER, rp, col, symm_hint=True)
+    ce.RACEColor(k, cache)
+    g = oracle.Graph(rp, col, symm_hint=True)
+    part = oracle.mpk_partition(g, k, cache)
+    assert np.array_equal(ce.getPerm(), g.perm)
+    assert np.array_equal(ce.mpkGroups(), part["groupPtr"])
+    assert len(part["groupPtr"]) > 3
+    assert ce.checkPowerCoverage()
+    perm, inv = ce.getPerm(), ce.getInvPerm()
+    prp, pcol, pval = M.permute(rp, col, val, perm, inv)
+    x0 = M.uniform_vec(n, 2)
+    xs = np.zeros((k + 1) * n)
+    xs[:n] = x0
+    dx = R.DeviceArray.from_host(xs)
+    fid = ce.registerDevice(R.K_MPK, prp, pcol, pval, dx, power=k)
+    ce.executeFunction(fid)
+    got = dx.download().reshape(k + 1, n)
+    ref = oracle.mpk_serial(prp, pcol, pval, x0, k)
+    for p in range(1, k + 1):
+        assert relerr(got[p], ref[p]) <= TOL, p
+
+
+def test_host_callback_runs_race_order(oracle):
+    """registerFunction with a CPU callback: GS through the host executor."""
+    rp, col = M.stencil27(8)
+    val = M.gs_values(rp, col, seed=1)
+    n = len(rp) - 1
+    ce, (prp, pcol, pval), zt = _prepared(rp, col, val, 8, R.ONE, oracle)
+    b = M.uniform_vec(n, 3)
+    x = np.zeros(n)
+    calls = []
+
+    def gs_cb(s, e, args):
+        calls.append((s, e))
+        oracle.run_calls(oracle.K_GS, prp, pcol, pval, x, b, [(s, e)])
+
+    fid = ce.registerFunction(gs_cb)
+    ce.executeFunction(fid)
+    xo = np.zeros(n)
+    oracle.run_calls(oracle.K_GS, prp, pcol, pval, xo, b, oracle.leaf_calls(zt))
+    assert relerr(x, xo) == 0.0
