@@ -110,6 +110,17 @@ typedef struct {
 /* Typed device registration: the GPU counterpart of registerFunction; the
  * matrix is uploaded and laid out once, the vectors stay caller-owned. */
 int race_register_device(race_handle h, const race_kernel_desc *desc, int *id);
+/* Layout options, applied to functions registered afterwards:
+ *   "layout"       0 = leaf bundles (execution-order SELL, LDS-resident waves),
+ *                  1 = one workgroup per leaf (CRS),
+ *                  2 = leaf bundles only (registration fails with
+ *                      RACE_ERR_INCOMPATIBILITY if a leaf does not fit)
+ *   "bundle_rows"  rows per bundle (default 512)
+ *   "bundle_inner" in-bundle couplings allowed per bundle (default unlimited)
+ *   "bundle_threads" workgroup size of the bundle kernels: 64, 128 or 256
+ *   "bundle_tpr"   lanes per matrix row in the bundle kernels: 1, 2, 4 (0 = auto)
+ * No counterpart in the reference (its executor has no layout choice). */
+int race_set_option(race_handle h, const char *key, int value);
 /* re-point a registered device function at other vectors */
 int race_set_vectors(race_handle h, int id, void *x, void *b);
 

@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "../../include/race_amd.h"
@@ -87,6 +88,7 @@ struct race_handle_s {
     std::vector<int> perm, inv;
     std::vector<Function> funcs;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int optLayout = 0, optBundleRows = 512, optBundleInner = 1 << 30, optBundleThreads = 256, optBundleTpr = 0;
     ~race_handle_s() {
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
@@ -304,11 +306,40 @@ int race_register_device(race_handle h, const race_kernel_desc* d, int* id) {
     ctx.fwd = &h->fwd;
     ctx.rev = &h->rev;
     ctx.mpk = &h->mpk;
+    ctx.layout = h->optLayout;
+    ctx.bundleRows = h->optBundleRows;
+    ctx.bundleInner = h->optBundleInner;
+    ctx.bundleThreads = h->optBundleThreads;
+    ctx.bundleTpr = h->optBundleTpr;
     Function fn;
     int e = makeDeviceFn(*d, ctx, &fn.dev);
     if (e) return e;
     h->funcs.push_back(std::move(fn));
     *id = (int)h->funcs.size() - 1;
+    return OK;
+}
+
+int race_set_option(race_handle h, const char* key, int value) {
+    if (checkHandle(h) || !key) return ERR_INVALID_ARG;
+    const std::string k(key);
+    if (k == "layout") {
+        if (value < 0 || value > 2) return ERR_INVALID_ARG;
+        h->optLayout = value;
+    } else if (k == "bundle_rows") {
+        if (value < 1 || value > 8192) return ERR_INVALID_ARG;
+        h->optBundleRows = value;
+    } else if (k == "bundle_threads") {
+        if (value != 64 && value != 128 && value != 256) return ERR_INVALID_ARG;
+        h->optBundleThreads = value;
+    } else if (k == "bundle_tpr") {
+        if (value != 0 && value != 1 && value != 2 && value != 4) return ERR_INVALID_ARG;
+        h->optBundleTpr = value;
+    } else if (k == "bundle_inner") {
+        if (value < 1) return ERR_INVALID_ARG;
+        h->optBundleInner = value;
+    } else {
+        return ERR_INVALID_ARG;
+    }
     return OK;
 }
 

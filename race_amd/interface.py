@@ -177,6 +177,9 @@ class Interface:
         L.check(self._L.race_register_device(self.h, ctypes.byref(d), ctypes.byref(fid)), "registerDevice")
         return fid.value
 
+    def setOption(self, key, value):
+        L.check(self._L.race_set_option(self.h, key.encode(), int(value)), f"setOption({key})")
+
     def setVectors(self, fid, x, b=None):
         L.check(self._L.race_set_vectors(self.h, fid, x.ptr if isinstance(x, DeviceArray) else x,
                                          b.ptr if isinstance(b, DeviceArray) else b), "setVectors")

@@ -99,8 +99,8 @@ def _prepared(rp, col, val, T, dist, oracle):
     return ce, (prp, pcol, pval), zt
 
 
-@pytest.mark.parametrize("T", [1, 4, 16, 64])
-def test_gs_matches_race_order(oracle, T):
+@pytest.mark.parametrize("T,layout", [(1, 1), (4, 1), (16, 1), (64, 1), (64, 0), (256, 2), (1024, 2)])
+def test_gs_matches_race_order(oracle, T, layout):
     rp, col = M.stencil27(12)
     val = M.gs_values(rp, col, seed=7)
     n = len(rp) - 1
@@ -109,6 +109,9 @@ def test_gs_matches_race_order(oracle, T):
     x0 = M.uniform_vec(n, 12)
     dx = R.DeviceArray.from_host(x0)
     db = R.DeviceArray.from_host(b)
+    ce.setOption("layout", layout)
+    if layout == 2:
+        ce.setOption("bundle_rows", 128)
     fid = ce.registerDevice(R.K_GS, prp, pcol, pval, dx, db)
     xo = x0.copy()
     for rev in (False, True, False):
@@ -118,8 +121,8 @@ def test_gs_matches_race_order(oracle, T):
         assert relerr(dx.download(), xo) <= TOL
 
 
-@pytest.mark.parametrize("T", [1, 8, 32])
-def test_kacz_matches_race_order(oracle, T):
+@pytest.mark.parametrize("T,layout", [(1, 1), (8, 1), (32, 1), (32, 0), (256, 2)])
+def test_kacz_matches_race_order(oracle, T, layout):
     rp, col = M.stencil7(14)
     val = M.jittered_values(rp, col, 0.1, seed=3)
     n = len(rp) - 1
@@ -127,6 +130,10 @@ def test_kacz_matches_race_order(oracle, T):
     b = M.uniform_vec(n, 5)
     dx = R.DeviceArray.from_host(np.zeros(n))
     db = R.DeviceArray.from_host(b)
+    ce.setOption("layout", layout)
+    if layout == 2:
+        ce.setOption("bundle_rows", 256)
+        ce.setOption("bundle_inner", 2048)
     fid = ce.registerDevice(R.K_KACZ, prp, pcol, pval, dx, db)
     xo = np.zeros(n)
     for rev in (False, True):
