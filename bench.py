@@ -3,7 +3,7 @@
 
 Workload: symmetric Gauss-Seidel (forward + backward sweep) under a RACE
 distance-1 colouring on the 3D 27-point stencil, 128^3 grid (2,097,152 rows,
-56,623,104 stored entries), off-diagonals U[-1,0), diagonal 1.05 x row |sum|,
+55,742,968 stored entries), off-diagonals U[-1,0), diagonal 1.05 x row |sum|,
 b ~ U[-1,1].  One step = one forward + one backward sweep over the whole
 matrix, with the matrix and vectors already resident in HBM.
 
@@ -42,9 +42,11 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--grid", type=int, default=128)
-    ap.add_argument("--threads", type=int, default=int(os.environ.get("RACE_BENCH_THREADS", "2048")))
+    ap.add_argument("--threads", type=int, default=int(os.environ.get("RACE_BENCH_THREADS", "16384")))
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="run a plain copy kernel of this many MiB first (byte-counter reference)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (single-GPU build)",
@@ -83,6 +85,14 @@ def main():
     log(f"register GS: {time.time() - t0:.2f}s, waves fwd={ce.waveCount(fid)} bwd={ce.waveCount(fid, True)}")
     pp, _, _ = ce.schedule()
 
+    if args.copy_mib:
+        nb = args.copy_mib << 20
+        src = R.DeviceArray(nb // 8)
+        dst = R.DeviceArray(nb // 8)
+        R.interface.L.check(R.interface.L.load().race_device_copy(dst.ptr, src.ptr, nb), "copy")
+        log(f"reference copy kernel: {args.copy_mib} MiB read + {args.copy_mib} MiB written")
+        del src, dst
+
     for _ in range(args.warmup):
         ce.executeAsync(fid, False)
         ce.executeAsync(fid, True)
@@ -104,7 +114,6 @@ def main():
     if not args.no_cpu:
         from oracle import race_oracle as O
         O.build()
-        lo, hi = ce.schedule(False)[1:], None
         pp_f, lo_f, hi_f = ce.schedule(False)
         pp_r, lo_r, hi_r = ce.schedule(True)
         fcalls = np.stack([lo_f, hi_f], axis=1).reshape(-1)
@@ -122,6 +131,15 @@ def main():
                          f"(oracle C restatement, 1 thread, {tc * 1e3:.0f} ms/step)"}
         # parity of the timed GPU state against the oracle after the same number of steps
         log(f"CPU oracle: {tc * 1e3:.1f} ms/step")
+
+    # HBM bytes per launch measured by the FETCH_SIZE/WRITE_SIZE passes recorded
+    # under profiles/ for this exact configuration (null when it differs)
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "gs_bundle_27pt128_traffic.json")
+    if os.path.exists(tfile):
+        rec = json.load(open(tfile))
+        if rec.get("grid") == g and rec.get("race_threads") == args.threads:
+            traffic = rec["hbm_bytes_per_launch"]
 
     out = {
         "metric": METRIC,
@@ -141,7 +159,7 @@ def main():
                    "rows": n, "nnz": nnz, "race_threads": args.threads,
                    "launches_per_step": launches_per_step},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(gbs / HBM_PEAK_GBS, 5), "traffic": None},
+                     "frac": round(gbs / HBM_PEAK_GBS, 5), "traffic": traffic},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

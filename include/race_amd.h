@@ -115,9 +115,9 @@ int race_register_device(race_handle h, const race_kernel_desc *desc, int *id);
  *                  1 = one workgroup per leaf (CRS),
  *                  2 = leaf bundles only (registration fails with
  *                      RACE_ERR_INCOMPATIBILITY if a leaf does not fit)
- *   "bundle_rows"  rows per bundle (default 512)
+ *   "bundle_rows"  rows per bundle (default 128)
  *   "bundle_inner" in-bundle couplings allowed per bundle (default unlimited)
- *   "bundle_threads" workgroup size of the bundle kernels: 64, 128 or 256
+ *   "bundle_threads" workgroup size of the bundle kernels: 64 ... 1024
  *   "bundle_tpr"   lanes per matrix row in the bundle kernels: 1, 2, 4 (0 = auto)
  * No counterpart in the reference (its executor has no layout choice). */
 int race_set_option(race_handle h, const char *key, int value);
@@ -168,6 +168,8 @@ int race_memcpy_h2d(void *dst, const void *src, size_t bytes);
 int race_memcpy_d2h(void *dst, const void *src, size_t bytes);
 int race_memset_device(void *dst, int value, size_t bytes);
 int race_device_synchronize(void);
+/* streaming device-to-device copy by a plain HIP kernel (bytes % 16 == 0) */
+int race_device_copy(void *dst, const void *src, size_t bytes);
 int race_device_count(int *count);
 
 /* error text, RACE_error_string (src/error.cpp:26-60) */

@@ -89,6 +89,7 @@ PROTOTYPES = {
     "race_memcpy_d2h": (ctypes.c_int, [vp, vp, ctypes.c_size_t]),
     "race_memset_device": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_size_t]),
     "race_device_synchronize": (ctypes.c_int, []),
+    "race_device_copy": (ctypes.c_int, [vp, vp, ctypes.c_size_t]),
     "race_device_count": (ctypes.c_int, [i32p]),
     "race_error_string": (ctypes.c_char_p, [ctypes.c_int]),
 }

@@ -88,7 +88,7 @@ struct race_handle_s {
     std::vector<int> perm, inv;
     std::vector<Function> funcs;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    int optLayout = 0, optBundleRows = 512, optBundleInner = 1 << 30, optBundleThreads = 256, optBundleTpr = 0;
+    int optLayout = 0, optBundleRows = 128, optBundleInner = 1 << 30, optBundleThreads = 256, optBundleTpr = 0;
     ~race_handle_s() {
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
@@ -329,7 +329,8 @@ int race_set_option(race_handle h, const char* key, int value) {
         if (value < 1 || value > 8192) return ERR_INVALID_ARG;
         h->optBundleRows = value;
     } else if (k == "bundle_threads") {
-        if (value != 64 && value != 128 && value != 256) return ERR_INVALID_ARG;
+        if (value != 64 && value != 128 && value != 256 && value != 512 && value != 1024)
+            return ERR_INVALID_ARG;
         h->optBundleThreads = value;
     } else if (k == "bundle_tpr") {
         if (value != 0 && value != 1 && value != 2 && value != 4) return ERR_INVALID_ARG;
@@ -569,6 +570,13 @@ int race_memcpy_d2h(void* dst, const void* src, size_t bytes) {
 
 int race_memset_device(void* dst, int value, size_t bytes) {
     return hipMemset(dst, value, bytes) == hipSuccess ? OK : ERR_INTERNAL;
+}
+
+int race_device_copy(void* dst, const void* src, size_t bytes) {
+    if (!dst || !src) return ERR_INVALID_ARG;
+    int e = deviceCopy(dst, src, bytes, nullptr);
+    if (e) return e;
+    return hipDeviceSynchronize() == hipSuccess ? OK : ERR_INTERNAL;
 }
 
 int race_device_synchronize(void) { return hipDeviceSynchronize() == hipSuccess ? OK : ERR_INTERNAL; }

@@ -222,7 +222,7 @@ __device__ __forceinline__ double laneSum(double v) {
 // row once; entries on columns outside the bundle (constant during the
 // phase) are folded into acc = b - sum a x.  The waves then only gather the
 // in-bundle columns from LDS; x of the bundle rows is written back last.
-template <int BS, int TPR, int E>
+template <int BS, int TPR, int E, bool ONE_RHS>
 __global__ void __launch_bounds__(BS) k_gs_bundle(
     const int* __restrict__ bSlot, const int* __restrict__ bWave, const long* __restrict__ bOff,
     const int* __restrict__ bLen, const int* __restrict__ waveSlot, const int* __restrict__ rowId,
@@ -259,16 +259,30 @@ __global__ void __launch_bounds__(BS) k_gs_bundle(
         for (int w = 0; w < nw; ++w)
             if (i >= ws[w] && i < ws[w + 1]) myWave = w;
 
-    for (int k = 0; k < nrhs; ++k) {
+    // branch-free split of the register entries: global gathers use column
+    // gi with coefficient gv, LDS gathers slot li with coefficient lv; the
+    // other coefficient of each entry is zero
+    int gi[E], li[E];
+    double gv[E], lv[E];
+#pragma unroll
+    for (int u = 0; u < E; ++u) {
+        const bool inner = cc[u] < 0;
+        gi[u] = inner ? row : cc[u];
+        li[u] = inner ? ~cc[u] : 0;
+        gv[u] = inner ? 0.0 : vv[u];
+        lv[u] = inner ? vv[u] : 0.0;
+    }
+
+    const int nr = ONE_RHS ? 1 : nrhs;
+    for (int k = 0; k < nr; ++k) {
         const size_t vo = (size_t)k * n;
         double xv[E];
 #pragma unroll
-        for (int u = 0; u < E; ++u) xv[u] = cc[u] >= 0 ? x[vo + cc[u]] : 0.0;
+        for (int u = 0; u < E; ++u) xv[u] = x[vo + gi[u]];
         double acc = (active && l == 0) ? b[vo + row] : 0.0;
         const double xr = active ? x[vo + row] : 0.0;
 #pragma unroll
-        for (int u = 0; u < E; ++u)
-            if (cc[u] >= 0) acc -= vv[u] * xv[u];
+        for (int u = 0; u < E; ++u) acc -= gv[u] * xv[u];
         acc = laneSum<TPR>(acc);
         if (active && l == 0) xl[i] = xr;
         __syncthreads();
@@ -276,8 +290,7 @@ __global__ void __launch_bounds__(BS) k_gs_bundle(
             if (myWave == w) {
                 double s = 0.0;
 #pragma unroll
-                for (int u = 0; u < E; ++u)
-                    if (cc[u] < 0) s += vv[u] * xl[~cc[u]];
+                for (int u = 0; u < E; ++u) s += lv[u] * xl[li[u]];
                 s = laneSum<TPR>(s);
                 if (l == 0) xl[i] = (acc - s) / dg;
             }
@@ -546,10 +559,16 @@ class BundleFn : public DeviceFn {
                                d.wave.p, d.off.p, d.len.p, d.waveSlot.p, d.rowId.p, d.rowNorm.p,
                                d.colSet.p, d.colSetPtr.p, d.scol.p, d.sval.p, (double*)x,
                                (const double*)b, n, nrhs, b0, d.h.phCols[p]);
+        else if (nrhs == 1)
+            hipLaunchKernelGGL((k_gs_bundle<BS, TPR, EE, true>), dim3(nbnd), dim3(BS), d.lds(p), st,
+                               d.slot.p, d.wave.p, d.off.p, d.len.p, d.waveSlot.p, d.rowId.p, d.diag.p,
+                               d.scol.p, d.sval.p, (double*)x, (const double*)b, n, nrhs, b0,
+                               d.h.phRows[p]);
         else
-            hipLaunchKernelGGL((k_gs_bundle<BS, TPR, EE>), dim3(nbnd), dim3(BS), d.lds(p), st, d.slot.p,
-                               d.wave.p, d.off.p, d.len.p, d.waveSlot.p, d.rowId.p, d.diag.p, d.scol.p,
-                               d.sval.p, (double*)x, (const double*)b, n, nrhs, b0, d.h.phRows[p]);
+            hipLaunchKernelGGL((k_gs_bundle<BS, TPR, EE, false>), dim3(nbnd), dim3(BS), d.lds(p), st,
+                               d.slot.p, d.wave.p, d.off.p, d.len.p, d.waveSlot.p, d.rowId.p, d.diag.p,
+                               d.scol.p, d.sval.p, (double*)x, (const double*)b, n, nrhs, b0,
+                               d.h.phRows[p]);
     }
     template <int BS>
     void launchBs(const Dev& d, int b0, int nbnd, int p, hipStream_t st) {
@@ -583,12 +602,13 @@ class BundleFn : public DeviceFn {
         for (int p = 0; p + 1 < (int)pb.size(); ++p) {
             const int b0 = pb[p], nbnd = pb[p + 1] - pb[p];
             if (nbnd <= 0) continue;
-            if (bs == 64)
-                launchBs<64>(d, b0, nbnd, p, st);
-            else if (bs == 128)
-                launchBs<128>(d, b0, nbnd, p, st);
-            else
-                launchBs<256>(d, b0, nbnd, p, st);
+            switch (bs) {
+                case 64: launchBs<64>(d, b0, nbnd, p, st); break;
+                case 128: launchBs<128>(d, b0, nbnd, p, st); break;
+                case 512: launchBs<512>(d, b0, nbnd, p, st); break;
+                case 1024: launchBs<1024>(d, b0, nbnd, p, st); break;
+                default: launchBs<256>(d, b0, nbnd, p, st); break;
+            }
         }
         RACE_HIP(hipGetLastError());
         return OK;
@@ -702,6 +722,24 @@ class MpkFn : public SpmvFn {
 };
 
 }  // namespace
+
+namespace {
+// plain streaming copy (16-byte vectors), used as the byte-counter reference
+__global__ void __launch_bounds__(kBlock) k_copy(const int4* __restrict__ src,
+                                                  int4* __restrict__ dst, long n16) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (; i < n16; i += stride) dst[i] = src[i];
+}
+}  // namespace
+
+int deviceCopy(void* dst, const void* src, size_t bytes, hipStream_t st) {
+    if (bytes % 16) return ERR_INVALID_ARG;
+    const long n16 = (long)(bytes / 16);
+    hipLaunchKernelGGL(k_copy, dim3(1024 * 8), dim3(kBlock), 0, st, (const int4*)src, (int4*)dst, n16);
+    RACE_HIP(hipGetLastError());
+    return OK;
+}
 
 int makeDeviceFn(const race_kernel_desc& d, const ExecContext& ctx, std::unique_ptr<DeviceFn>* out) {
     if (d.nrow <= 0 || !d.rowPtr || !d.col || !d.val) return ERR_INVALID_ARG;

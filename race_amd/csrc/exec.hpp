@@ -41,12 +41,14 @@ struct ExecContext {
     const Schedule* rev = nullptr;
     const MPKPartition* mpk = nullptr;
     int layout = 0;           // 0: leaf bundles (SELL, LDS waves), 1: one workgroup per leaf
-    int bundleRows = 512;     // rows per bundle
+    int bundleRows = 128;     // rows per bundle
     int bundleInner = 4096;   // LDS-staged entries per bundle
     int bundleThreads = 256;  // workgroup size of the bundle kernels (64, 128 or 256)
     int bundleTpr = 0;        // lanes per row in the bundle kernels (0: from the row length)
 };
 
 int makeDeviceFn(const race_kernel_desc& d, const ExecContext& ctx, std::unique_ptr<DeviceFn>* out);
+// streaming device copy, bytes a multiple of 16
+int deviceCopy(void* dst, const void* src, size_t bytes, hipStream_t st);
 
 }  // namespace race
