@@ -1,0 +1,361 @@
+"""Pinning the CPU oracle (oracle/race_oracle.c, oracle/race_oracle.py).
+
+The reference cannot be built in this image, and its tree holds no golden
+output vectors (only examples/matrices/spinSZ12.mm.gz).  Every oracle
+function is therefore pinned by checks a plausible mistake would fail:
+closed forms, textbook reductions computed independently with scipy,
+hand-traced cases, brute-force invariants.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.csgraph as csg
+from scipy.sparse.linalg import spsolve_triangular
+
+from race_amd import matgen as M
+
+from .conftest import relerr
+
+
+def grid_coords(g):
+    idx = np.arange(g ** 3)
+    return idx // (g * g), (idx // g) % g, idx % g
+
+
+# ---------------------------------------------------------------------------
+# traversal
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("g", [3, 5, 8])
+def test_bfs_closed_form_7pt(oracle, g):
+    rp, col = M.stencil7(g)
+    G = oracle.Graph(rp, col, symm_hint=True)
+    lr, ln, dist = G.traverse(oracle.TWO, 0, G.n, want_dist=True)
+    i, j, k = grid_coords(g)
+    lvl = i + j + k
+    assert len(lr) == 3 * g - 2
+    assert np.array_equal(dist, lvl)
+    assert np.array_equal(lr, np.bincount(lvl))
+    rowlen = np.diff(rp)
+    assert np.array_equal(ln, np.bincount(lvl, weights=rowlen).astype(int))
+    # stage-0 permutation = rows stably sorted by level
+    assert np.array_equal(G.perm, np.argsort(lvl, kind="stable"))
+    assert np.array_equal(G.inv[G.perm], np.arange(G.n))
+
+
+@pytest.mark.parametrize("g", [3, 6])
+def test_bfs_closed_form_27pt(oracle, g):
+    rp, col = M.stencil27(g)
+    G = oracle.Graph(rp, col, symm_hint=True)
+    lr, ln, dist = G.traverse(oracle.ONE, 0, G.n, want_dist=True)
+    i, j, k = grid_coords(g)
+    lvl = np.maximum(np.maximum(i, j), k)
+    assert np.array_equal(dist, lvl)
+    L = np.arange(g)
+    assert np.array_equal(lr, 3 * L * L + 3 * L + 1)
+    assert np.array_equal(G.perm, np.argsort(lvl, kind="stable"))
+
+
+def _random_graph(n, p, seed):
+    rng = np.random.default_rng(seed)
+    A = sp.random(n, n, density=p, random_state=rng, format="csr")
+    A = ((A + A.T) != 0).astype(np.float64)
+    A.setdiag(1.0)
+    A = A.tocsr()
+    A.sort_indices()
+    return A.indptr.astype(np.int32), A.indices.astype(np.int32)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_bfs_levels_are_shortest_paths_with_island_rule(oracle, seed):
+    """Connected part: levels = unweighted shortest path from row 0.
+    Islands: restart at the smallest unvisited row, one empty level apart
+    for colouring, adjacent for POWER (computed here with scipy)."""
+    n = 300
+    rp, col = _random_graph(n, 0.006 + 0.002 * seed, seed)
+    A = sp.csr_matrix((np.ones(len(col)), col, rp), shape=(n, n))
+    for dist_t in (oracle.ONE, oracle.POWER):
+        G = oracle.Graph(rp, col, symm_hint=True)
+        lr, ln, dist = G.traverse(dist_t, 0, n, want_dist=True)
+        want = np.full(n, -1)
+        base = 0
+        prev_max = None
+        while (want < 0).any():
+            root = int(np.nonzero(want < 0)[0][0])
+            d = csg.shortest_path(A, unweighted=True, indices=root)
+            comp = np.isfinite(d)
+            if prev_max is None:
+                base = 0
+            else:
+                base = prev_max + (2 if dist_t != oracle.POWER else 1)
+            want[comp] = base + d[comp].astype(int)
+            prev_max = want[comp].max()
+        assert np.array_equal(dist, want), dist_t
+        assert len(lr) == want.max() + 1
+
+
+def test_bfs_distance2_bridges_through_rows_outside_the_range(oracle):
+    """Hand-traced: range [0,4) of a 6-row graph in which rows 2 and 3 are
+    reached from row 1 only through row 5 (outside the range).  Distance-2
+    bridging puts 2 and 3 both at level 3 (level 2 holds only the bridge);
+    distance-1 cannot cross row 5: rows 2 and 3 are islands (no edge between
+    them), each restarted one empty level after the previous one: 3 and 5."""
+    edges = [(0, 1), (1, 5), (5, 2), (5, 3), (4, 4)]
+    n = 6
+    A = sp.lil_matrix((n, n))
+    for a, b in edges:
+        A[a, b] = 1
+        A[b, a] = 1
+    for r in range(n):
+        A[r, r] = 1
+    A = A.tocsr()
+    A.sort_indices()
+    rp, col = A.indptr.astype(np.int32), A.indices.astype(np.int32)
+    G = oracle.Graph(rp, col, symm_hint=True)
+    _, _, d2 = G.traverse(oracle.TWO, 0, 4, want_dist=True)
+    assert d2.tolist() == [0, 1, 3, 3]
+    G = oracle.Graph(rp, col, symm_hint=True)
+    _, _, d1 = G.traverse(oracle.ONE, 0, 4, want_dist=True)
+    assert d1.tolist() == [0, 1, 3, 5]
+
+
+def test_subrange_traversal_composes_permutation(oracle):
+    rp, col = M.stencil7(6)
+    G = oracle.Graph(rp, col, symm_hint=True)
+    G.traverse(oracle.ONE, 0, G.n)
+    before = G.perm.copy()
+    lo, hi = 40, 120
+    lr, ln = G.traverse(oracle.ONE, lo, hi)
+    assert lr.sum() == hi - lo
+    assert np.array_equal(G.perm[:lo], before[:lo]) and np.array_equal(G.perm[hi:], before[hi:])
+    assert sorted(G.perm[lo:hi]) == sorted(before[lo:hi])
+    assert np.array_equal(G.inv[G.perm], np.arange(G.n))
+
+
+def test_spin_fixture_and_symmetrisation(oracle, spin):
+    """examples/matrices/spinSZ12.mm.gz: 924 x 924, 6572 entries (its header).
+    The symmetrised pattern must be the pattern of A + A^T exactly."""
+    rp, col, val = spin
+    n = len(rp) - 1
+    assert n == 924 and rp[-1] == 6572
+    A = sp.csr_matrix((np.ones(len(col)), col, rp), shape=(n, n))
+    G = oracle.Graph(rp, col, symm_hint=False)
+    S = sp.csr_matrix((np.ones(len(G.col)), G.col, G.rowPtr), shape=(n, n))
+    want = ((A + A.T) != 0)
+    assert (S != want).nnz == 0
+    assert len(G.col) == want.nnz  # no duplicates added
+
+
+# ---------------------------------------------------------------------------
+# kernels
+# ---------------------------------------------------------------------------
+def _gs_matrix(g=6, seed=3):
+    rp, col = M.stencil27(g)
+    val = M.gs_values(rp, col, seed=seed)
+    rp, col, val = M.make_diag_first(rp, col, val)
+    return rp, col, val
+
+
+def test_gs_one_leaf_is_textbook_gauss_seidel(oracle):
+    rp, col, val = _gs_matrix()
+    n = len(rp) - 1
+    A = M.to_scipy(rp, col, val).tocsr()
+    b = M.uniform_vec(n, 1)
+    x0 = M.uniform_vec(n, 2)
+    Lw = sp.tril(A, k=0, format="csr")
+    Up = sp.triu(A, k=1, format="csr")
+    fwd = spsolve_triangular(Lw, b - Up @ x0, lower=True)
+    x = x0.copy()
+    oracle.run_calls(oracle.K_GS, rp, col, val, x, b, [(0, n)])
+    assert relerr(x, fwd) < 1e-13
+    Uw = sp.triu(A, k=0, format="csr")
+    Lo = sp.tril(A, k=-1, format="csr")
+    bwd = spsolve_triangular(Uw, b - Lo @ fwd, lower=False)
+    oracle.run_calls(oracle.K_GS, rp, col, val, x, b, [(n - 1, -1)])
+    assert relerr(x, bwd) < 1e-13
+
+
+def test_kacz_step_projects_onto_row(oracle):
+    """After the Kaczmarz step of row r, a_r . x == b_r, and x moved along a_r."""
+    rp, col, val = _gs_matrix()
+    n = len(rp) - 1
+    A = M.to_scipy(rp, col, val).tocsr()
+    b = M.uniform_vec(n, 4)
+    x = M.uniform_vec(n, 5)
+    rng = np.random.default_rng(0)
+    for r in rng.integers(0, n, 20):
+        x_old = x.copy()
+        oracle.run_calls(oracle.K_KACZ, rp, col, val, x, b, [(int(r), int(r) + 1)])
+        ar = A.getrow(int(r)).toarray().ravel()
+        assert abs(ar @ x - b[r]) < 1e-12 * (abs(b[r]) + np.abs(ar) @ np.abs(x))
+        dx = x - x_old
+        # dx is parallel to a_r
+        assert np.linalg.norm(dx - (dx @ ar) / (ar @ ar) * ar) < 1e-12 * max(1.0, np.linalg.norm(dx))
+
+
+def test_kacz_sweep_converges_to_solution(oracle):
+    rp, col, val = _gs_matrix(5, 8)
+    n = len(rp) - 1
+    A = M.to_scipy(rp, col, val).tocsr()
+    xs = M.uniform_vec(n, 9)
+    b = A @ xs
+    x = np.zeros(n)
+    # each step projects onto a hyperplane containing xs: the error never grows
+    prev = np.linalg.norm(xs)
+    e0 = prev
+    for _ in range(10):
+        oracle.run_calls(oracle.K_KACZ, rp, col, val, x, b, [(0, n)])
+        oracle.run_calls(oracle.K_KACZ, rp, col, val, x, b, [(n - 1, -1)])
+        e = np.linalg.norm(x - xs)
+        assert e <= prev * (1 + 1e-12)
+        prev = e
+    assert prev < 0.5 * e0
+
+
+def test_symmspmv_equals_full_spmv(oracle):
+    rp, col = M.stencil7(7)
+    val = M.jittered_values(rp, col, 0.1, seed=2)
+    rp, col, val = M.make_diag_first(rp, col, val)
+    n = len(rp) - 1
+    urp, ucol, uval = M.upper_triangle(rp, col, val)
+    x = M.uniform_vec(n, 3)
+    b = np.zeros(n)
+    # any split of the rows into sequential leaves gives the same sum
+    calls = [(0, 100), (100, 101), (101, n)]
+    oracle.run_calls(oracle.K_SYMMSPMV, urp, ucol, uval, x, b, calls)
+    A = M.to_scipy(rp, col, val)
+    assert relerr(b, A @ x) < 1e-14
+
+
+def test_mpk_equals_repeated_spmv(oracle):
+    rp, col = M.stencil7(6)
+    val = M.jittered_values(rp, col, 0.1, seed=5)
+    n = len(rp) - 1
+    A = M.to_scipy(rp, col, val)
+    x0 = M.uniform_vec(n, 6)
+    X = oracle.mpk_serial(rp, col, val, x0, 5)
+    y = x0.copy()
+    for p in range(1, 6):
+        y = A @ y
+        assert relerr(X[p], y) < 1e-14
+
+
+def test_multi_rhs_columns_are_independent(oracle):
+    rp, col, val = _gs_matrix(4, 1)
+    n = len(rp) - 1
+    b = M.uniform_vec(n, 1, nrhs=3)
+    x = np.zeros(3 * n)
+    oracle.run_calls(oracle.K_GS, rp, col, val, x, b, [(0, n)], nrhs=3)
+    for k in range(3):
+        xk = np.zeros(n)
+        oracle.run_calls(oracle.K_GS, rp, col, val, xk, b[k * n:(k + 1) * n].copy(), [(0, n)])
+        assert np.array_equal(xk, x[k * n:(k + 1) * n])
+
+
+# ---------------------------------------------------------------------------
+# load balancer, zone tree, schedule
+# ---------------------------------------------------------------------------
+def _phase_labels(n, phases):
+    """per phase: leaf label of every row (-1 when the row is not in the phase)"""
+    out = []
+    for p in phases:
+        lab = np.full(n, -1)
+        for k, (a, b) in enumerate(p):
+            lo, hi = (a, b) if b >= a else (b + 1, a + 1)
+            lab[lo:hi] = k
+        out.append(lab)
+    return out
+
+
+@pytest.mark.parametrize("T", [2, 5, 16, 64, 200])
+@pytest.mark.parametrize("dist", [1, 2])
+def test_schedule_is_a_valid_distance_k_colouring(oracle, T, dist):
+    """Leaves run concurrently in one phase share no edge (distance 1) and,
+    for distance 2, no common neighbour; every row runs exactly once."""
+    rp, col = (M.stencil27(7) if dist == 1 else M.stencil7(9))
+    G = oracle.Graph(rp, col, symm_hint=True)
+    zt = oracle.level_recursion(G, T, dist)
+    n = G.n
+    # permuted adjacency
+    prp, pcol, _ = M.permute(rp, col, None, G.perm, G.inv)
+    A = sp.csr_matrix((np.ones(len(pcol)), pcol, prp), shape=(n, n))
+    if dist == 2:
+        A = ((A @ A) != 0).astype(np.float64)
+    A = A.tocoo()
+    covered = np.zeros(n, int)
+    for rev in (False, True):
+        ph = oracle.phases(zt, rev)
+        for lab in _phase_labels(n, ph):
+            if not rev:
+                covered += lab >= 0
+            same_phase = (lab[A.row] >= 0) & (lab[A.col] >= 0)
+            assert np.all(lab[A.row][same_phase] == lab[A.col][same_phase])
+    assert np.all(covered == 1)
+    # the serial leaf order visits every row once too
+    calls = oracle.leaf_calls(zt)
+    assert sum(b - a for a, b in calls) == n
+
+
+@pytest.mark.parametrize("seed", range(40))
+def test_split_zones_invariants(oracle, seed):
+    rng = np.random.default_rng(seed)
+    L = int(rng.integers(8, 200))
+    lr = rng.integers(1, 3000, L)
+    ln = lr * rng.integers(2, 28, L)
+    T = int(rng.integers(1, max(2, L // 4)))
+    dist = int(rng.choice([1, 2]))
+    r = oracle.split_zones(lr, ln, T, 40.0, dist)
+    lp = r["levelPtr"]
+    assert lp[0] == 0 and lp[-1] == L
+    assert all(b >= a for a, b in zip(lp, lp[1:]))
+    assert sum(r["zone_rows"]) == lr.sum()
+    assert len(lp) - 1 == r["bpt"] * r["threads"]
+    assert sum(r["scale"]) in (T - 1, T, T + 1) or r["threads"] < T
+
+
+def test_level_recursion_thread_accounting(oracle):
+    """nthreads of every inner node = sum over threads of the max child
+    nthreads per colour; the root reports the available threads."""
+    rp, col = M.stencil27(8)
+    for T in (3, 12, 50):
+        G = oracle.Graph(rp, col, symm_hint=True)
+        zt = oracle.level_recursion(G, T, 1)
+        for idx, leaf in enumerate(zt.tree):
+            ch = leaf.children
+            if not ch:
+                continue
+            best = 0
+            for sb in range(len(ch) // 2):
+                nt = (ch[2 * sb + 1] - ch[2 * sb]) // zt.bpt
+                best = max(best, sum(max(zt.tree[ch[2 * sb] + zt.bpt * t + b].nthreadsZ
+                                         for b in range(zt.bpt)) for t in range(nt)))
+            assert leaf.nthreadsZ == best
+        assert zt.available_threads == zt.tree[0].nthreadsZ
+        assert np.array_equal(np.sort(G.perm), np.arange(G.n))
+
+
+def test_one_thread_is_the_serial_order(oracle):
+    rp, col = M.stencil27(6)
+    G = oracle.Graph(rp, col, symm_hint=True)
+    zt = oracle.level_recursion(G, 1, 1)
+    assert oracle.leaf_calls(zt) == [(0, G.n)] or sum(b - a for a, b in oracle.leaf_calls(zt)) == G.n
+    assert [(a, b) for a, b in oracle.leaf_calls(zt) if b > a][0][0] == 0
+
+
+def test_mpk_partition_respects_cache_bound(oracle):
+    rp, col = M.stencil7(12)
+    G = oracle.Graph(rp, col, symm_hint=True)
+    k, cache = 4, 96 * 1024.0
+    part = oracle.mpk_partition(G, k, cache)
+    gl = part["groupLevel"]
+    ln = part["levelNnz"]
+    elem = [2.0 * (k + 1) * int(v) for v in ln]
+    assert gl[0] == 0 and gl[-1] == len(ln)
+    for a, b in zip(gl, gl[1:]):
+        assert b > a
+        # a group is cut before the level that reaches the bound
+        if b - a > 1:
+            assert sum(elem[a:b - 1]) < part["cache_elem"]
+    # consecutive groups: adding the next group's first level reaches the bound
+    for a, b, c in zip(gl, gl[1:], gl[2:]):
+        assert sum(elem[a:b + 1]) >= part["cache_elem"]
+    assert part["groupPtr"][-1] == G.n
