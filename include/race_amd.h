@@ -18,8 +18,9 @@
 extern "C" {
 #endif
 
-/* status codes == RACE_error (include/error.h:29-41) */
-enum race_status {
+/* status codes: the reference's RACE_error, same names and values
+ * (include/error.h:29-41) */
+enum RACE_error {
     RACE_SUCCESS = 0,
     RACE_ERR_INVALID_ARG = 1,
     RACE_ERR_MATRIX_SYMM = 2,
@@ -32,6 +33,9 @@ enum race_status {
     RACE_ERR_INTERNAL = 9,
     RACE_ERR_NOT_IMPLEMENTED = 10
 };
+#ifndef __cplusplus
+typedef enum RACE_error RACE_error;
+#endif
 
 /* RACE::dist, d2Method, LBTarget, PinMethod (include/type.h:31-55) */
 enum race_dist { RACE_ONE = 1, RACE_TWO = 2, RACE_POWER = 3 };

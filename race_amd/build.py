@@ -45,7 +45,20 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_drop_in()
     return LIB
+
+
+DROP_IN_SRC = os.path.join(os.path.dirname(HERE), "tests", "cpp", "drop_in.cpp")
+DROP_IN_BIN = os.path.join(os.path.dirname(HERE), "tests", "cpp", "drop_in")
+
+
+def build_drop_in():
+    """C++ program using RACE::Interface (include/race_amd_interface.hpp)."""
+    inc = os.path.join(os.path.dirname(HERE), "include")
+    subprocess.check_call([HIPCC, "-std=c++17", "-O2", f"--offload-arch={ARCH}", "-I", inc, DROP_IN_SRC,
+                           "-L", HERE, "-lrace_amd", "-Wl,-rpath,$ORIGIN/../../race_amd",
+                           "-o", DROP_IN_BIN])
 
 
 if __name__ == "__main__":

@@ -210,3 +210,16 @@ def test_host_callback_runs_race_order(oracle):
     xo = np.zeros(n)
     oracle.run_calls(oracle.K_GS, prp, pcol, pval, xo, b, oracle.leaf_calls(zt))
     assert relerr(x, xo) == 0.0
+
+
+def test_cpp_drop_in_program():
+    """A C++ program written against RACE::Interface (include/race_amd_interface.hpp)
+    runs a CPU Gauss-Seidel callback and the GPU kernel and compares them."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "drop_in")
+    assert os.path.exists(exe), "build with python -m race_amd.build"
+    p = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    print(p.stdout, p.stderr)
+    assert p.returncode == 0
+    assert "relerr=" in p.stdout
