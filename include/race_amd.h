@@ -119,14 +119,31 @@ int race_register_device(race_handle h, const race_kernel_desc *desc, int *id);
  *                  1 = one workgroup per leaf (CRS),
  *                  2 = leaf bundles only (registration fails with
  *                      RACE_ERR_INCOMPATIBILITY if a leaf does not fit)
- *   "bundle_rows"  rows per bundle (default 128)
- *   "bundle_inner" in-bundle couplings allowed per bundle (default unlimited)
- *   "bundle_threads" workgroup size of the bundle kernels: 64 ... 1024
+ *   "bundle_rows"  rows per bundle (default 256)
+ *   "bundle_local" LDS columns (rows + outside columns) per bundle (default 6144)
+ *   "xcd_order"    1 = consecutive bundles of a phase run on the same XCD (default 0)
+ *   "bundle_threads" workgroup size of the bundle kernels: 128, 256, 512 (default) or 1024
  *   "bundle_tpr"   lanes per matrix row in the bundle kernels: 1, 2, 4 (0 = auto)
  * No counterpart in the reference (its executor has no layout choice). */
 int race_set_option(race_handle h, const char *key, int value);
 /* re-point a registered device function at other vectors */
 int race_set_vectors(race_handle h, int id, void *x, void *b);
+
+/* FP64 iterative refinement around a registered Gauss-Seidel or Kaczmarz
+ * function (no counterpart in the reference).  a64: the FP64 matrix (full
+ * CRS, diagonal first, permuted) and the FP64 device vectors x and b.  The
+ * inner function's vectors are replaced by internal workspace; one step is
+ * r = b - A x (FP64), inner_sweeps forward+backward sweeps on A d = r in the
+ * inner function's precision from d = 0, x += d (FP64).  race_execute(id)
+ * runs one step. */
+int race_register_refinement(race_handle h, const race_kernel_desc *a64, int inner_id,
+                             int inner_sweeps, int *id);
+/* steps until ||b - A x|| / ||b|| <= tol or max_outer steps; reports the
+ * steps taken, the inner sweeps run and the final relative residual */
+int race_refine(race_handle h, int id, double tol, int max_outer, int *outer, long *sweeps,
+                double *relres);
+/* relative residual ||b - A x|| / ||b|| of a refinement function's vectors */
+int race_residual(race_handle h, int id, double *relres);
 
 /* Interface::executeFunction (src/interface.cpp:541-572): synchronous. */
 int race_execute(race_handle h, int id, int rev);

@@ -67,6 +67,11 @@ PROTOTYPES = {
     "race_register_device": (ctypes.c_int, [vp, ctypes.POINTER(KernelDesc), i32p]),
     "race_set_vectors": (ctypes.c_int, [vp, ctypes.c_int, vp, vp]),
     "race_set_option": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_int]),
+    "race_register_refinement": (ctypes.c_int, [vp, ctypes.POINTER(KernelDesc), ctypes.c_int, ctypes.c_int,
+                                                i32p]),
+    "race_refine": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, i32p,
+                                   ctypes.POINTER(ctypes.c_long), f64p]),
+    "race_residual": (ctypes.c_int, [vp, ctypes.c_int, f64p]),
     "race_execute": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int]),
     "race_execute_async": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, vp]),
     "race_set_power": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int]),

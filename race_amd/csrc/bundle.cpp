@@ -13,6 +13,8 @@ namespace {
 // order (forward: lo..hi-1, reverse: hi-1..lo).
 //  GS   : row r writes x[r] and reads x[c], c != r.
 //  KACZ : row r reads and writes x[c] for every column c of the row.
+// Rows of equal depth in a leaf are independent; running the depths in
+// order reproduces the sequential result exactly.
 void leafDepths(const Schedule& s, int kind, int n, const int* rp, const int* col,
                 std::vector<int>& depth) {
     depth.assign(n, 0);
@@ -32,6 +34,8 @@ void leafDepths(const Schedule& s, int kind, int n, const int* rp, const int* co
                     W[col[j]] = d;
                 }
             } else {
+                // true dependency on earlier writers of what r reads, anti
+                // dependency on earlier readers of what r writes
                 for (int j = rp[r]; j < rp[r + 1]; ++j) {
                     const int c = col[j];
                     if (c != r && stamp[c] == tag) d = std::max(d, W[c]);
@@ -62,52 +66,52 @@ void leafDepths(const Schedule& s, int kind, int n, const int* rp, const int* co
 }  // namespace
 
 BundleTable buildBundles(const Schedule& s, int kind, int n, const int* rp, const int* col,
-                         const double* val, int cap, int innerCap, int tpr) {
+                         const double* val, int cap, int localCap, int tpr) {
     BundleTable t;
     t.kind = kind;
     t.cap = cap;
     t.tpr = tpr;
-    // Kaczmarz holds every touched column in LDS: bound it by the entries
-    if (kind == RACE_K_KACZ) innerCap = std::min(innerCap, 6144);
+    localCap = std::min(localCap, 65535);
+    const bool gs = kind != RACE_K_KACZ;
     std::vector<int> depth;
     leafDepths(s, kind, n, rp, col, depth);
-    const bool gs = kind != RACE_K_KACZ;
 
-    std::vector<int> bundleOf(n, -1), slotOf(n, -1);
-    std::vector<int> colLocal(n, -1), colStamp(n, -1);
-    std::vector<int> rows, cnt;
+    // owner[r]: bundle whose rows include r; seen[c] tags columns already in
+    // the local space of the bundle being filled (tags are never reset)
+    std::vector<int> owner(n, -1), seen(n, -1), localOf(n, -1);
     t.slot.push_back(0);
     t.wave.push_back(0);
-    t.innerPtr.reserve(n + 1);
+    t.extPtr.push_back(0);
 
-    // staged entries of one leaf: GS -> couplings inside the leaf, KACZ -> all
-    auto leafStaged = [&](int lo, int hi) {
-        long c = 0;
-        for (int r = lo; r < hi; ++r) {
-            if (gs) {
-                for (int j = rp[r] + 1; j < rp[r + 1]; ++j)
-                    if (col[j] >= lo && col[j] < hi && col[j] != r) ++c;
-            } else {
-                c += rp[r + 1] - rp[r];
+    std::vector<int> curLeaves, rows, cnt, extCols;
+    int curRows = 0, curLocal = 0;
+    int bundleId = 0;  // id of the bundle being filled
+
+    // local columns a leaf adds to the bundle tagged `tag`
+    auto countNew = [&](int l, int tag) {
+        int add = 0;
+        const int lo = s.leafLo[l], hi = s.leafHi[l];
+        for (int r = lo; r < hi; ++r)
+            for (int j = rp[r]; j < rp[r + 1]; ++j) {
+                const int c = col[j];
+                if (gs && c >= lo && c < hi) continue;  // a row of this leaf
+                if (seen[c] != tag) {
+                    seen[c] = tag;
+                    ++add;
+                }
             }
-        }
-        return c;
+        return add + (gs ? hi - lo : 0);
     };
-    auto leafWidth = [&](int lo, int hi) {
-        int w = 0;
-        for (int r = lo; r < hi; ++r) w = std::max(w, rp[r + 1] - rp[r]);
-        return w;
+    // drop the tags of a leaf that did not fit, so it can be recounted
+    auto untag = [&](int l, int tag) {
+        for (int r = s.leafLo[l]; r < s.leafHi[l]; ++r)
+            for (int j = rp[r]; j < rp[r + 1]; ++j)
+                if (seen[col[j]] == tag) seen[col[j]] = -2;
     };
-    int curWidth = 0;
-
-    std::vector<int> bCols;  // columns held in LDS per bundle (Kaczmarz)
-    std::vector<int> curLeaves;
-    long curStaged = 0;
-    int curRows = 0;
 
     auto flush = [&]() {
         if (curLeaves.empty()) return;
-        const int b = (int)t.slot.size() - 1;
+        const int b = bundleId;
         const int s0 = t.slot.back();
         rows.clear();
         int maxD = 0;
@@ -116,19 +120,14 @@ BundleTable buildBundles(const Schedule& s, int kind, int n, const int* rp, cons
                 rows.push_back(r);
                 maxD = std::max(maxD, depth[r]);
             }
-        // order by (depth, row)
+        // slots ordered by (depth, row)
+        std::sort(rows.begin(), rows.end());
         cnt.assign(maxD + 2, 0);
         for (int r : rows) cnt[depth[r] + 1]++;
         for (int d = 1; d <= maxD + 1; ++d) cnt[d] += cnt[d - 1];
         std::vector<int> ordered(rows.size());
-        std::sort(rows.begin(), rows.end());
         for (int r : rows) ordered[cnt[depth[r]]++] = r;
         const int nb = (int)ordered.size();
-        for (int i = 0; i < nb; ++i) {
-            bundleOf[ordered[i]] = b;
-            slotOf[ordered[i]] = i;
-        }
-        // wave boundaries (depths are contiguous 1..maxD)
         {
             int i = 0;
             for (int d = 1; d <= maxD; ++d) {
@@ -137,39 +136,53 @@ BundleTable buildBundles(const Schedule& s, int kind, int n, const int* rp, cons
             }
         }
         t.wave.push_back((int)t.waveSlot.size());
-        t.maxWaves = std::max(t.maxWaves, maxD);
-        // KACZ: the set of columns touched by the bundle, held in LDS
-        int ncols = 0;
-        if (!gs) {
-            t.colSetPtr.push_back((int)t.colSet.size());
+        // local column space (sorted outside columns: coalesced gathers)
+        const int tagL = -(b + 3);  // distinct from the packing tags
+        extCols.clear();
+        if (gs) {
+            for (int i = 0; i < nb; ++i) {
+                owner[ordered[i]] = b;
+                localOf[ordered[i]] = i;
+            }
+            for (int r : ordered)
+                for (int j = rp[r] + 1; j < rp[r + 1]; ++j) {
+                    const int c = col[j];
+                    if (owner[c] != b && seen[c] != tagL) {
+                        seen[c] = tagL;
+                        extCols.push_back(c);
+                    }
+                }
+            std::sort(extCols.begin(), extCols.end());
+            for (size_t e = 0; e < extCols.size(); ++e) localOf[extCols[e]] = nb + (int)e;
+        } else {
             for (int r : ordered)
                 for (int j = rp[r]; j < rp[r + 1]; ++j) {
                     const int c = col[j];
-                    if (colStamp[c] != b) {
-                        colStamp[c] = b;
-                        colLocal[c] = ncols++;
-                        t.colSet.push_back(c);
+                    if (seen[c] != tagL) {
+                        seen[c] = tagL;
+                        extCols.push_back(c);
                     }
                 }
-            t.maxCols = std::max(t.maxCols, ncols);
+            std::sort(extCols.begin(), extCols.end());
+            for (size_t e = 0; e < extCols.size(); ++e) localOf[extCols[e]] = (int)e;
         }
-        bCols.push_back(ncols);
-        // per-lane register width: off-diagonal entries (GS) or all entries (KACZ)
+        t.ext.insert(t.ext.end(), extCols.begin(), extCols.end());
+        t.extPtr.push_back((int)t.ext.size());
+        const int local = (gs ? nb : 0) + (int)extCols.size();
+        // entry block
         int rowMax = 0;
         for (int r : ordered) rowMax = std::max(rowMax, (rp[r + 1] - rp[r]) - (gs ? 1 : 0));
         const int width = (rowMax + tpr - 1) / tpr;
         t.len.push_back(width);
         t.maxLen = std::max(t.maxLen, width);
-        const long off = (long)t.scol.size();
+        const long off = (long)t.sidx.size();
         const long lanes = (long)nb * tpr;
         t.off.push_back(off);
-        t.scol.resize(off + (long)width * lanes);
+        t.sidx.resize(off + (long)width * lanes);
         t.sval.resize(off + (long)width * lanes);
-        int staged = 0;
         for (int i = 0; i < nb; ++i) {
             const int r = ordered[i];
             t.rowId.push_back(r);
-            t.innerPtr.push_back(staged);
             const int j0 = gs ? rp[r] + 1 : rp[r];
             const int cntRow = rp[r + 1] - j0;
             double nrm = 0;
@@ -177,22 +190,12 @@ BundleTable buildBundles(const Schedule& s, int kind, int n, const int* rp, cons
                 const long pos = off + (long)(k / tpr) * lanes + (long)i * tpr + (k % tpr);
                 if (k < cntRow) {
                     const int j = j0 + k;
-                    const int c = col[j];
-                    int enc;
-                    if (gs) {
-                        enc = (bundleOf[c] == b) ? ~slotOf[c] : c;
-                        if (enc < 0) ++staged;
-                    } else {
-                        enc = ~colLocal[c];
-                        ++staged;
-                        nrm += val[j] * val[j];
-                    }
-                    t.scol[pos] = enc;
+                    t.sidx[pos] = (uint16_t)localOf[col[j]];
                     t.sval[pos] = val[j];
+                    nrm += val[j] * val[j];
                 } else {
-                    // padding: zero value; GS reads global column 0, KACZ local
-                    // column 0 (the kernel never writes through a zero value)
-                    t.scol[pos] = gs ? 0 : ~0;
+                    // padding: zero value on local column 0 (never written through)
+                    t.sidx[pos] = 0;
                     t.sval[pos] = 0.0;
                     ++t.paddedEntries;
                 }
@@ -202,61 +205,46 @@ BundleTable buildBundles(const Schedule& s, int kind, int n, const int* rp, cons
             else
                 t.rowNorm.push_back(nrm);
         }
-        t.innerEnd.push_back(staged);
-        t.maxInner = std::max(t.maxInner, staged);
-        t.maxRows = std::max(t.maxRows, nb);
         t.slot.push_back(s0 + nb);
+        t.phRows.back() = std::max(t.phRows.back(), nb);
+        t.phLocal.back() = std::max(t.phLocal.back(), local);
+        t.phWaves.back() = std::max(t.phWaves.back(), maxD);
         curLeaves.clear();
         curRows = 0;
-        curStaged = 0;
+        curLocal = 0;
+        ++bundleId;
     };
 
     for (int p = 0; p < s.numPhases(); ++p) {
         t.phaseBundle.push_back((int)t.slot.size() - 1);
+        t.phRows.push_back(0);
+        t.phLocal.push_back(0);
+        t.phWaves.push_back(0);
         for (int l = s.phasePtr[p]; l < s.phasePtr[p + 1]; ++l) {
             const int rowsL = s.leafHi[l] - s.leafLo[l];
-            const long st = leafStaged(s.leafLo[l], s.leafHi[l]);
-            if (rowsL > cap || st > innerCap) {
+            if (rowsL > cap) {
                 t.ok = false;
                 return t;
             }
-            // Kaczmarz stages the padded SELL block: rows x widest row
-            const int wL = gs ? 0 : leafWidth(s.leafLo[l], s.leafHi[l]);
-            if (!gs && (long)rowsL * wL > innerCap) {
-                t.ok = false;
-                return t;
-            }
-            if (curRows + rowsL > cap || curStaged + st > innerCap ||
-                (!gs && (long)(curRows + rowsL) * std::max(curWidth, wL) > innerCap)) {
+            int add = countNew(l, bundleId);
+            if (!curLeaves.empty() && (curRows + rowsL > cap || curLocal + add > localCap)) {
+                untag(l, bundleId);
                 flush();
-                curWidth = 0;
+                add = countNew(l, bundleId);
+            }
+            if (add > localCap) {
+                t.ok = false;
+                return t;
             }
             curLeaves.push_back(l);
             curRows += rowsL;
-            curStaged += st;
-            curWidth = std::max(curWidth, wL);
+            curLocal += add;
         }
         flush();
-        // depth of the phase = deepest bundle; LDS needs of the phase
-        int deepest = 0, mr = 0, mi = 0, mc = 0, me = 0;
-        for (int b = t.phaseBundle.back(); b + 1 < (int)t.slot.size(); ++b) {
-            const int rows_b = t.slot[b + 1] - t.slot[b];
-            deepest = std::max(deepest, t.wave[b + 1] - t.wave[b]);
-            mr = std::max(mr, rows_b);
-            mi = std::max(mi, t.innerEnd[b]);
-            me = std::max(me, t.len[b] * rows_b * tpr);
-            mc = std::max(mc, bCols[b]);
-        }
-        t.critical += deepest;
-        t.phRows.push_back(mr);
-        t.phInner.push_back(mi);
-        t.phWaves.push_back(deepest);
-        t.phEntries.push_back(me);
-        t.phCols.push_back(mc);
+        t.critical += t.phWaves.back();
     }
     t.phaseBundle.push_back((int)t.slot.size() - 1);
     t.waveSlot.push_back(t.slot.back());
-    if (!gs) t.colSetPtr.push_back((int)t.colSet.size());
     return t;
 }
 

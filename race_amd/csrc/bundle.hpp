@@ -1,16 +1,22 @@
-// bundle.hpp -- leaf bundles: the execution-order SELL layout for
-// Gauss-Seidel and Kaczmarz phases.
+// bundle.hpp -- leaf bundles: the execution-order layout for Gauss-Seidel
+// and Kaczmarz phases.
 //
-// The leaves of one colour phase are independent, so a workgroup can run
+// The leaves of one colour phase are independent, so one workgroup runs
 // several of them at once.  A bundle is a run of whole leaves of one phase
-// (at most `cap` rows).  Its rows are re-ordered by dependency wave, and the
-// bundle's matrix rows are stored column-major (SELL with the bundle as the
-// chunk), so lane i of the workgroup reads entry j of slot i at
-// off + j*rows + i: one coalesced stream per entry column.
+// (at most `cap` rows).  Its rows ("slots") are ordered by dependency wave.
 //
-// Column encoding: c >= 0 is a global column (constant during the phase);
-// c < 0 is ~slot of a column that lives in the workgroup's LDS copy of x.
+// Every bundle has a local column space held in LDS:
+//   Gauss-Seidel: [0, rows)      the bundle's own rows (updated by the waves)
+//                 [rows, rows+e) the distinct columns outside the bundle that
+//                                its rows read (constant during the phase)
+//   Kaczmarz:     [0, cols)      every column the bundle's rows touch
+// so a matrix entry is a 16-bit local index plus a value.  With tpr lanes per
+// row, row entry k goes to lane k % tpr at register slot k / tpr and the
+// bundle's block is laid out column-major over lanes:
+//   entry(u, slot, lane) = off + u*(rows*tpr) + slot*tpr + lane
+// so each load instruction of a workgroup reads one contiguous stream.
 #pragma once
+#include <cstdint>
 #include <vector>
 
 #include "engine.hpp"
@@ -19,39 +25,35 @@ namespace race {
 
 struct BundleTable {
     int kind = 0;                   // RACE_K_GS or RACE_K_KACZ
-    int cap = 512;                  // rows per bundle
+    int cap = 128;                  // rows per bundle
+    int tpr = 1;                    // lanes per row
     std::vector<int> phaseBundle;   // phase -> [first bundle, last bundle)
-    std::vector<int> slot;          // bundle -> first slot (rows in slot order)
+    std::vector<int> slot;          // bundle -> first slot
     std::vector<int> wave;          // bundle -> first wave
     std::vector<int> waveSlot;      // wave -> first slot (absolute)
-    std::vector<long> off;          // bundle -> first SELL entry
+    std::vector<long> off;          // bundle -> first entry of its block
     std::vector<int> len;           // bundle -> register entries per lane
-    int tpr = 1;                    // lanes per row
-    int maxLen = 0;                 // largest per-lane width over all bundles
-    std::vector<int> innerPtr;      // slot -> first staged entry, bundle-relative
-    std::vector<int> innerEnd;      // bundle -> number of staged entries
+    std::vector<int> extPtr;        // bundle -> first entry in ext
+    std::vector<int> ext;           // global columns of the local column space
+                                    // (GS: outside columns; KACZ: all columns)
     std::vector<int> rowId;         // slot -> matrix row
     std::vector<double> diag;       // slot -> diagonal value (GS)
     std::vector<double> rowNorm;    // slot -> sum of squares of the row (KACZ)
-    std::vector<int> colSet;        // KACZ: bundle columns held in LDS
-    std::vector<int> colSetPtr;     // KACZ: bundle -> first entry in colSet
-    std::vector<int> scol;          // SELL columns (encoded)
-    std::vector<double> sval;       // SELL values
+    std::vector<uint16_t> sidx;     // entries: local column
+    std::vector<double> sval;       // entries: value
     long critical = 0;              // sum over phases of the deepest bundle
-    int numPhases() const { return (int)phaseBundle.size() - 1; }
-    int maxRows = 0, maxInner = 0, maxWaves = 0, maxCols = 0;
-    // per-phase maxima, used to size the LDS of each phase launch
-    std::vector<int> phRows, phInner, phWaves, phCols, phEntries;
+    int maxLen = 0;                 // largest per-lane width
     long paddedEntries = 0;
-    bool ok = true;                 // false if some leaf exceeds the bundle limits
+    bool ok = true;                 // false if a leaf exceeds the bundle limits
+    // per-phase maxima (LDS sizing of each launch)
+    std::vector<int> phRows, phLocal, phWaves;
+    int numPhases() const { return (int)phaseBundle.size() - 1; }
 };
 
-// Build the bundle table of a schedule direction.  rowPtr/col/val: permuted
-// CRS with the diagonal first in each row.
-// tpr lanes per row: row entry k (0-based among the stored entries) goes to
-// lane k % tpr at register position k / tpr; a bundle's block is laid out as
-// off + u*(rows*tpr) + slot*tpr + lane.
+// Build the bundle table of one sweep direction.  rowPtr/col/val: permuted
+// CRS with the diagonal first in each row.  localCap bounds the local column
+// space of a bundle (LDS entries, at most 65535).
 BundleTable buildBundles(const Schedule& s, int kind, int n, const int* rowPtr, const int* col,
-                         const double* val, int cap, int innerCap, int tpr);
+                         const double* val, int cap, int localCap, int tpr);
 
 }  // namespace race

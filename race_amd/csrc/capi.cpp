@@ -88,7 +88,7 @@ struct race_handle_s {
     std::vector<int> perm, inv;
     std::vector<Function> funcs;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    int optLayout = 0, optBundleRows = 128, optBundleInner = 1 << 30, optBundleThreads = 256, optBundleTpr = 0;
+    int optLayout = 0, optBundleRows = 256, optBundleLocal = 6144, optBundleThreads = 512, optBundleTpr = 0, optXcd = 0;
     ~race_handle_s() {
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
@@ -308,7 +308,8 @@ int race_register_device(race_handle h, const race_kernel_desc* d, int* id) {
     ctx.mpk = &h->mpk;
     ctx.layout = h->optLayout;
     ctx.bundleRows = h->optBundleRows;
-    ctx.bundleInner = h->optBundleInner;
+    ctx.bundleLocal = h->optBundleLocal;
+    ctx.xcdOrder = h->optXcd;
     ctx.bundleThreads = h->optBundleThreads;
     ctx.bundleTpr = h->optBundleTpr;
     Function fn;
@@ -329,19 +330,47 @@ int race_set_option(race_handle h, const char* key, int value) {
         if (value < 1 || value > 8192) return ERR_INVALID_ARG;
         h->optBundleRows = value;
     } else if (k == "bundle_threads") {
-        if (value != 64 && value != 128 && value != 256 && value != 512 && value != 1024)
-            return ERR_INVALID_ARG;
+        if (value != 128 && value != 256 && value != 512 && value != 1024) return ERR_INVALID_ARG;
         h->optBundleThreads = value;
     } else if (k == "bundle_tpr") {
         if (value != 0 && value != 1 && value != 2 && value != 4) return ERR_INVALID_ARG;
         h->optBundleTpr = value;
-    } else if (k == "bundle_inner") {
-        if (value < 1) return ERR_INVALID_ARG;
-        h->optBundleInner = value;
+    } else if (k == "bundle_local") {
+        if (value < 1 || value > 65535) return ERR_INVALID_ARG;
+        h->optBundleLocal = value;
+    } else if (k == "xcd_order") {
+        if (value != 0 && value != 1) return ERR_INVALID_ARG;
+        h->optXcd = value;
     } else {
         return ERR_INVALID_ARG;
     }
     return OK;
+}
+
+int race_register_refinement(race_handle h, const race_kernel_desc* a64, int inner_id, int inner_sweeps,
+                             int* id) {
+    if (checkHandle(h) || !a64 || !id || inner_id < 0 || inner_id >= (int)h->funcs.size())
+        return ERR_INVALID_ARG;
+    if (!h->funcs[inner_id].dev || a64->nrow != h->nrow) return ERR_INVALID_ARG;
+    Function fn;
+    int e = makeRefineFn(*a64, h->funcs[inner_id].dev.get(), inner_sweeps, &fn.dev);
+    if (e) return e;
+    h->funcs.push_back(std::move(fn));
+    *id = (int)h->funcs.size() - 1;
+    return OK;
+}
+
+int race_refine(race_handle h, int id, double tol, int max_outer, int* outer, long* sweeps,
+                double* relres) {
+    if (checkHandle(h) || id < 0 || id >= (int)h->funcs.size() || !h->funcs[id].dev)
+        return ERR_INVALID_ARG;
+    return refineSolve(h->funcs[id].dev.get(), tol, max_outer, h->stream, outer, sweeps, relres);
+}
+
+int race_residual(race_handle h, int id, double* relres) {
+    if (checkHandle(h) || id < 0 || id >= (int)h->funcs.size() || !h->funcs[id].dev || !relres)
+        return ERR_INVALID_ARG;
+    return refineResidual(h->funcs[id].dev.get(), h->stream, relres);
 }
 
 int race_set_vectors(race_handle h, int id, void* x, void* b) {

@@ -3,8 +3,8 @@
 #include <algorithm>
 #include <cstring>
 
-#include "bundle.hpp"
 #include "exec.hpp"
+#include "exec_common.hpp"
 
 namespace race {
 
@@ -90,16 +90,7 @@ WaveTable buildWaves(const Schedule& s, int kind, int n, const int* rp, const in
 // ---------------------------------------------------------------------------
 namespace {
 
-constexpr int kBlock = 256;
-constexpr size_t kMaxLds = 64 * 1024;  // LDS bytes one workgroup may allocate
-constexpr int kBatch = 16;             // SELL entries loaded per batch in bundle prologues
 
-template <int W>
-__device__ __forceinline__ double groupSum(double v) {
-#pragma unroll
-    for (int o = W / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, W);
-    return v;
-}
 
 // Gauss-Seidel: one workgroup per leaf, waves in order, TPR lanes per row.
 // x[row] = (b[row] - sum_{j != diag} a_j x[c_j]) / a_diag, diagonal first.
@@ -207,176 +198,6 @@ __global__ void __launch_bounds__(kBlock) k_spmv(const int* __restrict__ rp,
 }
 
 
-// ---------------------------------------------------------------------------
-// leaf-bundle kernels: one workgroup per bundle, TPR lanes per row, each lane
-// holding E (column, value) pairs of its row in registers.
-// ---------------------------------------------------------------------------
-template <int TPR>
-__device__ __forceinline__ double laneSum(double v) {
-#pragma unroll
-    for (int o = TPR / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, TPR);
-    return v;
-}
-
-// Gauss-Seidel on one bundle.  Prologue: every lane loads its share of the
-// row once; entries on columns outside the bundle (constant during the
-// phase) are folded into acc = b - sum a x.  The waves then only gather the
-// in-bundle columns from LDS; x of the bundle rows is written back last.
-template <int BS, int TPR, int E, bool ONE_RHS>
-__global__ void __launch_bounds__(BS) k_gs_bundle(
-    const int* __restrict__ bSlot, const int* __restrict__ bWave, const long* __restrict__ bOff,
-    const int* __restrict__ bLen, const int* __restrict__ waveSlot, const int* __restrict__ rowId,
-    const double* __restrict__ diag, const int* __restrict__ scol, const double* __restrict__ sval,
-    double* __restrict__ x, const double* __restrict__ b, int n, int nrhs, int bundle0, int ldsRows) {
-    extern __shared__ double smem[];
-    double* xl = smem;
-    int* ws = reinterpret_cast<int*>(xl + ldsRows);
-
-    const int bnd = bundle0 + blockIdx.x;
-    const int s0 = bSlot[bnd], nb = bSlot[bnd + 1] - s0;
-    const long off = bOff[bnd];
-    const int wb = bLen[bnd];
-    const int w0 = bWave[bnd], nw = bWave[bnd + 1] - w0;
-    const int t = threadIdx.x, i = t / TPR, l = t % TPR;
-    const bool active = i < nb;
-    const long lanes = (long)nb * TPR;
-
-    int cc[E];
-    double vv[E];
-#pragma unroll
-    for (int u = 0; u < E; ++u) {
-        const bool in = active && u < wb;
-        const long e = off + (long)u * lanes + t;
-        cc[u] = in ? scol[e] : 0;
-        vv[u] = in ? sval[e] : 0.0;
-    }
-    const int row = active ? rowId[s0 + i] : 0;
-    const double dg = active ? diag[s0 + i] : 1.0;
-    for (int w = t; w <= nw; w += BS) ws[w] = waveSlot[w0 + w] - s0;
-    __syncthreads();
-    int myWave = -1;
-    if (active)
-        for (int w = 0; w < nw; ++w)
-            if (i >= ws[w] && i < ws[w + 1]) myWave = w;
-
-    // branch-free split of the register entries: global gathers use column
-    // gi with coefficient gv, LDS gathers slot li with coefficient lv; the
-    // other coefficient of each entry is zero
-    int gi[E], li[E];
-    double gv[E], lv[E];
-#pragma unroll
-    for (int u = 0; u < E; ++u) {
-        const bool inner = cc[u] < 0;
-        gi[u] = inner ? row : cc[u];
-        li[u] = inner ? ~cc[u] : 0;
-        gv[u] = inner ? 0.0 : vv[u];
-        lv[u] = inner ? vv[u] : 0.0;
-    }
-
-    const int nr = ONE_RHS ? 1 : nrhs;
-    for (int k = 0; k < nr; ++k) {
-        const size_t vo = (size_t)k * n;
-        double xv[E];
-#pragma unroll
-        for (int u = 0; u < E; ++u) xv[u] = x[vo + gi[u]];
-        double acc = (active && l == 0) ? b[vo + row] : 0.0;
-        const double xr = active ? x[vo + row] : 0.0;
-#pragma unroll
-        for (int u = 0; u < E; ++u) acc -= gv[u] * xv[u];
-        acc = laneSum<TPR>(acc);
-        if (active && l == 0) xl[i] = xr;
-        __syncthreads();
-        for (int w = 0; w < nw; ++w) {
-            if (myWave == w) {
-                double s = 0.0;
-#pragma unroll
-                for (int u = 0; u < E; ++u) s += lv[u] * xl[li[u]];
-                s = laneSum<TPR>(s);
-                if (l == 0) xl[i] = (acc - s) / dg;
-            }
-            __syncthreads();
-        }
-        if (active && l == 0) x[vo + row] = xl[i];
-        __syncthreads();
-    }
-}
-
-// Kaczmarz on one bundle: every column the bundle touches lives in LDS; the
-// row entries stay in registers for all right-hand sides.  Rows of one wave
-// share no column, so their LDS updates never collide.
-template <int BS, int TPR, int E>
-__global__ void __launch_bounds__(BS) k_kacz_bundle(
-    const int* __restrict__ bSlot, const int* __restrict__ bWave, const long* __restrict__ bOff,
-    const int* __restrict__ bLen, const int* __restrict__ waveSlot, const int* __restrict__ rowId,
-    const double* __restrict__ rowNorm, const int* __restrict__ colSet,
-    const int* __restrict__ colSetPtr, const int* __restrict__ scol, const double* __restrict__ sval,
-    double* __restrict__ x, const double* __restrict__ b, int n, int nrhs, int bundle0, int ldsCols) {
-    extern __shared__ double smem[];
-    double* xl = smem;
-    int* ws = reinterpret_cast<int*>(xl + ldsCols);
-
-    const int bnd = bundle0 + blockIdx.x;
-    const int s0 = bSlot[bnd], nb = bSlot[bnd + 1] - s0;
-    const long off = bOff[bnd];
-    const int wb = bLen[bnd];
-    const int w0 = bWave[bnd], nw = bWave[bnd + 1] - w0;
-    const int c0 = colSetPtr[bnd], nc = colSetPtr[bnd + 1] - c0;
-    const int t = threadIdx.x, i = t / TPR, l = t % TPR;
-    const bool active = i < nb;
-    const long lanes = (long)nb * TPR;
-
-    int cc[E];
-    double vv[E];
-#pragma unroll
-    for (int u = 0; u < E; ++u) {
-        const bool in = active && u < wb;
-        const long e = off + (long)u * lanes + t;
-        cc[u] = in ? ~scol[e] : 0;
-        vv[u] = in ? sval[e] : 0.0;
-    }
-    const int row = active ? rowId[s0 + i] : 0;
-    const double rn = active ? rowNorm[s0 + i] : 1.0;
-    for (int w = t; w <= nw; w += BS) ws[w] = waveSlot[w0 + w] - s0;
-    __syncthreads();
-    int myWave = -1;
-    if (active)
-        for (int w = 0; w < nw; ++w)
-            if (i >= ws[w] && i < ws[w + 1]) myWave = w;
-
-    for (int k = 0; k < nrhs; ++k) {
-        const size_t vo = (size_t)k * n;
-        for (int c = t; c < nc; c += BS) xl[c] = x[vo + colSet[c0 + c]];
-        const double br = active ? b[vo + row] : 0.0;
-        __syncthreads();
-        for (int w = 0; w < nw; ++w) {
-            if (myWave == w) {
-                double dot = 0.0;
-#pragma unroll
-                for (int u = 0; u < E; ++u) dot += vv[u] * xl[cc[u]];
-                dot = laneSum<TPR>(dot);
-                const double scale = (dot - br) / rn;
-#pragma unroll
-                for (int u = 0; u < E; ++u)
-                    if (vv[u] != 0.0) xl[cc[u]] -= scale * vv[u];
-            }
-            __syncthreads();
-        }
-        for (int c = t; c < nc; c += BS) x[vo + colSet[c0 + c]] = xl[c];
-        __syncthreads();
-    }
-}
-
-inline unsigned gridFor(long n, int block = kBlock) {
-    long g = (n + block - 1) / block;
-    return (unsigned)(g < 1 ? 1 : g);
-}
-
-template <typename T>
-int upload(DevBuf<T>& d, const T* h, size_t n) {
-    RACE_HIP(d.reserve(n));
-    if (n) RACE_HIP(hipMemcpy(d.p, h, n * sizeof(T), hipMemcpyHostToDevice));
-    return OK;
-}
 
 // device CRS copy of a caller matrix
 struct DevCrs {
@@ -410,14 +231,6 @@ struct DevWaves {
     }
 };
 
-// threads per row from the mean row length
-int pickTpr(long nnz, int n) {
-    double m = n ? (double)nnz / n : 1;
-    if (m <= 6) return 4;
-    if (m <= 12) return 8;
-    if (m <= 24) return 16;
-    return 32;
-}
 
 // ---------------------------------------------------------------------------
 class WaveFn : public DeviceFn {
@@ -469,151 +282,6 @@ class WaveFn : public DeviceFn {
     }
 };
 
-
-// Gauss-Seidel / Kaczmarz over leaf bundles; falls back to WaveFn when a
-// leaf does not fit the bundle limits.
-class BundleFn : public DeviceFn {
-  public:
-    struct Dev {
-        BundleTable h;
-        DevBuf<int> slot, wave, len, waveSlot, rowId, scol, colSet, colSetPtr;
-        DevBuf<long> off;
-        DevBuf<double> diag, rowNorm, sval;
-        int load(BundleTable&& t) {
-            h = std::move(t);
-            int e;
-            if ((e = upload(slot, h.slot.data(), h.slot.size()))) return e;
-            if ((e = upload(wave, h.wave.data(), h.wave.size()))) return e;
-            if ((e = upload(len, h.len.data(), h.len.size()))) return e;
-            if ((e = upload(off, h.off.data(), h.off.size()))) return e;
-            if ((e = upload(waveSlot, h.waveSlot.data(), h.waveSlot.size()))) return e;
-            if ((e = upload(rowId, h.rowId.data(), h.rowId.size()))) return e;
-            if ((e = upload(scol, h.scol.data(), h.scol.size()))) return e;
-            if ((e = upload(sval, h.sval.data(), h.sval.size()))) return e;
-            if (h.kind == RACE_K_KACZ) {
-                if ((e = upload(rowNorm, h.rowNorm.data(), h.rowNorm.size()))) return e;
-                if ((e = upload(colSet, h.colSet.data(), h.colSet.size()))) return e;
-                if ((e = upload(colSetPtr, h.colSetPtr.data(), h.colSetPtr.size()))) return e;
-            } else {
-                if ((e = upload(diag, h.diag.data(), h.diag.size()))) return e;
-            }
-            return OK;
-        }
-        size_t lds(int p) const {
-            const int held = h.kind == RACE_K_KACZ ? h.phCols[p] : h.phRows[p];
-            return (size_t)held * sizeof(double) + (size_t)(h.phWaves[p] + 1) * sizeof(int) + 16;
-        }
-    };
-    Dev df, dr;
-    int nrhs = 1, n = 0, bs = 256, tpr = 2, E = 16;
-    std::unique_ptr<WaveFn> fallback;
-
-    // register widths with a kernel instantiation, per lanes-per-row
-    static int pickE(int tpr, int need) {
-        static const int e1[] = {4, 8, 16, 32}, e2[] = {4, 8, 13, 16}, e4[] = {2, 4, 7, 8};
-        const int* c = tpr == 1 ? e1 : tpr == 2 ? e2 : e4;
-        for (int q = 0; q < 4; ++q)
-            if (c[q] >= need) return c[q];
-        return -1;
-    }
-
-    int init(const race_kernel_desc& d, const ExecContext& ctx, int cap, int innerCap) {
-        kind = d.kind;
-        nrhs = d.nrhs > 0 ? d.nrhs : 1;
-        n = d.nrow;
-        x = d.x;
-        b = d.b;
-        bs = ctx.bundleThreads;
-        int maxLen = 0;
-        for (int r = 0; r < n; ++r) maxLen = std::max(maxLen, d.rowPtr[r + 1] - d.rowPtr[r]);
-        if (kind != RACE_K_KACZ) maxLen -= 1;  // the diagonal is kept apart
-        tpr = ctx.bundleTpr > 0 ? ctx.bundleTpr : (maxLen <= 8 ? 1 : maxLen <= 32 ? 2 : 4);
-        E = pickE(tpr, (maxLen + tpr - 1) / tpr);
-        const int rowsCap = std::min(cap, bs / tpr);
-        if (E < 0 || rowsCap < 1) {
-            if (ctx.layout == 2) return ERR_INCOMPATIBILITY;
-            fallback = std::make_unique<WaveFn>();
-            return fallback->init(d, ctx);
-        }
-        BundleTable tf = buildBundles(*ctx.fwd, kind, d.nrow, d.rowPtr, d.col, d.val, rowsCap, innerCap, tpr);
-        BundleTable tr = buildBundles(*ctx.rev, kind, d.nrow, d.rowPtr, d.col, d.val, rowsCap, innerCap, tpr);
-        if (!tf.ok || !tr.ok) {
-            if (ctx.layout == 2) return ERR_INCOMPATIBILITY;  // bundles required
-            fallback = std::make_unique<WaveFn>();
-            return fallback->init(d, ctx);
-        }
-        int e;
-        if ((e = df.load(std::move(tf)))) return e;
-        if ((e = dr.load(std::move(tr)))) return e;
-        for (int p = 0; p < df.h.numPhases(); ++p)
-            if (df.lds(p) > kMaxLds) return ERR_INTERNAL;
-        for (int p = 0; p < dr.h.numPhases(); ++p)
-            if (dr.lds(p) > kMaxLds) return ERR_INTERNAL;
-        return OK;
-    }
-
-    template <int BS, int TPR, int EE>
-    void launch(const Dev& d, int b0, int nbnd, int p, hipStream_t st) {
-        if (kind == RACE_K_KACZ)
-            hipLaunchKernelGGL((k_kacz_bundle<BS, TPR, EE>), dim3(nbnd), dim3(BS), d.lds(p), st, d.slot.p,
-                               d.wave.p, d.off.p, d.len.p, d.waveSlot.p, d.rowId.p, d.rowNorm.p,
-                               d.colSet.p, d.colSetPtr.p, d.scol.p, d.sval.p, (double*)x,
-                               (const double*)b, n, nrhs, b0, d.h.phCols[p]);
-        else if (nrhs == 1)
-            hipLaunchKernelGGL((k_gs_bundle<BS, TPR, EE, true>), dim3(nbnd), dim3(BS), d.lds(p), st,
-                               d.slot.p, d.wave.p, d.off.p, d.len.p, d.waveSlot.p, d.rowId.p, d.diag.p,
-                               d.scol.p, d.sval.p, (double*)x, (const double*)b, n, nrhs, b0,
-                               d.h.phRows[p]);
-        else
-            hipLaunchKernelGGL((k_gs_bundle<BS, TPR, EE, false>), dim3(nbnd), dim3(BS), d.lds(p), st,
-                               d.slot.p, d.wave.p, d.off.p, d.len.p, d.waveSlot.p, d.rowId.p, d.diag.p,
-                               d.scol.p, d.sval.p, (double*)x, (const double*)b, n, nrhs, b0,
-                               d.h.phRows[p]);
-    }
-    template <int BS>
-    void launchBs(const Dev& d, int b0, int nbnd, int p, hipStream_t st) {
-        switch (tpr * 100 + E) {
-            case 104: launch<BS, 1, 4>(d, b0, nbnd, p, st); break;
-            case 108: launch<BS, 1, 8>(d, b0, nbnd, p, st); break;
-            case 116: launch<BS, 1, 16>(d, b0, nbnd, p, st); break;
-            case 132: launch<BS, 1, 32>(d, b0, nbnd, p, st); break;
-            case 204: launch<BS, 2, 4>(d, b0, nbnd, p, st); break;
-            case 208: launch<BS, 2, 8>(d, b0, nbnd, p, st); break;
-            case 213: launch<BS, 2, 13>(d, b0, nbnd, p, st); break;
-            case 216: launch<BS, 2, 16>(d, b0, nbnd, p, st); break;
-            case 402: launch<BS, 4, 2>(d, b0, nbnd, p, st); break;
-            case 404: launch<BS, 4, 4>(d, b0, nbnd, p, st); break;
-            case 407: launch<BS, 4, 7>(d, b0, nbnd, p, st); break;
-            default: launch<BS, 4, 8>(d, b0, nbnd, p, st); break;
-        }
-    }
-    int waves(bool rev) const override {
-        if (fallback) return fallback->waves(rev);
-        return (int)(rev ? dr.h.critical : df.h.critical);
-    }
-    int run(bool rev, hipStream_t st) override {
-        if (fallback) {
-            fallback->x = x;
-            fallback->b = b;
-            return fallback->run(rev, st);
-        }
-        const Dev& d = rev ? dr : df;
-        const std::vector<int>& pb = d.h.phaseBundle;
-        for (int p = 0; p + 1 < (int)pb.size(); ++p) {
-            const int b0 = pb[p], nbnd = pb[p + 1] - pb[p];
-            if (nbnd <= 0) continue;
-            switch (bs) {
-                case 64: launchBs<64>(d, b0, nbnd, p, st); break;
-                case 128: launchBs<128>(d, b0, nbnd, p, st); break;
-                case 512: launchBs<512>(d, b0, nbnd, p, st); break;
-                case 1024: launchBs<1024>(d, b0, nbnd, p, st); break;
-                default: launchBs<256>(d, b0, nbnd, p, st); break;
-            }
-        }
-        RACE_HIP(hipGetLastError());
-        return OK;
-    }
-};
 
 class SymmSpmvFn : public DeviceFn {
   public:
@@ -723,6 +391,13 @@ class MpkFn : public SpmvFn {
 
 }  // namespace
 
+int makeWaveFn(const race_kernel_desc& d, const ExecContext& ctx, std::unique_ptr<DeviceFn>* out) {
+    auto f = std::make_unique<WaveFn>();
+    int e = f->init(d, ctx);
+    *out = std::move(f);
+    return e;
+}
+
 namespace {
 // plain streaming copy (16-byte vectors), used as the byte-counter reference
 __global__ void __launch_bounds__(kBlock) k_copy(const int4* __restrict__ src,
@@ -743,7 +418,11 @@ int deviceCopy(void* dst, const void* src, size_t bytes, hipStream_t st) {
 
 int makeDeviceFn(const race_kernel_desc& d, const ExecContext& ctx, std::unique_ptr<DeviceFn>* out) {
     if (d.nrow <= 0 || !d.rowPtr || !d.col || !d.val) return ERR_INVALID_ARG;
-    if (d.val_precision != RACE_FP64 || d.vec_precision != RACE_FP64) return ERR_NOT_IMPLEMENTED;
+    const bool fp64 = d.val_precision == RACE_FP64 && d.vec_precision == RACE_FP64;
+    const bool bundleKind = d.kind == RACE_K_GS || d.kind == RACE_K_KACZ;
+    if (!fp64 && !(bundleKind && ctx.layout != 1)) return ERR_NOT_IMPLEMENTED;
+    if (d.vec_precision != RACE_FP64 && d.vec_precision != RACE_FP32) return ERR_INVALID_ARG;
+    if (d.val_precision < RACE_FP64 || d.val_precision > RACE_BF16) return ERR_INVALID_ARG;
     int e = OK;
     switch (d.kind) {
         case RACE_K_GS:
@@ -754,9 +433,7 @@ int makeDeviceFn(const race_kernel_desc& d, const ExecContext& ctx, std::unique_
                 e = f->init(d, ctx);
                 *out = std::move(f);
             } else {
-                auto f = std::make_unique<BundleFn>();
-                e = f->init(d, ctx, ctx.bundleRows, ctx.bundleInner);
-                *out = std::move(f);
+                e = makeBundleFn(d, ctx, out);
             }
             break;
         }
@@ -782,6 +459,7 @@ int makeDeviceFn(const race_kernel_desc& d, const ExecContext& ctx, std::unique_
         default:
             return ERR_INVALID_ARG;
     }
+    if (*out) (*out)->vecPrecision = d.vec_precision;
     return e;
 }
 

@@ -34,6 +34,7 @@ class DeviceFn {
     void* b = nullptr;
     int power = 1;
     int kind = 0;
+    int vecPrecision = 0;  // RACE_FP64 or RACE_FP32 vectors
 };
 
 struct ExecContext {
@@ -41,13 +42,23 @@ struct ExecContext {
     const Schedule* rev = nullptr;
     const MPKPartition* mpk = nullptr;
     int layout = 0;           // 0: leaf bundles (SELL, LDS waves), 1: one workgroup per leaf
-    int bundleRows = 128;     // rows per bundle
-    int bundleInner = 4096;   // LDS-staged entries per bundle
-    int bundleThreads = 256;  // workgroup size of the bundle kernels (64, 128 or 256)
+    int bundleRows = 256;     // rows per bundle
+    int bundleLocal = 6144;   // LDS columns (local column space) per bundle
+    int xcdOrder = 0;         // map consecutive bundles to the same XCD
+    int bundleThreads = 512;  // workgroup size of the bundle kernels (128 ... 1024)
     int bundleTpr = 0;        // lanes per row in the bundle kernels (0: from the row length)
 };
 
 int makeDeviceFn(const race_kernel_desc& d, const ExecContext& ctx, std::unique_ptr<DeviceFn>* out);
+// Gauss-Seidel / Kaczmarz over leaf bundles (bundle_exec.hip)
+int makeBundleFn(const race_kernel_desc& d, const ExecContext& ctx, std::unique_ptr<DeviceFn>* out);
+// FP64 iterative refinement around an inner GS/KACZ function (refine.hip)
+int makeRefineFn(const race_kernel_desc& d, DeviceFn* inner, int sweeps, std::unique_ptr<DeviceFn>* out);
+int refineSolve(DeviceFn* f, double tol, int maxOuter, hipStream_t st, int* outer, long* sweeps,
+                double* relres);
+int refineResidual(DeviceFn* f, hipStream_t st, double* relres);
+// Gauss-Seidel / Kaczmarz with one workgroup per leaf (exec.hip)
+int makeWaveFn(const race_kernel_desc& d, const ExecContext& ctx, std::unique_ptr<DeviceFn>* out);
 // streaming device copy, bytes a multiple of 16
 int deviceCopy(void* dst, const void* src, size_t bytes, hipStream_t st);
 

@@ -1,0 +1,232 @@
+// refine.hip -- FP64 iterative refinement around a low-precision sweep.
+//
+// One refinement step: r = b - A x in FP64 (and ||r||, ||b|| by reduction),
+// d = 0, `sweeps` forward+backward sweeps of the inner Gauss-Seidel or
+// Kaczmarz function on A d = r in the inner function's precision, x += d in
+// FP64.  FP64 is used only for the residual and the update (and the
+// Kaczmarz row norms inside the inner function).  With an FP64 inner
+// function the step is exactly `sweeps` plain symmetric sweeps on A x = b
+// (the sweep is affine, so the correction form gives the same iterate),
+// which is how the plain FP64 path is run for comparison.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "exec.hpp"
+#include "exec_common.hpp"
+
+namespace race {
+
+namespace {
+
+template <typename T, int TPR>
+__global__ void __launch_bounds__(kBlock) k_residual(const int* __restrict__ rp,
+                                                     const int* __restrict__ col,
+                                                     const double* __restrict__ val,
+                                                     const double* __restrict__ x,
+                                                     const double* __restrict__ b, T* __restrict__ r,
+                                                     int n, int nrhs, double* __restrict__ norms) {
+    __shared__ double sr[kBlock / TPR], sb[kBlock / TPR];
+    const int lane = threadIdx.x % TPR, grp = threadIdx.x / TPR;
+    const int rowsPerPass = gridDim.x * (blockDim.x / TPR);
+    double rr = 0.0, bb = 0.0;
+    // grid-stride over rows: a bounded grid keeps the norm atomics few
+    for (int row = (blockIdx.x * blockDim.x + threadIdx.x) / TPR; row < n; row += rowsPerPass) {
+        const int s = rp[row], e = rp[row + 1];
+        for (int k = 0; k < nrhs; ++k) {
+            const size_t o = (size_t)k * n;
+            double t = 0.0;
+            for (int j = s + lane; j < e; j += TPR) t += val[j] * x[o + col[j]];
+            t = groupSum<TPR>(t);
+            const double bv = b[o + row];
+            const double rv = bv - t;
+            if (lane == 0) {
+                r[o + row] = (T)rv;
+                rr += rv * rv;
+                bb += bv * bv;
+            }
+        }
+    }
+    if (lane == 0) {
+        sr[grp] = rr;
+        sb[grp] = bb;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0, c = 0.0;
+        for (int g = 0; g < kBlock / TPR; ++g) {
+            a += sr[g];
+            c += sb[g];
+        }
+        atomicAdd(&norms[0], a);
+        atomicAdd(&norms[1], c);
+    }
+}
+
+template <typename T>
+__global__ void k_zero(T* __restrict__ v, long n) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) v[i] = T(0);
+}
+
+template <typename T>
+__global__ void k_update(double* __restrict__ x, const T* __restrict__ d, long n) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) x[i] += (double)d[i];
+}
+
+class RefineFn : public DeviceFn {
+  public:
+    int n = 0, nrhs = 1, tpr = 8, sweeps = 1;
+    long nnz = 0;
+    DevBuf<int> rp, col;
+    DevBuf<double> val, norms, r64, d64;
+    DevBuf<float> r32, d32;
+    DeviceFn* inner = nullptr;
+    double* hostNorms = nullptr;
+    double lastRelres = -1;
+    long totalSweeps = 0;
+
+    ~RefineFn() override {
+        if (hostNorms) (void)hipHostFree(hostNorms);
+    }
+
+    int init(const race_kernel_desc& d, DeviceFn* in, int innerSweeps) {
+        if (!in || (in->kind != RACE_K_GS && in->kind != RACE_K_KACZ) || innerSweeps < 1)
+            return ERR_INVALID_ARG;
+        if (d.val_precision != RACE_FP64 || d.vec_precision != RACE_FP64) return ERR_INVALID_ARG;
+        kind = d.kind;
+        inner = in;
+        sweeps = innerSweeps;
+        n = d.nrow;
+        nrhs = d.nrhs > 0 ? d.nrhs : 1;
+        x = d.x;
+        b = d.b;
+        nnz = d.rowPtr[n];
+        tpr = pickTpr(nnz, n);
+        int e;
+        if ((e = upload(rp, d.rowPtr, (size_t)n + 1))) return e;
+        if ((e = upload(col, d.col, (size_t)nnz))) return e;
+        if ((e = upload(val, d.val, (size_t)nnz))) return e;
+        RACE_HIP(norms.reserve(2));
+        const size_t len = (size_t)n * nrhs;
+        if (inner->vecPrecision == RACE_FP32) {
+            RACE_HIP(r32.reserve(len));
+            RACE_HIP(d32.reserve(len));
+            inner->x = d32.p;
+            inner->b = r32.p;
+        } else {
+            RACE_HIP(r64.reserve(len));
+            RACE_HIP(d64.reserve(len));
+            inner->x = d64.p;
+            inner->b = r64.p;
+        }
+        RACE_HIP(hipHostMalloc(reinterpret_cast<void**>(&hostNorms), 2 * sizeof(double)));
+        return OK;
+    }
+
+    template <int TPR>
+    void residualKernel(hipStream_t st) {
+        const long threads = std::min((long)n * TPR, 4096l * kBlock);
+        if (inner->vecPrecision == RACE_FP32)
+            hipLaunchKernelGGL((k_residual<float, TPR>), dim3(gridFor(threads)), dim3(kBlock), 0, st, rp.p,
+                               col.p, val.p, (const double*)x, (const double*)b, r32.p, n, nrhs, norms.p);
+        else
+            hipLaunchKernelGGL((k_residual<double, TPR>), dim3(gridFor(threads)), dim3(kBlock), 0, st, rp.p,
+                               col.p, val.p, (const double*)x, (const double*)b, r64.p, n, nrhs, norms.p);
+    }
+
+    // r = b - A x; relres = ||r|| / ||b|| when requested (synchronises)
+    int residual(hipStream_t st, double* relres) {
+        hipLaunchKernelGGL(k_zero<double>, dim3(1), dim3(64), 0, st, norms.p, 2l);
+        switch (tpr) {
+            case 4: residualKernel<4>(st); break;
+            case 8: residualKernel<8>(st); break;
+            case 16: residualKernel<16>(st); break;
+            default: residualKernel<32>(st); break;
+        }
+        RACE_HIP(hipGetLastError());
+        if (relres) {
+            RACE_HIP(hipMemcpyAsync(hostNorms, norms.p, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+            RACE_HIP(hipStreamSynchronize(st));
+            *relres = hostNorms[1] > 0 ? std::sqrt(hostNorms[0] / hostNorms[1]) : std::sqrt(hostNorms[0]);
+            lastRelres = *relres;
+        }
+        return OK;
+    }
+
+    // correction sweeps on A d = r and x += d (residual already in r)
+    int correct(hipStream_t st) {
+        const long len = (long)n * nrhs;
+        const unsigned g = gridFor(len);
+        if (inner->vecPrecision == RACE_FP32)
+            hipLaunchKernelGGL(k_zero<float>, dim3(g), dim3(kBlock), 0, st, d32.p, len);
+        else
+            hipLaunchKernelGGL(k_zero<double>, dim3(g), dim3(kBlock), 0, st, d64.p, len);
+        for (int s = 0; s < sweeps; ++s) {
+            int e = inner->run(false, st);
+            if (e) return e;
+            e = inner->run(true, st);
+            if (e) return e;
+        }
+        totalSweeps += sweeps;
+        if (inner->vecPrecision == RACE_FP32)
+            hipLaunchKernelGGL(k_update<float>, dim3(g), dim3(kBlock), 0, st, (double*)x, d32.p, len);
+        else
+            hipLaunchKernelGGL(k_update<double>, dim3(g), dim3(kBlock), 0, st, (double*)x, d64.p, len);
+        RACE_HIP(hipGetLastError());
+        return OK;
+    }
+
+    // one refinement step (asynchronous)
+    int run(bool, hipStream_t st) override {
+        int e = residual(st, nullptr);
+        if (e) return e;
+        return correct(st);
+    }
+
+    int solve(double tol, int maxOuter, hipStream_t st, int* outer, double* relres) {
+        double rr = 0;
+        int it = 0;
+        int e = residual(st, &rr);
+        if (e) return e;
+        while (rr > tol && it < maxOuter) {
+            if ((e = correct(st))) return e;
+            ++it;
+            if ((e = residual(st, &rr))) return e;
+        }
+        if (outer) *outer = it;
+        if (relres) *relres = rr;
+        return OK;
+    }
+};
+
+}  // namespace
+
+int makeRefineFn(const race_kernel_desc& d, DeviceFn* inner, int sweeps, std::unique_ptr<DeviceFn>* out) {
+    auto f = std::make_unique<RefineFn>();
+    int e = f->init(d, inner, sweeps);
+    *out = std::move(f);
+    return e;
+}
+
+int refineSolve(DeviceFn* f, double tol, int maxOuter, hipStream_t st, int* outer, long* sweeps,
+                double* relres) {
+    RefineFn* r = dynamic_cast<RefineFn*>(f);
+    if (!r) return ERR_INVALID_ARG;
+    const long before = r->totalSweeps;
+    int e = r->solve(tol, maxOuter, st, outer, relres);
+    if (sweeps) *sweeps = r->totalSweeps - before;
+    return e;
+}
+
+int refineResidual(DeviceFn* f, hipStream_t st, double* relres) {
+    RefineFn* r = dynamic_cast<RefineFn*>(f);
+    if (!r) return ERR_INVALID_ARG;
+    return r->residual(st, relres);
+}
+
+}  // namespace race

@@ -177,6 +177,32 @@ class Interface:
         L.check(self._L.race_register_device(self.h, ctypes.byref(d), ctypes.byref(fid)), "registerDevice")
         return fid.value
 
+    def registerRefinement(self, kind, rowPtr, col, val, x, b, inner_id, sweeps=1, nrhs=1):
+        """FP64 iterative refinement around the registered function inner_id;
+        (rowPtr, col, val) is the full FP64 matrix, x and b FP64 device vectors."""
+        rp, c = _i32(rowPtr), _i32(col)
+        v = np.ascontiguousarray(val, dtype=np.float64)
+        d = L.KernelDesc(kind=int(kind), nrow=self.nrow, rowPtr=_ip(rp), col=_ip(c),
+                         val=v.ctypes.data_as(L.f64p), val_precision=FP64, vec_precision=FP64,
+                         nrhs=int(nrhs), x=x.ptr if isinstance(x, DeviceArray) else x,
+                         b=b.ptr if isinstance(b, DeviceArray) else b, power=1)
+        fid = ctypes.c_int()
+        L.check(self._L.race_register_refinement(self.h, ctypes.byref(d), int(inner_id), int(sweeps),
+                                                 ctypes.byref(fid)), "registerRefinement")
+        return fid.value
+
+    def refine(self, fid, tol, max_outer=1000):
+        """run refinement steps to ||b-Ax||/||b|| <= tol: (steps, inner sweeps, relres)"""
+        o, sw, rr = ctypes.c_int(), ctypes.c_long(), ctypes.c_double()
+        L.check(self._L.race_refine(self.h, fid, float(tol), int(max_outer), ctypes.byref(o),
+                                    ctypes.byref(sw), ctypes.byref(rr)), "refine")
+        return o.value, sw.value, rr.value
+
+    def residual(self, fid):
+        rr = ctypes.c_double()
+        L.check(self._L.race_residual(self.h, fid, ctypes.byref(rr)), "residual")
+        return rr.value
+
     def setOption(self, key, value):
         L.check(self._L.race_set_option(self.h, key.encode(), int(value)), f"setOption({key})")
 

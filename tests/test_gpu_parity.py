@@ -133,7 +133,7 @@ def test_kacz_matches_race_order(oracle, T, layout):
     ce.setOption("layout", layout)
     if layout == 2:
         ce.setOption("bundle_rows", 256)
-        ce.setOption("bundle_inner", 2048)
+        ce.setOption("bundle_local", 2048)
     fid = ce.registerDevice(R.K_KACZ, prp, pcol, pval, dx, db)
     xo = np.zeros(n)
     for rev in (False, True):
