@@ -9,8 +9,9 @@ matrix, with the matrix and vectors already resident in HBM.
 
 Prints one JSON line: GB/s of algorithmic traffic (per sweep: nnz*(8+4) +
 n*(4 + 3*8) bytes for FP64), the roofline of the Gauss-Seidel kernel against
-the 8 TB/s HBM peak, and the CPU oracle (the same sweeps in RACE order,
-single thread) timed on this host beside it.
+the 8 TB/s HBM peak, the CPU oracle (the same sweeps in RACE order, single
+thread) timed on this host beside it, and the time to a relative residual of
+1e-10 of FP64 iterative refinement around FP64, FP32 and BF16 inner sweeps.
 """
 import argparse
 import json
@@ -45,6 +46,9 @@ def main():
     ap.add_argument("--threads", type=int, default=int(os.environ.get("RACE_BENCH_THREADS", "16384")))
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--inner-sweeps", type=int, default=4)
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="run a plain copy kernel of this many MiB first (byte-counter reference)")
     args = ap.parse_args()
@@ -109,6 +113,35 @@ def main():
     launches_per_step = 2 * (len(pp) - 1)
     log(f"GPU: {ms_step:.3f} ms/step, {gbs:.1f} GB/s, {launches_per_step} launches/step")
 
+    # mixed precision vs FP64 to the same residual: FP64 iterative refinement
+    # around inner symmetric GS sweeps in FP64 / FP32 / BF16 storage
+    mixed = None
+    if not args.no_mixed:
+        A = M.to_scipy(prp, pcol, pval)
+        xs = M.uniform_vec(n, 99)
+        bs = A @ xs
+        mixed = {"tol": args.tol, "inner_sweeps": args.inner_sweeps}
+        for name, vp, xp, dt in (("fp64", R.FP64, R.FP64, np.float64), ("fp32", R.FP32, R.FP32, np.float32),
+                                 ("bf16", R.BF16, R.FP32, np.float32)):
+            wx, wb = R.DeviceArray(n, dt), R.DeviceArray(n, dt)
+            inner = ce.registerDevice(R.K_GS, prp, pcol, pval, wx, wb, val_precision=vp, vec_precision=xp)
+            xd = R.DeviceArray.from_host(np.zeros(n))
+            bd = R.DeviceArray.from_host(bs)
+            rid = ce.registerRefinement(R.K_GS, prp, pcol, pval, xd, bd, inner, sweeps=args.inner_sweeps)
+            ce.refine(rid, 1e-2, max_outer=1)  # warm-up step
+            xd.upload(np.zeros(n))
+            t0 = time.time()
+            steps, sweeps, rr = ce.refine(rid, args.tol, max_outer=2000)
+            tt = time.time() - t0
+            err = float(np.linalg.norm(xd.download() - xs) / np.linalg.norm(xs))
+            mixed[name] = {"seconds": round(tt, 5), "refinements": steps, "sweeps": sweeps,
+                           "relres": rr, "rel_err_vs_solution": err}
+            log(f"refinement {name}: {tt * 1e3:.1f} ms, {steps} steps, {sweeps} sweeps, relres {rr:.2e}, "
+                f"err {err:.2e}")
+            del wx, wb, xd, bd
+        mixed["speedup_fp32_vs_fp64"] = round(mixed["fp64"]["seconds"] / mixed["fp32"]["seconds"], 3)
+        mixed["speedup_bf16_vs_fp64"] = round(mixed["fp64"]["seconds"] / mixed["bf16"]["seconds"], 3)
+
     # CPU oracle beside it: the same sweeps in RACE order, single thread
     cpu = None
     if not args.no_cpu:
@@ -161,6 +194,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(gbs, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 5), "traffic": traffic},
         "cpu_baseline": cpu,
+        "mixed_precision": mixed,
     }
     print(json.dumps(out), flush=True)
 
