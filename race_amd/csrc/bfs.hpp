@@ -32,6 +32,9 @@ class GpuTraverser : public Traverser {
     int init(int n, const int* rowPtr, const int* col, const int* initPerm,
              const int* initInvPerm, hipStream_t stream);
     int traverse(int dist, int lo, int hi, LevelData* out) override;
+    // disjoint ranges, traversed concurrently (one workgroup per range)
+    int traverseBatch(int dist, const std::vector<std::pair<int, int>>& ranges,
+                      std::vector<LevelData>* outs) override;
     const std::vector<int>& perm() const override { return hPerm; }
     const std::vector<int>& invPerm() const override { return hInv; }
     int rowLen(int origRow) const override { return hRowLen[origRow]; }
@@ -40,11 +43,19 @@ class GpuTraverser : public Traverser {
     // device permutation (perm[new] = old)
     int* devPerm() const { return dPerm.p; }
     int* devInvPerm() const { return dInv.p; }
+    // ranges of at most this many rows are traversed by one workgroup each
+    // (batched); larger ranges level by level over the whole device
+    int rangeBlockMax = 1 << 17;
     double lastTraverseMs = 0;
     long traversals = 0;
 
   private:
+    int traverseLevels(int dist, int lo, int hi, LevelData* out);
     hipStream_t st = nullptr;
+    int stampNext = 0, bvisitSlots = 0;
+    DevBuf<int> dBVisit, dBatchQ, dLvRow, dLvNnz, dLvStart, dPackRow, dPackNnz, dNLev;
+    DevBuf<unsigned char> dJobs;
+    std::vector<int> hNLev, hPackRow, hPackNnz;
     std::vector<int> hPerm, hInv, hRowLen;
     DevBuf<int> dRowPtr, dCol, dPerm, dInv, dVisit, dDist, dQ0, dQ1, dCount;
     DevBuf<int> dLevelRow, dLevelNnz, dKeys, dKeysOut, dVals, dValsOut, dTmp;

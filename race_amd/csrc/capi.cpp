@@ -89,6 +89,7 @@ struct race_handle_s {
     std::vector<Function> funcs;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int optLayout = 0, optBundleRows = 256, optBundleLocal = 6144, optBundleThreads = 512, optBundleTpr = 0, optXcd = 0;
+    int optBfsBlockMax = 1 << 17;
     ~race_handle_s() {
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
@@ -166,6 +167,7 @@ int race_destroy(race_handle h) {
 static int ensureTraverser(race_handle h) {
     if (h->trav) return OK;
     h->trav = std::make_unique<GpuTraverser>();
+    h->trav->rangeBlockMax = h->optBfsBlockMax;
     return h->trav->init(h->nrow, h->rowPtr.data(), h->col.data(),
                          h->initPerm.empty() ? nullptr : h->initPerm.data(),
                          h->initInv.empty() ? nullptr : h->initInv.data(), h->stream);
@@ -341,6 +343,10 @@ int race_set_option(race_handle h, const char* key, int value) {
     } else if (k == "xcd_order") {
         if (value != 0 && value != 1) return ERR_INVALID_ARG;
         h->optXcd = value;
+    } else if (k == "bfs_block_max") {
+        if (value < 0) return ERR_INVALID_ARG;
+        h->optBfsBlockMax = value;
+        if (h->trav) h->trav->rangeBlockMax = value;
     } else {
         return ERR_INVALID_ARG;
     }

@@ -434,15 +434,29 @@ void LevelRecursion::idealNthreads(int parent, int parentSub, int currLvl) {
 int LevelRecursion::recursivePartition(int parent, int parentSub, int currLvl) {
     idealNthreads(parent, parentSub, currLvl);
     const std::vector<int> ch = zt->tree[parent].children;
+    // The ranges traversed below are fixed here (valueZ and idealNthreadsZ
+    // of these children change no more) and disjoint, and a traversal
+    // depends only on the rows of its range, so all of them are traversed
+    // up front as one batch; the loop then consumes the results in order.
+    std::vector<std::pair<int, int>> ranges;
+    for (int ci = ch[2 * parentSub]; ci < ch[2 * parentSub + 1]; ++ci) {
+        if (zt->tree[ci].idealNthreadsZ <= 1) continue;
+        const std::vector<int>& rng = zt->tree[ci].valueZ;
+        for (size_t j = 0; j + 1 < rng.size(); ++j)
+            if (rng[j + 1] > rng[j]) ranges.push_back({rng[j], rng[j + 1]});
+    }
+    std::vector<LevelData> lds;
+    int berr = trav->traverseBatch(dist, ranges, &lds);
+    if (berr != OK) return berr;
+    size_t next = 0;
     for (int ci = ch[2 * parentSub]; ci < ch[2 * parentSub + 1]; ++ci) {
         const int nt = zt->tree[ci].idealNthreadsZ;
         const std::vector<int> rng = zt->tree[ci].valueZ;
         if (nt <= 1) continue;
         for (size_t j = 0; j + 1 < rng.size(); ++j) {
             if (rng[j + 1] > rng[j]) {
-                LevelData ld;
-                int err = trav->traverse(dist, rng[j], rng[j + 1], &ld);
-                if (err != OK) return err;
+                const LevelData& ld = lds[next++];
+                int err;
                 bool ok = zt->spawnChild(ci, (int)j, nt, ld, efficiency(currLvl));
                 if (!ok) {
                     const std::vector<int>& cc = zt->tree[ci].children;

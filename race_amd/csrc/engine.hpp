@@ -101,6 +101,17 @@ class Traverser {
   public:
     virtual ~Traverser() {}
     virtual int traverse(int dist, int lo, int hi, LevelData* out) = 0;  // Err
+    // disjoint ranges whose rows are final: same result as traversing them
+    // one after another in the given order
+    virtual int traverseBatch(int dist, const std::vector<std::pair<int, int>>& ranges,
+                              std::vector<LevelData>* outs) {
+        outs->assign(ranges.size(), LevelData());
+        for (size_t r = 0; r < ranges.size(); ++r) {
+            int e = traverse(dist, ranges[r].first, ranges[r].second, &(*outs)[r]);
+            if (e) return e;
+        }
+        return 0;
+    }
     virtual const std::vector<int>& perm() const = 0;     // host mirror, perm[new]=old
     virtual const std::vector<int>& invPerm() const = 0;  // host mirror
     virtual int rowLen(int origRow) const = 0;            // graph row length

@@ -33,15 +33,24 @@ def graphs():
     return out
 
 
-def _iface(rp, col, T, dist, symm=True):
-    return R.Interface(len(rp) - 1, T, dist, rp, col, symm_hint=symm)
+def _iface(rp, col, T, dist, symm=True, bfs_block_max=None):
+    ce = R.Interface(len(rp) - 1, T, dist, rp, col, symm_hint=symm)
+    if bfs_block_max is not None:
+        ce.setOption("bfs_block_max", bfs_block_max)
+    return ce
 
 
+# traversal paths: default (ranges up to 2^17 rows in one workgroup),
+# 0 = every range level by level over the device, 200 = both, by range size
+BFS_PATHS = [None, 0, 200]
+
+
+@pytest.mark.parametrize("bmax", BFS_PATHS)
 @pytest.mark.parametrize("dist", [R.ONE, R.TWO])
-def test_stage0_perm_matches_oracle(oracle, dist):
+def test_stage0_perm_matches_oracle(oracle, dist, bmax):
     for name, rp, col in graphs():
-        print("case", name, dist, flush=True)
-        ce = _iface(rp, col, 1, dist)
+        print("case", name, dist, bmax, flush=True)
+        ce = _iface(rp, col, 1, dist, bfs_block_max=bmax)
         ce.RACEColor()
         g = oracle.Graph(rp, col, symm_hint=True)
         g.traverse(dist, 0, g.n)
@@ -59,12 +68,13 @@ def test_spin_symmetrized_perm(oracle, spin):
         assert np.array_equal(ce.getPerm(), g.perm)
 
 
+@pytest.mark.parametrize("bmax", BFS_PATHS)
 @pytest.mark.parametrize("T", [2, 3, 8, 20, 64])
 @pytest.mark.parametrize("dist", [R.ONE, R.TWO])
-def test_recursion_tree_and_schedule(oracle, T, dist):
+def test_recursion_tree_and_schedule(oracle, T, dist, bmax):
     for name, rp, col in graphs()[:2]:
-        print("case", name, T, dist, flush=True)
-        ce = _iface(rp, col, T, dist)
+        print("case", name, T, dist, bmax, flush=True)
+        ce = _iface(rp, col, T, dist, bfs_block_max=bmax)
         ce.RACEColor()
         g = oracle.Graph(rp, col, symm_hint=True)
         zt = oracle.level_recursion(g, T, dist)
@@ -161,6 +171,19 @@ def test_symmspmv(oracle, T):
     got = db.download()
     assert relerr(got, bo) <= TOL
     assert relerr(got, full) <= TOL
+
+
+@pytest.mark.parametrize("bmax", BFS_PATHS)
+def test_mpk_partition_all_graphs(oracle, bmax):
+    """POWER traversals (adjacent island levels) and level groups, every graph."""
+    for name, rp, col in graphs():
+        print("case", name, bmax, flush=True)
+        ce = _iface(rp, col, 1, R.POWER, bfs_block_max=bmax)
+        ce.RACEColor(3, 16 * 1024.0)
+        g = oracle.Graph(rp, col, symm_hint=True)
+        part = oracle.mpk_partition(g, 3, 16 * 1024.0)
+        assert np.array_equal(ce.getPerm(), g.perm), name
+        assert np.array_equal(ce.mpkGroups(), part["groupPtr"]), name
 
 
 def test_mpk_groups_and_powers(oracle):
