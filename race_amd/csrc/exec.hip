@@ -358,37 +358,6 @@ class SpmvFn : public DeviceFn {
     }
 };
 
-// Level-blocked matrix powers: level group g at power p runs after groups
-// g-1, g, g+1 at power p-1 (BFS levels only touch neighbouring levels).  The
-// launch order is the reference's trapezoidal wavefront: step L runs powers
-// 1..k on groups L, L-1, ..., L-k+1 (src/functionManager.cpp:618-822).
-class MpkFn : public SpmvFn {
-  public:
-    std::vector<int> groupPtr;
-    int init(const race_kernel_desc& d, const ExecContext& ctx) {
-        int e = SpmvFn::init(d);
-        if (e) return e;
-        if (!ctx.mpk) return ERR_INVALID_ARG;
-        groupPtr = ctx.mpk->groupPtr;
-        power = d.power > 0 ? d.power : 1;
-        nrhs = 1;
-        return OK;
-    }
-    int run(bool, hipStream_t st) override {
-        const int G = (int)groupPtr.size() - 1;
-        const long n = A.n;
-        double* xx = (double*)x;
-        for (int L = 0; L < G + power - 1; ++L)
-            for (int p = 0; p < power; ++p) {
-                const int g = L - p;
-                if (g < 0 || g >= G) continue;
-                rows(groupPtr[g], groupPtr[g + 1], (long)p * n, (long)(p + 1) * n, xx, xx, st);
-            }
-        RACE_HIP(hipGetLastError());
-        return OK;
-    }
-};
-
 }  // namespace
 
 int makeWaveFn(const race_kernel_desc& d, const ExecContext& ctx, std::unique_ptr<DeviceFn>* out) {
@@ -445,17 +414,18 @@ int makeDeviceFn(const race_kernel_desc& d, const ExecContext& ctx, std::unique_
             break;
         }
         case RACE_K_SPMV: {
+            if (d.nrhs <= 1) {
+                e = makeSellSpmvFn(d, out);
+                break;
+            }
             auto f = std::make_unique<SpmvFn>();
             e = f->init(d);
             *out = std::move(f);
             break;
         }
-        case RACE_K_MPK: {
-            auto f = std::make_unique<MpkFn>();
-            e = f->init(d, ctx);
-            *out = std::move(f);
+        case RACE_K_MPK:
+            e = makeMpkFn(d, ctx, out);
             break;
-        }
         default:
             return ERR_INVALID_ARG;
     }

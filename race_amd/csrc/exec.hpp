@@ -59,6 +59,9 @@ int refineSolve(DeviceFn* f, double tol, int maxOuter, hipStream_t st, int* oute
 int refineResidual(DeviceFn* f, hipStream_t st, double* relres);
 // Gauss-Seidel / Kaczmarz with one workgroup per leaf (exec.hip)
 int makeWaveFn(const race_kernel_desc& d, const ExecContext& ctx, std::unique_ptr<DeviceFn>* out);
+// SELL-C SpMV (one right-hand side) and level-blocked matrix powers (sell.hip)
+int makeSellSpmvFn(const race_kernel_desc& d, std::unique_ptr<DeviceFn>* out);
+int makeMpkFn(const race_kernel_desc& d, const ExecContext& ctx, std::unique_ptr<DeviceFn>* out);
 // streaming device copy, bytes a multiple of 16
 int deviceCopy(void* dst, const void* src, size_t bytes, hipStream_t st);
 

@@ -1,0 +1,227 @@
+// sell.hip -- SELL-C matrices for the row-parallel kernels: SpMV and the
+// level-blocked matrix powers (MPK).
+//
+// A chunk is up to 64 consecutive rows (one wavefront, one row per lane);
+// its entries are stored column-major, entry j of lane i at off + 64 j + i,
+// so every load instruction of a wave reads one contiguous 512-byte line of
+// values (256 of indices).  Rows shorter than the chunk's widest row are
+// padded with value 0 on the row's own column.  Chunks never cross a "cut"
+// row (the level-group boundaries for MPK), so a row range made of whole
+// groups is a contiguous range of chunks.  Rows keep their order (sigma = 1:
+// the level ordering is what the MPK blocking relies on).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "exec.hpp"
+#include "exec_common.hpp"
+
+namespace race {
+
+namespace {
+
+constexpr int kC = 64;               // rows per chunk (one wavefront)
+constexpr int kWavesPerBlock = kBlock / kC;
+
+struct SellChunk {
+    int row0, rows, width, pad;
+    long off;
+};
+
+struct SellMatrix {
+    int n = 0, nChunk = 0;
+    long entries = 0;
+    std::vector<int> cutChunk;  // chunk index of every cut
+    DevBuf<SellChunk> chunks;
+    DevBuf<int> col;
+    DevBuf<double> val;
+
+    // cuts: sorted row boundaries starting with 0 and ending with n
+    int build(const race_kernel_desc& d, const std::vector<int>& cuts) {
+        n = d.nrow;
+        const int* rp = d.rowPtr;
+        std::vector<SellChunk> ch;
+        cutChunk.clear();
+        long total = 0;
+        for (size_t k = 0; k + 1 < cuts.size(); ++k) {
+            cutChunk.push_back((int)ch.size());
+            for (int r0 = cuts[k]; r0 < cuts[k + 1]; r0 += kC) {
+                const int rows = std::min(kC, cuts[k + 1] - r0);
+                int w = 0;
+                for (int i = 0; i < rows; ++i) w = std::max(w, rp[r0 + i + 1] - rp[r0 + i]);
+                ch.push_back(SellChunk{r0, rows, w, 0, total});
+                total += (long)w * kC;
+            }
+        }
+        cutChunk.push_back((int)ch.size());
+        nChunk = (int)ch.size();
+        entries = total;
+        std::vector<int> hc((size_t)total);
+        std::vector<double> hv((size_t)total);
+        for (const SellChunk& c : ch)
+            for (int i = 0; i < kC; ++i) {
+                const int r = c.row0 + std::min(i, c.rows - 1);
+                const int len = i < c.rows ? rp[r + 1] - rp[r] : 0;
+                for (int j = 0; j < c.width; ++j) {
+                    const long pos = c.off + (long)j * kC + i;
+                    if (j < len) {
+                        hc[pos] = d.col[rp[r] + j];
+                        hv[pos] = d.val[rp[r] + j];
+                    } else {
+                        hc[pos] = r;
+                        hv[pos] = 0.0;
+                    }
+                }
+            }
+        int e;
+        if ((e = upload(chunks, ch.data(), ch.size()))) return e;
+        if ((e = upload(col, hc.data(), hc.size()))) return e;
+        if ((e = upload(val, hv.data(), hv.size()))) return e;
+        return OK;
+    }
+};
+
+// y[row] = sum_j a_j x[c_j] for the rows of one chunk; sequential in j, as
+// the CRS row order
+__device__ __forceinline__ double sellRow(const SellChunk& k, int lane, const int* __restrict__ col,
+                                          const double* __restrict__ val, const double* __restrict__ x) {
+    const int* __restrict__ cc = col + k.off + lane;
+    const double* __restrict__ vv = val + k.off + lane;
+    double t = 0.0;
+    int j = 0;
+    for (; j + 4 <= k.width; j += 4) {
+        const int c0 = cc[(j + 0) * kC], c1 = cc[(j + 1) * kC], c2 = cc[(j + 2) * kC], c3 = cc[(j + 3) * kC];
+        const double v0 = vv[(j + 0) * kC], v1 = vv[(j + 1) * kC], v2 = vv[(j + 2) * kC], v3 = vv[(j + 3) * kC];
+        const double x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
+        t += v0 * x0;
+        t += v1 * x1;
+        t += v2 * x2;
+        t += v3 * x3;
+    }
+    for (; j < k.width; ++j) t += vv[j * kC] * x[cc[j * kC]];
+    return t;
+}
+
+__global__ void __launch_bounds__(kBlock) k_sell_spmv(const SellChunk* __restrict__ chunks, int nChunk,
+                                                      const int* __restrict__ col, const double* __restrict__ val,
+                                                      const double* __restrict__ x, double* __restrict__ y) {
+    const int c = blockIdx.x * kWavesPerBlock + threadIdx.x / kC;
+    const int lane = threadIdx.x % kC;
+    if (c >= nChunk) return;
+    const SellChunk k = chunks[c];
+    if (lane >= k.rows) return;
+    y[k.row0 + lane] = sellRow(k, lane, col, val, x);
+}
+
+// One wavefront step of the matrix powers: segment s computes power q_s on
+// chunks [c0, c1) (whole level groups) from power q_s - 1.
+struct MpkSeg {
+    int c0, c1, q, blockStart;
+};
+
+__global__ void __launch_bounds__(kBlock) k_sell_mpk_step(const SellChunk* __restrict__ chunks,
+                                                          const int* __restrict__ col,
+                                                          const double* __restrict__ val, double* __restrict__ x,
+                                                          long n, const MpkSeg* __restrict__ segs, int nseg) {
+    int s = 0;
+    while (s + 1 < nseg && (int)blockIdx.x >= segs[s + 1].blockStart) ++s;
+    const MpkSeg sg = segs[s];
+    const int c = sg.c0 + ((int)blockIdx.x - sg.blockStart) * kWavesPerBlock + (int)threadIdx.x / kC;
+    const int lane = threadIdx.x % kC;
+    if (c >= sg.c1) return;
+    const SellChunk k = chunks[c];
+    if (lane >= k.rows) return;
+    const double* __restrict__ xin = x + (long)(sg.q - 1) * n;
+    x[(long)sg.q * n + k.row0 + lane] = sellRow(k, lane, col, val, xin);
+}
+
+// b = A x, one right-hand side
+class SellSpmvFn : public DeviceFn {
+  public:
+    SellMatrix A;
+    int init(const race_kernel_desc& d) {
+        kind = d.kind;
+        x = d.x;
+        b = d.b;
+        return A.build(d, {0, d.nrow});
+    }
+    int run(bool, hipStream_t st) override {
+        const unsigned g = (unsigned)std::max(1, (A.nChunk + kWavesPerBlock - 1) / kWavesPerBlock);
+        hipLaunchKernelGGL(k_sell_spmv, dim3(g), dim3(kBlock), 0, st, A.chunks.p, A.nChunk, A.col.p, A.val.p,
+                           (const double*)x, (double*)b);
+        RACE_HIP(hipGetLastError());
+        return OK;
+    }
+};
+
+// Level-blocked matrix powers x_q = A x_{q-1}, q = 1..k, x_q at x + q n.
+// Level group g at power q reads power q - 1 of groups g - 1, g, g + 1
+// (BFS levels only touch neighbouring levels): the dependency rule of the
+// reference's trapezoidal wavefront (src/functionManager.cpp:618-822).  Step
+// L computes power q on group L - 2 (q - 1); its inputs were written by steps
+// L - 3 .. L - 1, so the segments of a step are independent and run as one
+// launch.  A step touches at most 2k consecutive level groups, which the
+// cache model (src/matrixPower.cpp:88-140, 685-940) sized to stay in the
+// Infinity Cache, so A is read from HBM about once per call.
+class MpkFn : public DeviceFn {
+  public:
+    SellMatrix A;
+    std::vector<int> stepSeg, stepBlocks;  // segments stepSeg[L] .. stepSeg[L+1] of step L
+    DevBuf<MpkSeg> segs;
+    int init(const race_kernel_desc& d, const ExecContext& ctx) {
+        kind = d.kind;
+        x = d.x;
+        b = d.b;
+        if (!ctx.mpk) return ERR_INVALID_ARG;
+        power = d.power > 0 ? d.power : 1;
+        const std::vector<int>& gp = ctx.mpk->groupPtr;
+        if (gp.size() < 2 || gp.front() != 0 || gp.back() != d.nrow) return ERR_INVALID_ARG;
+        int e;
+        if ((e = A.build(d, gp))) return e;
+        const int G = (int)gp.size() - 1;
+        std::vector<MpkSeg> all;
+        stepSeg.assign(1, 0);
+        stepBlocks.clear();
+        for (int L = 0; L < G + 2 * (power - 1); ++L) {
+            int blocks = 0;
+            for (int q = 1; q <= power; ++q) {
+                const int g = L - 2 * (q - 1);
+                if (g < 0 || g >= G) continue;
+                const int c0 = A.cutChunk[g], c1 = A.cutChunk[g + 1];
+                if (c1 <= c0) continue;
+                all.push_back(MpkSeg{c0, c1, q, blocks});
+                blocks += (c1 - c0 + kWavesPerBlock - 1) / kWavesPerBlock;
+            }
+            stepSeg.push_back((int)all.size());
+            stepBlocks.push_back(blocks);
+        }
+        return upload(segs, all.data(), all.size());
+    }
+    int run(bool, hipStream_t st) override {
+        for (int L = 0; L + 1 < (int)stepSeg.size(); ++L) {
+            if (stepBlocks[L] == 0) continue;
+            hipLaunchKernelGGL(k_sell_mpk_step, dim3(stepBlocks[L]), dim3(kBlock), 0, st, A.chunks.p, A.col.p,
+                               A.val.p, (double*)x, (long)A.n, segs.p + stepSeg[L], stepSeg[L + 1] - stepSeg[L]);
+        }
+        RACE_HIP(hipGetLastError());
+        return OK;
+    }
+};
+
+}  // namespace
+
+int makeSellSpmvFn(const race_kernel_desc& d, std::unique_ptr<DeviceFn>* out) {
+    auto f = std::make_unique<SellSpmvFn>();
+    int e = f->init(d);
+    *out = std::move(f);
+    return e;
+}
+
+int makeMpkFn(const race_kernel_desc& d, const ExecContext& ctx, std::unique_ptr<DeviceFn>* out) {
+    auto f = std::make_unique<MpkFn>();
+    int e = f->init(d, ctx);
+    *out = std::move(f);
+    return e;
+}
+
+}  // namespace race

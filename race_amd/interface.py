@@ -88,6 +88,7 @@ class Interface:
         self._L = L.load()
         self.nrow = int(nrow)
         self._keep = []
+        self._vecs = {}
         rp, c = _i32(rowPtr), _i32(col)
         ip = _i32(initPerm) if initPerm is not None else None
         iip = _i32(initInvPerm) if initInvPerm is not None else None
@@ -175,6 +176,8 @@ class Interface:
                          b=(b.ptr if isinstance(b, DeviceArray) else b), power=int(power))
         fid = ctypes.c_int()
         L.check(self._L.race_register_device(self.h, ctypes.byref(d), ctypes.byref(fid)), "registerDevice")
+        # device arrays stay alive while a registered function may use them
+        self._vecs[fid.value] = (x, b)
         return fid.value
 
     def registerRefinement(self, kind, rowPtr, col, val, x, b, inner_id, sweeps=1, nrhs=1):
@@ -189,6 +192,7 @@ class Interface:
         fid = ctypes.c_int()
         L.check(self._L.race_register_refinement(self.h, ctypes.byref(d), int(inner_id), int(sweeps),
                                                  ctypes.byref(fid)), "registerRefinement")
+        self._vecs[fid.value] = (x, b)
         return fid.value
 
     def refine(self, fid, tol, max_outer=1000):
@@ -209,6 +213,7 @@ class Interface:
     def setVectors(self, fid, x, b=None):
         L.check(self._L.race_set_vectors(self.h, fid, x.ptr if isinstance(x, DeviceArray) else x,
                                          b.ptr if isinstance(b, DeviceArray) else b), "setVectors")
+        self._vecs[fid] = (x, b)
 
     def executeFunction(self, fid, rev=False):
         L.check(self._L.race_execute(self.h, int(fid), int(bool(rev))), "executeFunction")

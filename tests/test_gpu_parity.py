@@ -214,6 +214,27 @@ def test_mpk_groups_and_powers(oracle):
         assert relerr(got[p], ref[p]) <= TOL, p
 
 
+def test_registered_vectors_stay_alive(oracle):
+    """A device array passed to registerDevice without another reference
+    must live as long as the registration (the kernel writes through it)."""
+    import gc
+    rp, col = M.stencil7(8)
+    val = M.laplace7_values(rp, col)
+    n = len(rp) - 1
+    ce = _iface(rp, col, 1, R.ONE)
+    ce.RACEColor()
+    x = M.uniform_vec(n, 3)
+    dx = R.DeviceArray.from_host(x)
+    fid = ce.registerDevice(R.K_SPMV, rp, col, val, dx, R.DeviceArray(n))
+    gc.collect()
+    kept = ce._vecs.get(fid)
+    if kept is None or kept[1] is None or not kept[1].ptr:
+        pytest.fail("registered output array was not retained")
+    ce.executeFunction(fid)
+    ref = M.to_scipy(rp, col, val) @ x
+    assert relerr(kept[1].download(), ref) <= TOL
+
+
 def test_host_callback_runs_race_order(oracle):
     """registerFunction with a CPU callback: GS through the host executor."""
     rp, col = M.stencil27(8)
