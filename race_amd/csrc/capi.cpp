@@ -89,7 +89,7 @@ struct race_handle_s {
     std::vector<Function> funcs;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int optLayout = 0, optBundleRows = 256, optBundleLocal = 6144, optBundleThreads = 512, optBundleTpr = 0, optXcd = 0;
-    int optBfsBlockMax = 1 << 17;
+    int optBfsBlockMax = 1 << 17, optSymmMode = 0;
     ~race_handle_s() {
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
@@ -314,6 +314,7 @@ int race_register_device(race_handle h, const race_kernel_desc* d, int* id) {
     ctx.xcdOrder = h->optXcd;
     ctx.bundleThreads = h->optBundleThreads;
     ctx.bundleTpr = h->optBundleTpr;
+    ctx.symmMode = h->optSymmMode;
     Function fn;
     int e = makeDeviceFn(*d, ctx, &fn.dev);
     if (e) return e;
@@ -343,6 +344,9 @@ int race_set_option(race_handle h, const char* key, int value) {
     } else if (k == "xcd_order") {
         if (value != 0 && value != 1) return ERR_INVALID_ARG;
         h->optXcd = value;
+    } else if (k == "symm_mode") {
+        if (value < 0 || value > 3) return ERR_INVALID_ARG;
+        h->optSymmMode = value;
     } else if (k == "bfs_block_max") {
         if (value < 0) return ERR_INVALID_ARG;
         h->optBfsBlockMax = value;

@@ -47,6 +47,7 @@ struct ExecContext {
     int xcdOrder = 0;         // map consecutive bundles to the same XCD
     int bundleThreads = 512;  // workgroup size of the bundle kernels (128 ... 1024)
     int bundleTpr = 0;        // lanes per row in the bundle kernels (0: from the row length)
+    int symmMode = 0;         // SymmSpMV: 0 auto, 1 colour phases, 2 SELL atomics, 3 expanded
 };
 
 int makeDeviceFn(const race_kernel_desc& d, const ExecContext& ctx, std::unique_ptr<DeviceFn>* out);
@@ -62,6 +63,8 @@ int makeWaveFn(const race_kernel_desc& d, const ExecContext& ctx, std::unique_pt
 // SELL-C SpMV (one right-hand side) and level-blocked matrix powers (sell.hip)
 int makeSellSpmvFn(const race_kernel_desc& d, std::unique_ptr<DeviceFn>* out);
 int makeMpkFn(const race_kernel_desc& d, const ExecContext& ctx, std::unique_ptr<DeviceFn>* out);
+// single-launch SymmSpMV: mode 2 SELL atomics, mode 3 expanded full matrix (sell.hip)
+int makeSymmSellFn(const race_kernel_desc& d, int mode, std::unique_ptr<DeviceFn>* out);
 // streaming device copy, bytes a multiple of 16
 int deviceCopy(void* dst, const void* src, size_t bytes, hipStream_t st);
 

@@ -113,6 +113,59 @@ __global__ void __launch_bounds__(kBlock) k_sell_spmv(const SellChunk* __restric
     y[k.row0 + lane] = sellRow(k, lane, col, val, x);
 }
 
+// b += A x for nrhs right-hand sides (vectors column-major, v[r + k n])
+__global__ void __launch_bounds__(kBlock) k_sell_spmv_acc(const SellChunk* __restrict__ chunks, int nChunk,
+                                                          const int* __restrict__ col,
+                                                          const double* __restrict__ val,
+                                                          const double* __restrict__ x, double* __restrict__ y,
+                                                          long n, int nrhs) {
+    const int c = blockIdx.x * kWavesPerBlock + threadIdx.x / kC;
+    const int lane = threadIdx.x % kC;
+    if (c >= nChunk) return;
+    const SellChunk k = chunks[c];
+    if (lane >= k.rows) return;
+    for (int q = 0; q < nrhs; ++q) y[q * n + k.row0 + lane] += sellRow(k, lane, col, val, x + q * n);
+}
+
+// SymmSpMV over the upper triangle (diagonal included) in one pass:
+// b[r] += a_rr x_r + sum_{c>r} a_rc x_c and b[c] += a_rc x_r, the update of
+// SYMM_SPMV_KERNEL (examples/src/helpers/kernels.cpp:403-427).  Rows of
+// different leaves of a distance-2 colour never write the same b entry, but
+// rows inside a leaf do, and all rows run at once here: every write to b is
+// a device-scope FP64 atomic add.  Padding entries (value 0 on the row's
+// own column) add nothing.
+__global__ void __launch_bounds__(kBlock) k_sell_symm_atomic(const SellChunk* __restrict__ chunks, int nChunk,
+                                                             const int* __restrict__ col,
+                                                             const double* __restrict__ val,
+                                                             const double* __restrict__ x, double* b, long n,
+                                                             int nrhs) {
+    const int c = blockIdx.x * kWavesPerBlock + threadIdx.x / kC;
+    const int lane = threadIdx.x % kC;
+    if (c >= nChunk) return;
+    const SellChunk k = chunks[c];
+    if (lane >= k.rows) return;
+    const int r = k.row0 + lane;
+    const int* __restrict__ cc = col + k.off + lane;
+    const double* __restrict__ vv = val + k.off + lane;
+    for (int q = 0; q < nrhs; ++q) {
+        const double* __restrict__ xq = x + q * n;
+        double* bq = b + q * n;
+        const double xr = xq[r];
+        double t = 0.0;
+        for (int j = 0; j < k.width; ++j) {
+            const int cj = cc[j * kC];
+            const double v = vv[j * kC];
+            if (cj == r) {
+                t += v * xr;
+            } else {
+                t += v * xq[cj];
+                atomicAdd(&bq[cj], v * xr);
+            }
+        }
+        atomicAdd(&bq[r], t);
+    }
+}
+
 // One wavefront step of the matrix powers: segment s computes power q_s on
 // chunks [c0, c1) (whole level groups) from power q_s - 1.
 struct MpkSeg {
@@ -149,6 +202,66 @@ class SellSpmvFn : public DeviceFn {
         const unsigned g = (unsigned)std::max(1, (A.nChunk + kWavesPerBlock - 1) / kWavesPerBlock);
         hipLaunchKernelGGL(k_sell_spmv, dim3(g), dim3(kBlock), 0, st, A.chunks.p, A.nChunk, A.col.p, A.val.p,
                            (const double*)x, (double*)b);
+        RACE_HIP(hipGetLastError());
+        return OK;
+    }
+};
+
+// SymmSpMV b += A x from the upper triangle, in one launch:
+//  mode 2: SELL-C upper triangle, FP64 atomics for every write to b;
+//  mode 3: the full matrix (upper triangle and its transpose) built once at
+//          registration, SELL-C SpMV accumulating into b (no atomics, reads
+//          the strict upper part twice).
+class SymmSellFn : public DeviceFn {
+  public:
+    SellMatrix A;
+    int mode = 2, nrhs = 1;
+    int init(const race_kernel_desc& d, int mode_) {
+        kind = d.kind;
+        x = d.x;
+        b = d.b;
+        mode = mode_;
+        nrhs = d.nrhs > 0 ? d.nrhs : 1;
+        const int n = d.nrow;
+        if (mode == 2) return A.build(d, {0, n});
+        // full pattern: row r holds its upper entries, then the transposed
+        // entries (c, r) of rows c < r, each in increasing c
+        std::vector<int> cnt(n + 1, 0);
+        for (int r = 0; r < n; ++r)
+            for (int j = d.rowPtr[r]; j < d.rowPtr[r + 1]; ++j) {
+                ++cnt[r + 1];
+                if (d.col[j] != r) ++cnt[d.col[j] + 1];
+            }
+        for (int r = 0; r < n; ++r) cnt[r + 1] += cnt[r];
+        std::vector<int> rp(cnt), fc(cnt[n]);
+        std::vector<double> fv(cnt[n]);
+        std::vector<int> pos(cnt.begin(), cnt.end() - 1);
+        for (int r = 0; r < n; ++r)
+            for (int j = d.rowPtr[r]; j < d.rowPtr[r + 1]; ++j) {
+                fc[pos[r]] = d.col[j];
+                fv[pos[r]++] = d.val[j];
+            }
+        for (int r = 0; r < n; ++r)
+            for (int j = d.rowPtr[r]; j < d.rowPtr[r + 1]; ++j) {
+                const int c = d.col[j];
+                if (c == r) continue;
+                fc[pos[c]] = r;
+                fv[pos[c]++] = d.val[j];
+            }
+        race_kernel_desc f = d;
+        f.rowPtr = rp.data();
+        f.col = fc.data();
+        f.val = fv.data();
+        return A.build(f, {0, n});
+    }
+    int run(bool, hipStream_t st) override {
+        const unsigned g = (unsigned)std::max(1, (A.nChunk + kWavesPerBlock - 1) / kWavesPerBlock);
+        if (mode == 2)
+            hipLaunchKernelGGL(k_sell_symm_atomic, dim3(g), dim3(kBlock), 0, st, A.chunks.p, A.nChunk, A.col.p,
+                               A.val.p, (const double*)x, (double*)b, (long)A.n, nrhs);
+        else
+            hipLaunchKernelGGL(k_sell_spmv_acc, dim3(g), dim3(kBlock), 0, st, A.chunks.p, A.nChunk, A.col.p,
+                               A.val.p, (const double*)x, (double*)b, (long)A.n, nrhs);
         RACE_HIP(hipGetLastError());
         return OK;
     }
@@ -213,6 +326,13 @@ class MpkFn : public DeviceFn {
 int makeSellSpmvFn(const race_kernel_desc& d, std::unique_ptr<DeviceFn>* out) {
     auto f = std::make_unique<SellSpmvFn>();
     int e = f->init(d);
+    *out = std::move(f);
+    return e;
+}
+
+int makeSymmSellFn(const race_kernel_desc& d, int mode, std::unique_ptr<DeviceFn>* out) {
+    auto f = std::make_unique<SymmSellFn>();
+    int e = f->init(d, mode);
     *out = std::move(f);
     return e;
 }

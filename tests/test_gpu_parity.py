@@ -152,16 +152,20 @@ def test_kacz_matches_race_order(oracle, T, layout):
         assert relerr(dx.download(), xo) <= TOL
 
 
+# symm_mode: 0 default, 1 colour phases, 2 SELL atomics, 3 expanded full matrix
+@pytest.mark.parametrize("mode", [0, 1, 2, 3])
 @pytest.mark.parametrize("T", [1, 8, 64])
-def test_symmspmv(oracle, T):
+def test_symmspmv(oracle, T, mode):
     rp, col = M.stencil7(16)
-    val = M.laplace7_values(rp, col)
+    val = M.jittered_values(rp, col, 0.1, seed=5)
     n = len(rp) - 1
     ce, (prp, pcol, pval), zt = _prepared(rp, col, val, T, R.TWO, oracle)
     urp, ucol, uval = M.upper_triangle(prp, pcol, pval)
     x = M.uniform_vec(n, 1234)
     dx = R.DeviceArray.from_host(x)
     db = R.DeviceArray.from_host(np.zeros(n))
+    if mode:
+        ce.setOption("symm_mode", mode)
     fid = ce.registerDevice(R.K_SYMMSPMV, urp, ucol, uval, dx, db)
     ce.executeFunction(fid)
     bo = np.zeros(n)
