@@ -124,6 +124,12 @@ int race_register_device(race_handle h, const race_kernel_desc *desc, int *id);
  *   "xcd_order"    1 = consecutive bundles of a phase run on the same XCD (default 0)
  *   "bundle_threads" workgroup size of the bundle kernels: 128, 256, 512 (default) or 1024
  *   "bundle_tpr"   lanes per matrix row in the bundle kernels: 1, 2, 4 (0 = auto)
+ *   "symm_mode"    SymmSpMV: 1 = colour phases (CRS, one thread per row),
+ *                  2 = one launch over a SELL-C upper triangle, FP64 atomics,
+ *                  3 = one launch over the expanded full matrix (default, 0)
+ *   "bfs_block_max" ranges of at most this many rows are traversed one
+ *                  workgroup each, batched (default 131072; 0 = every range
+ *                  level by level over the whole device); set before colouring
  * No counterpart in the reference (its executor has no layout choice). */
 int race_set_option(race_handle h, const char *key, int value);
 /* re-point a registered device function at other vectors */
