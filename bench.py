@@ -10,8 +10,10 @@ matrix, with the matrix and vectors already resident in HBM.
 Prints one JSON line: GB/s of algorithmic traffic (per sweep: nnz*(8+4) +
 n*(4 + 3*8) bytes for FP64), the roofline of the Gauss-Seidel kernel against
 the 8 TB/s HBM peak, the CPU oracle (the same sweeps in RACE order, single
-thread) timed on this host beside it, and the time to a relative residual of
-1e-10 of FP64 iterative refinement around FP64, FP32 and BF16 inner sweeps.
+thread) timed on this host beside it, the time to a relative residual of
+1e-10 of FP64 iterative refinement around FP64, FP32 and BF16 inner sweeps,
+and ("other_kernels") SymmSpMV on the 7-pt 64^3 grid (hot and cold caches)
+and the level-blocked matrix powers, k = 8, on the 7-pt 256^3 grid.
 """
 import argparse
 import json
@@ -37,6 +39,98 @@ def gs_bytes(n, nnz, sv=8, sx=8, si=4):
     return nnz * (sv + si) + n * (si + 3 * sx)
 
 
+def flush_copy(R, a, b):
+    """1 GiB device copy: evicts the L2s and the 256 MB Infinity Cache."""
+    R.interface.L.check(R.interface.L.load().race_device_copy(b.ptr, a.ptr, a.nbytes), "copy")
+
+
+def bench_symmspmv(R, M, g=64, threads=64, reps=200):
+    """SymmSpMV b += A x, 7-pt g^3 (diag 6, off -1), upper triangle with the
+    diagonal first, RACE distance-2 colouring.  Hot: back-to-back calls
+    (working set in the caches); cold: each call after a 1 GiB copy."""
+    rp, col = M.stencil7(g)
+    val = M.laplace7_values(rp, col)
+    n = len(rp) - 1
+    ce = R.Interface(n, threads, R.TWO, rp, col, symm_hint=True)
+    ce.RACEColor()
+    perm, inv = ce.getPerm(), ce.getInvPerm()
+    prp, pcol, pval = M.permute(rp, col, val, perm, inv)
+    prp, pcol, pval = M.make_diag_first(prp, pcol, pval)
+    urp, ucol, uval = M.upper_triangle(prp, pcol, pval)
+    nnzu = int(urp[-1])
+    byts = nnzu * 12 + n * (4 + 3 * 8) + 4
+    dx = R.DeviceArray.from_host(M.uniform_vec(n, 1234))
+    db = R.DeviceArray.from_host(np.zeros(n))
+    fid = ce.registerDevice(R.K_SYMMSPMV, urp, ucol, uval, dx, db)
+    for _ in range(5):
+        ce.executeAsync(fid)
+    ce.timerBegin()
+    for _ in range(reps):
+        ce.executeAsync(fid)
+    hot = ce.timerEnd() / reps
+    fa, fb = R.DeviceArray(1 << 27), R.DeviceArray(1 << 27)
+    cold = []
+    for _ in range(20):
+        flush_copy(R, fa, fb)
+        ce.timerBegin()
+        ce.executeAsync(fid)
+        cold.append(ce.timerEnd())
+    cold = float(np.median(cold))
+    out = {"workload": f"SymmSpMV 7-pt {g}^3 (n={n}, nnz_upper={nnzu}), distance-2 colouring, "
+                       f"{threads} RACE threads, FP64",
+           "bytes_per_call": byts, "hot_us": round(hot * 1e3, 2), "hot_gbs": round(byts / hot / 1e6, 1),
+           "cold_us": round(cold * 1e3, 2), "cold_gbs": round(byts / cold / 1e6, 1),
+           "cold_frac_hbm": round(byts / cold / 1e6 / HBM_PEAK_GBS, 4)}
+    log(f"SymmSpMV {g}^3: hot {out['hot_us']} us ({out['hot_gbs']} GB/s), cold {out['cold_us']} us "
+        f"({out['cold_gbs']} GB/s)")
+    return out
+
+
+def bench_mpk(R, M, g=256, k=8, cache_mb=512.0, reps=10):
+    """Matrix powers x_q = A x_{q-1}, q = 1..k, 7-pt g^3 (diag 6, off -1,
+    jitter 10 %), RACE level groups for a cache of cache_mb MB, safety 2."""
+    rp, col = M.stencil7(g)
+    val = M.jittered_values(rp, col, 0.1, seed=9)
+    n = len(rp) - 1
+    nnz = int(rp[-1])
+    ce = R.Interface(n, 1, R.POWER, rp, col, symm_hint=True)
+    ce.RACEColor(k, cache_mb * 1e6)
+    groups = len(ce.mpkGroups()) - 1
+    perm, inv = ce.getPerm(), ce.getInvPerm()
+    prp, pcol, pval = M.permute(rp, col, val, perm, inv)
+    del rp, col, val
+    xs = np.zeros((k + 1) * n)
+    xs[:n] = M.uniform_vec(n, 2)
+    dx = R.DeviceArray.from_host(xs)
+    del xs
+    fid = ce.registerDevice(R.K_MPK, prp, pcol, pval, dx, power=k)
+    for _ in range(2):
+        ce.executeAsync(fid)
+    ce.timerBegin()
+    for _ in range(reps):
+        ce.executeAsync(fid)
+    ms = ce.timerEnd() / reps
+    eff = k * (nnz * 12 + n * (4 + 2 * 8))
+    floor = nnz * 12 + n * 4 + (k + 1) * n * 8
+    # the same k products as k separate SpMVs, for comparison
+    dy = R.DeviceArray(n)
+    sid = ce.registerDevice(R.K_SPMV, prp, pcol, pval, dx, dy)
+    ce.executeAsync(sid)
+    ce.timerBegin()
+    for _ in range(reps):
+        ce.executeAsync(sid)
+    ms1 = ce.timerEnd() / reps
+    out = {"workload": f"MPK 7-pt {g}^3 (n={n}, nnz={nnz}), k={k}, cache {cache_mb:.0f} MB, "
+                       f"{groups} level groups, FP64",
+           "ms_per_call": round(ms, 4), "effective_gbs": round(eff / ms / 1e6, 1),
+           "effective_frac_hbm": round(eff / ms / 1e6 / HBM_PEAK_GBS, 4),
+           "compulsory_gbs": round(floor / ms / 1e6, 1), "spmv_ms": round(ms1, 4),
+           "spmv_gbs": round((nnz * 12 + n * 20) / ms1 / 1e6, 1), "speedup_vs_k_spmv": round(k * ms1 / ms, 3)}
+    log(f"MPK {g}^3 k={k}: {ms:.3f} ms/call, effective {out['effective_gbs']} GB/s; "
+        f"SpMV {ms1:.3f} ms ({out['spmv_gbs']} GB/s)")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -49,6 +143,8 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--inner-sweeps", type=int, default=4)
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the SymmSpMV 64^3 and MPK 256^3 measurements")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="run a plain copy kernel of this many MiB first (byte-counter reference)")
     args = ap.parse_args()
@@ -142,6 +238,10 @@ def main():
         mixed["speedup_fp32_vs_fp64"] = round(mixed["fp64"]["seconds"] / mixed["fp32"]["seconds"], 3)
         mixed["speedup_bf16_vs_fp64"] = round(mixed["fp64"]["seconds"] / mixed["bf16"]["seconds"], 3)
 
+    extra = None
+    if not args.no_extra:
+        extra = {"symmspmv": bench_symmspmv(R, M), "mpk": bench_mpk(R, M)}
+
     # CPU oracle beside it: the same sweeps in RACE order, single thread
     cpu = None
     if not args.no_cpu:
@@ -195,6 +295,7 @@ def main():
                      "frac": round(gbs / HBM_PEAK_GBS, 5), "traffic": traffic},
         "cpu_baseline": cpu,
         "mixed_precision": mixed,
+        "other_kernels": extra,
     }
     print(json.dumps(out), flush=True)
 
