@@ -188,6 +188,33 @@ int race_lb_split(const int *level_row, const int *level_nnz, int nlevels, int n
                   double efficiency, int dist, int d2_method, int lb_target, int cap,
                   int *nblocks, int *zone_rows, int *level_ptr, int *nthreads_out, int *scale);
 
+/* ---- SELL-C-sigma helpers ----
+ * Entry j of row r of a SELL-C-sigma matrix sits at
+ * chunk_start[r / C] + r % C + j * C; row_len[r] is the row length,
+ * chunk_len[c] the (padded) length of chunk c.  val_precision is RACE_FP64
+ * (val is double*) or RACE_FP32 (float*). */
+/* sell_c_sigmize (src/sell_c_sigmize.cpp:48-62): padding slot j >= row_len[r]
+ * gets column start + (r - start) % simd_width, start = first row of the
+ * colouring leaf that holds r (the reference runs it as a RACE callback).
+ * Needs a coloured handle (RACEColor()). */
+int race_sell_c_sigmize(race_handle h, int simd_width, int C, int *col, const int *chunk_start,
+                        const int *row_len, const int *chunk_len);
+/* Interface::simdify (src/interface.cpp:824-832, include/simdify.h:70-357):
+ * sell_c_sigmize, then entries are swapped inside rows so that no two rows of
+ * a SIMD group (simd_width consecutive rows of a chunk) hold the same column
+ * at one position.  RACE_ERR_INCOMPATIBILITY where that is impossible (the
+ * reference returns false). */
+int race_simdify(race_handle h, int simd_width, int C, int nrows, int *col, const int *chunk_start,
+                 const int *row_len, const int *chunk_len, void *val, int val_precision, int diag_first);
+/* the rearrangement of race_simdify without the padding step (host only) */
+int race_simdify_resolve(int simd_width, int C, int nrows, int *col, const int *chunk_start,
+                         const int *chunk_len, void *val, int val_precision, int diag_first);
+/* Interface::simdifyD1 (src/interface.cpp:834-842, include/simdify.h:389-447):
+ * the entries of each row on the rows of preceding lanes of its SIMD group
+ * are swapped to the end of the row (host only). */
+int race_simdify_d1(int simd_width, int C, int nrows, int *col, const int *chunk_start,
+                    const int *chunk_len, void *val, int val_precision, int dist2_compatible);
+
 /* ---- device memory helpers (for callers without a HIP toolchain) ---- */
 int race_device_malloc(void **p, size_t bytes);
 int race_device_free(void *p);

@@ -158,6 +158,30 @@ class Interface {
         if (race_check_power_coverage(h_) != RACE_SUCCESS)
             std::fprintf(stderr, "RACE: power coverage check failed\n");
     }
+    // Interface::simdify / simdifyD1 (include/interface.h:179-185,
+    // src/interface.cpp:824-842): SELL-C-sigma rearrangement in place
+    bool simdify(int simdWidth, int C, int nrows, int* col, int* chunkStart, int* rl, int* clp, double* val,
+                 bool diagFirst = false) {
+        return race_simdify(h_, simdWidth, C, nrows, col, chunkStart, rl, clp, val, RACE_FP64, diagFirst ? 1 : 0) ==
+               RACE_SUCCESS;
+    }
+    bool simdify(int simdWidth, int C, int nrows, int* col, int* chunkStart, int* rl, int* clp, float* val,
+                 bool diagFirst = false) {
+        return race_simdify(h_, simdWidth, C, nrows, col, chunkStart, rl, clp, val, RACE_FP32, diagFirst ? 1 : 0) ==
+               RACE_SUCCESS;
+    }
+    bool simdifyD1(int simdWidth, int C, int nrows, int* col, int* chunkStart, int* rl, int* clp, double* val,
+                   bool dist2Compatible = false) {
+        (void)rl;
+        return race_simdify_d1(simdWidth, C, nrows, col, chunkStart, clp, val, RACE_FP64, dist2Compatible ? 1 : 0) ==
+               RACE_SUCCESS;
+    }
+    bool simdifyD1(int simdWidth, int C, int nrows, int* col, int* chunkStart, int* rl, int* clp, float* val,
+                   bool dist2Compatible = false) {
+        (void)rl;
+        return race_simdify_d1(simdWidth, C, nrows, col, chunkStart, clp, val, RACE_FP32, dist2Compatible ? 1 : 0) ==
+               RACE_SUCCESS;
+    }
     race_handle handle() const { return h_; }
 
   private:
@@ -184,5 +208,11 @@ class Interface {
 };
 
 }  // namespace RACE
+
+// sell_c_sigmize (include/sell_c_sigmize.h:41, src/sell_c_sigmize.cpp:48-62)
+inline void sell_c_sigmize(int simdWidth, int C, int* col, int* chunkStart, int* rl, int* clp,
+                           RACE::Interface* ce) {
+    race_sell_c_sigmize(ce->handle(), simdWidth, C, col, chunkStart, rl, clp);
+}
 
 #endif  // RACE_AMD_INTERFACE_HPP

@@ -23,6 +23,11 @@ C restatement in race_oracle.c through ctypes:
                        mtxPower::findPartition/consolidatePartition
                        src/matrixPowerRecursive.cpp:340-445,
                        src/matrixPower.cpp:88-140,505-604,685-940
+  sell_c_sigmize       sell_c_sigmize_Kernel run over the leaf calls
+                       src/sell_c_sigmize.cpp:26-62
+  simdify_resolve      simdifyTemplate after its sell_c_sigmize call
+                       include/simdify.h:70-357
+  simdify_d1           simdifyD1Template include/simdify.h:389-447
 
 Pinning (the reference cannot be built in this image -- its build needs the
 GAP and hwloc packages that are fetched from the network plus a generated
@@ -39,6 +44,14 @@ config.h):
     scheduled leaves is checked independent.
   * mpk_partition (stage 0): every group respects the cache bound, groups
     tile the level range, and MPK through the groups equals repeated SpMV.
+  * sell_c_sigmize / simdify_resolve / simdify_d1: the 4x4 all-equal block
+    of the reference's testSimdify (src/simdify.cpp:41-77) traced by hand
+    to its XOR Latin square; every row keeps its (column, value) multiset;
+    after simdify no two rows of a SIMD group share a column at a position
+    whenever it reports success; after simdify_d1 the entry on the row of
+    lane m sits at position cl-1-m (cl-1-(i-1-m) when dist2Compatible), the
+    picture of include/simdify.h:367-381; padding columns equal leaf start
+    + offset mod width.
 """
 import ctypes
 import os
@@ -704,3 +717,94 @@ def mpk_partition(graph, k, cache_bytes, safety=2.0, cutoff=50):
     glevels.append(total_level)
     return dict(levelPtr=level_ptr, groupPtr=groups, groupLevel=glevels, hopeless=hopeless,
                 levelRow=lr, levelNnz=ln, cache_elem=cache_elem, byte_per_nnz=byte_per_nnz)
+
+
+# ---------------------------------------------------------------------------
+# SELL-C-sigma helpers.  Entry j of row r at chunkStart[r // C] + r % C + j C.
+
+def sell_c_sigmize(calls, simdWidth, C, col, chunkStart, rl, clp):
+    """src/sell_c_sigmize.cpp:26-62 run as a RACE callback: calls are the
+    (start, end) leaf calls of the forward colouring (leaf_calls)."""
+    for start, end in calls:
+        for row in range(start, end):
+            chunk = row // C
+            base = chunkStart[chunk] + row % C
+            for j in range(rl[row], clp[chunk]):
+                col[base + j * C] = start + ((row - start) % simdWidth)
+
+
+def simdify_resolve(simdWidth, C, nrows, col, chunkStart, clp, val, diagFirst=False):
+    """include/simdify.h:70-357 without the sell_c_sigmize call (:78);
+    returns False where the reference returns false."""
+    nChunk = nrows // C
+    simdInChunk = C // simdWidth
+    startingIdx = 1 if diagFirst else 0
+    simdCol = [0] * simdWidth
+    for chunk in range(nChunk + 1):
+        if chunk * simdInChunk * simdWidth >= nrows:
+            break
+        for simdIdx in range(simdInChunk):
+            base = chunkStart[chunk] + simdIdx * simdWidth
+            for j in range(startingIdx, clp[chunk]):
+                row = chunk * simdInChunk * simdWidth + simdIdx * simdWidth
+                i = 0
+                while i < simdWidth and row < nrows:
+                    pos = j                      # currCol points at position pos, lane i
+                    ctr = 0
+                    wrapped = False
+                    repeat = True
+                    while repeat:
+                        repeat = any(col[base + C * pos + i] == simdCol[k] for k in range(i))
+                        if wrapped:
+                            here = col[base + C * j + i]
+                            for k in range(simdWidth):
+                                if k != i and here == col[base + C * pos + k]:
+                                    repeat = True
+                                    break
+                        if repeat:
+                            if ctr < clp[chunk]:
+                                if pos != clp[chunk] - 1:
+                                    pos += 1
+                                else:
+                                    pos = startingIdx
+                                    wrapped = True
+                                ctr += 1
+                            else:
+                                return False
+                    simdCol[i] = col[base + C * pos + i]
+                    if pos != j:
+                        a, b = base + C * j + i, base + C * pos + i
+                        col[a], col[b] = col[b], col[a]
+                        val[a], val[b] = val[b], val[a]
+                    i += 1
+                    row += 1
+    return True
+
+
+def simdify_d1(simdWidth, C, nrows, col, chunkStart, cl, val, dist2Compatible=False):
+    """include/simdify.h:389-447, including its row counter that carries
+    over between SIMD groups."""
+    nChunk = nrows // C
+    simdInChunk = C // simdWidth
+    row = 0
+    for chunk in range(nChunk + 1):
+        for simdIdx in range(simdInChunk):
+            i = 1
+            while i < simdWidth and row < nrows:
+                row = chunk * simdInChunk * simdWidth + simdIdx * simdWidth + i
+                dep = [0] * i
+                for k in range(i):
+                    dep[k if dist2Compatible else i - k - 1] = row - (k + 1)
+                lane = chunkStart[chunk] + simdIdx * simdWidth + i
+                for j in range(cl[chunk]):
+                    for k in range(i):
+                        if col[lane + C * j] == dep[k]:
+                            e = cl[chunk] - (k + 1)
+                            if e != j:
+                                a, b = lane + C * j, lane + C * e
+                                col[a], col[b] = col[b], col[a]
+                                val[a], val[b] = val[b], val[a]
+                i += 1
+                row += 1
+    return True
+

@@ -88,6 +88,13 @@ PROTOTYPES = {
     "race_lb_split": (ctypes.c_int, [i32p, i32p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                      ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, i32p, i32p,
                                      i32p, i32p, i32p]),
+    "race_sell_c_sigmize": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, i32p, i32p, i32p, i32p]),
+    "race_simdify": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, i32p, i32p, i32p, i32p, vp,
+                                    ctypes.c_int, ctypes.c_int]),
+    "race_simdify_resolve": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, i32p, i32p, i32p, vp,
+                                            ctypes.c_int, ctypes.c_int]),
+    "race_simdify_d1": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, i32p, i32p, i32p, vp,
+                                       ctypes.c_int, ctypes.c_int]),
     "race_device_malloc": (ctypes.c_int, [ctypes.POINTER(vp), ctypes.c_size_t]),
     "race_device_free": (ctypes.c_int, [vp]),
     "race_memcpy_h2d": (ctypes.c_int, [vp, vp, ctypes.c_size_t]),

@@ -15,6 +15,7 @@
 
 #include "../../include/race_amd.h"
 #include "bfs.hpp"
+#include "simdify.hpp"
 #include "engine.hpp"
 #include "exec.hpp"
 
@@ -573,6 +574,47 @@ int race_get_preprocess_stats(race_handle h, double* bfs_ms, int* traversals) {
     if (bfs_ms) *bfs_ms = h->trav->lastTraverseMs;
     if (traversals) *traversals = (int)h->trav->traversals;
     return OK;
+}
+
+int race_sell_c_sigmize(race_handle h, int simd_width, int C, int* col, const int* chunk_start,
+                        const int* row_len, const int* chunk_len) {
+    if (checkHandle(h) || !col || !chunk_start || !row_len || !chunk_len || simd_width <= 0 || C <= 0)
+        return ERR_INVALID_ARG;
+    if (h->dist == DIST_POWER || !h->colored) return ERR_INVALID_ARG;
+    sellCSigmize(h->fwd, simd_width, C, col, chunk_start, row_len, chunk_len);
+    return OK;
+}
+
+int race_simdify_resolve(int simd_width, int C, int nrows, int* col, const int* chunk_start,
+                         const int* chunk_len, void* val, int val_precision, int diag_first) {
+    if (!col || !chunk_start || !chunk_len || !val || nrows < 0) return ERR_INVALID_ARG;
+    if (val_precision == RACE_FP64)
+        return simdifyResolve<double>(simd_width, C, nrows, col, chunk_start, chunk_len, (double*)val,
+                                      diag_first != 0);
+    if (val_precision == RACE_FP32)
+        return simdifyResolve<float>(simd_width, C, nrows, col, chunk_start, chunk_len, (float*)val,
+                                     diag_first != 0);
+    return ERR_INVALID_ARG;
+}
+
+int race_simdify(race_handle h, int simd_width, int C, int nrows, int* col, const int* chunk_start,
+                 const int* row_len, const int* chunk_len, void* val, int val_precision, int diag_first) {
+    if (val_precision != RACE_FP64 && val_precision != RACE_FP32) return ERR_INVALID_ARG;
+    int e = race_sell_c_sigmize(h, simd_width, C, col, chunk_start, row_len, chunk_len);
+    if (e) return e;
+    return race_simdify_resolve(simd_width, C, nrows, col, chunk_start, chunk_len, val, val_precision, diag_first);
+}
+
+int race_simdify_d1(int simd_width, int C, int nrows, int* col, const int* chunk_start, const int* chunk_len,
+                    void* val, int val_precision, int dist2_compatible) {
+    if (!col || !chunk_start || !chunk_len || !val || nrows < 0) return ERR_INVALID_ARG;
+    if (val_precision == RACE_FP64)
+        return simdifyD1<double>(simd_width, C, nrows, col, chunk_start, chunk_len, (double*)val,
+                                 dist2_compatible != 0);
+    if (val_precision == RACE_FP32)
+        return simdifyD1<float>(simd_width, C, nrows, col, chunk_start, chunk_len, (float*)val,
+                                dist2_compatible != 0);
+    return ERR_INVALID_ARG;
 }
 
 int race_lb_split(const int* level_row, const int* level_nnz, int nlevels, int nthreads,

@@ -277,10 +277,66 @@ class Interface:
         L.check(self._L.race_timer_end(self.h, ctypes.byref(v)), "timerEnd")
         return float(v.value)
 
+    def sell_c_sigmize(self, simdWidth, C, col, chunkStart, rl, clp):
+        """sell_c_sigmize(simdWidth, C, col, chunkStart, rl, clp, ce) in place."""
+        for a in (col, chunkStart, rl, clp):
+            if not (isinstance(a, np.ndarray) and a.dtype == np.int32 and a.flags.c_contiguous):
+                raise TypeError("SELL arrays must be contiguous int32")
+        L.check(self._L.race_sell_c_sigmize(self.h, int(simdWidth), int(C), _ip(col), _ip(chunkStart), _ip(rl),
+                                            _ip(clp)), "sell_c_sigmize")
+
+    def simdify(self, simdWidth, C, nrows, col, chunkStart, rl, clp, val, diagFirst=False):
+        """Interface::simdify in place: padding by the colouring, then the
+        distance-2 rearrangement.  True on success."""
+        prec = _sell_args(col, chunkStart, clp, val)
+        rl = _i32(rl)
+        st = self._L.race_simdify(self.h, int(simdWidth), int(C), int(nrows), _ip(col), _ip(chunkStart), _ip(rl),
+                                  _ip(clp), val.ctypes.data, prec, int(bool(diagFirst)))
+        if st == 6:
+            return False
+        L.check(st, "simdify")
+        return True
+
+    def simdifyD1(self, simdWidth, C, nrows, col, chunkStart, rl, clp, val, dist2Compatible=False):
+        return simdifyD1(simdWidth, C, nrows, col, chunkStart, rl, clp, val, dist2Compatible)
+
     def preprocessStats(self):
         ms, n = ctypes.c_double(), ctypes.c_int()
         L.check(self._L.race_get_preprocess_stats(self.h, ctypes.byref(ms), ctypes.byref(n)), "stats")
         return ms.value, n.value
+
+
+def _sell_args(col, chunkStart, clp, val):
+    """Checked views for the in-place SELL-C-sigma helpers."""
+    for a, dt in ((col, np.int32), (chunkStart, np.int32), (clp, np.int32)):
+        if not (isinstance(a, np.ndarray) and a.dtype == dt and a.flags.c_contiguous):
+            raise TypeError("col, chunkStart and chunk lengths must be contiguous int32 arrays")
+    if not (isinstance(val, np.ndarray) and val.dtype in (np.float64, np.float32) and val.flags.c_contiguous):
+        raise TypeError("val must be a contiguous float64 or float32 array")
+    if len(val) != len(col):
+        raise ValueError("col and val differ in length")
+    return FP64 if val.dtype == np.float64 else FP32
+
+
+def simdify_resolve(simdWidth, C, nrows, col, chunkStart, clp, val, diagFirst=False):
+    """Distance-2 SIMD rearrangement of a SELL-C-sigma matrix in place (host
+    only; the padding step of Interface.simdify not included).  True on
+    success, False where a SIMD group cannot be resolved."""
+    prec = _sell_args(col, chunkStart, clp, val)
+    st = L.load().race_simdify_resolve(int(simdWidth), int(C), int(nrows), _ip(col), _ip(chunkStart), _ip(clp),
+                                       val.ctypes.data, prec, int(bool(diagFirst)))
+    if st == 6:  # RACE_ERR_INCOMPATIBILITY
+        return False
+    L.check(st, "simdify")
+    return True
+
+
+def simdifyD1(simdWidth, C, nrows, col, chunkStart, rl, clp, val, dist2Compatible=False):
+    """Interface::simdifyD1 in place (host only)."""
+    prec = _sell_args(col, chunkStart, clp, val)
+    L.check(L.load().race_simdify_d1(int(simdWidth), int(C), int(nrows), _ip(col), _ip(chunkStart), _ip(clp),
+                                     val.ctypes.data, prec, int(bool(dist2Compatible))), "simdifyD1")
+    return True
 
 
 def lb_split(levelRow, levelNnz, nthreads, eff, dist, d2=TWO_BLOCK, lb_target=LB_NNZ):

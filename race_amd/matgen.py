@@ -175,3 +175,29 @@ def to_scipy(rowPtr, col, val):
     import scipy.sparse as sp
     n = len(rowPtr) - 1
     return sp.csr_matrix((val, col, rowPtr), shape=(n, n))
+
+
+def to_sell(rowPtr, col, val, C, pad_col=0):
+    """SELL-C arrays (sigma = 1) of a CRS matrix, the layout the SELL-C-sigma
+    helpers take: entry j of row r at chunkStart[r // C] + r % C + j * C.
+    Returns chunkStart, rowLen, chunkLen (widest row of each chunk), col and
+    val of length chunkStart[-1] + chunkLen[-1] * C; padding slots hold
+    column pad_col and value 0."""
+    rowPtr = np.asarray(rowPtr, np.int64)
+    n = len(rowPtr) - 1
+    rl = np.diff(rowPtr).astype(np.int32)
+    nch = (n + C - 1) // C
+    clp = np.zeros(nch, np.int32)
+    for c in range(nch):
+        clp[c] = rl[c * C:(c + 1) * C].max() if n else 0
+    cs = np.zeros(nch + 1, np.int32)
+    cs[1:] = np.cumsum(clp.astype(np.int64) * C)
+    total = int(cs[-1])
+    scol = np.full(total, pad_col, np.int32)
+    sval = np.zeros(total, np.asarray(val).dtype)
+    for r in range(n):
+        base = cs[r // C] + r % C
+        for j in range(rl[r]):
+            scol[base + j * C] = col[rowPtr[r] + j]
+            sval[base + j * C] = val[rowPtr[r] + j]
+    return cs, rl, clp, scol, sval

@@ -271,3 +271,27 @@ def test_cpp_drop_in_program():
     print(p.stdout, p.stderr)
     assert p.returncode == 0
     assert "relerr=" in p.stdout
+
+
+@pytest.mark.parametrize("T,C,w", [(8, 8, 4), (64, 16, 8), (20, 64, 4)])
+def test_sell_c_sigmize_and_simdify(oracle, T, C, w):
+    """Padding by the colouring leaves (the reference runs sell_c_sigmize as
+    a RACE callback), then the distance-2 rearrangement, against the oracle
+    on the same leaf calls."""
+    rp, col = M.stencil27(6)
+    val = M.laplace7_values(rp, col)
+    n = len(rp) - 1
+    ce, (prp, pcol, pval), zt = _prepared(rp, col, val, T, R.TWO, oracle)
+    calls = oracle.leaf_calls(zt)
+    cs, rl, clp, scol, sval = M.to_sell(prp, pcol, pval, C, pad_col=-1)
+    a = scol.copy()
+    ce.sell_c_sigmize(w, C, a, cs, rl, clp)
+    b = scol.copy()
+    oracle.sell_c_sigmize(calls, w, C, b, cs, rl, clp)
+    assert np.array_equal(a, b)
+    pc, pv = scol.copy(), sval.copy()
+    ok = ce.simdify(w, C, n, pc, cs, rl, clp, pv, diagFirst=True)
+    oc, ov = scol.copy(), sval.copy()
+    oracle.sell_c_sigmize(calls, w, C, oc, cs, rl, clp)
+    assert ok == oracle.simdify_resolve(w, C, n, oc, cs, clp, ov, True)
+    assert np.array_equal(pc, oc) and np.array_equal(pv, ov)
