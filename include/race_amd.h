@@ -188,6 +188,38 @@ int race_lb_split(const int *level_row, const int *level_nnz, int nlevels, int n
                   double efficiency, int dist, int d2_method, int lb_target, int cap,
                   int *nblocks, int *zone_rows, int *level_ptr, int *nthreads_out, int *scale);
 
+/* ---- batches: one function on many matrices (no counterpart in the
+ * reference) ----
+ * Every matrix keeps its own handle (colouring), CRS and device vectors; a
+ * batch runs them together: Gauss-Seidel / Kaczmarz as one fused launch per
+ * colour-phase index over the bundles of all matrices, SpMV / SymmSpMV as
+ * one launch over the SELL chunks of all matrices.  All matrices of a batch
+ * share kind, precisions and nrhs.  Handles must stay alive while the batch
+ * is built (race_batch_build, race_batch_autotune, first execution). */
+typedef struct race_batch_s *race_batch;
+int race_batch_create(race_batch *out, int kind, int val_precision, int vec_precision, int nrhs);
+/* d: like race_register_device (host arrays are copied) */
+int race_batch_add(race_batch b, race_handle h, const race_kernel_desc *d);
+/* "bundle_threads", "bundle_rows", "bundle_tpr" (0, 1, 2, 4, 8, 16, 64),
+ * "bundle_local" (GS / KACZ); "sell_c" (8, 16, 32, 64), "sell_sigma"
+ * (row-sorting window, 1 = none) (SpMV / SymmSpMV) */
+int race_batch_set_option(race_batch b, const char *key, int value);
+int race_batch_get_option(race_batch b, const char *key, int *value);
+int race_batch_build(race_batch b);
+int race_batch_execute(race_batch b, int rev);
+int race_batch_execute_async(race_batch b, int rev, void *stream);
+/* average milliseconds of one execution over reps (after one warm-up) */
+int race_batch_time(race_batch b, int rev, int reps, float *ms);
+/* the layout tuner: 12 candidate layouts per kernel family; race_batch_candidate
+ * describes candidate index (bundles: threads, rows, lanes per row; SELL: C,
+ * sigma, 0); race_batch_autotune times every candidate (times[i] = -1 where a
+ * layout does not fit), keeps the fastest and reports its index */
+int race_batch_candidates(int kind, int *count);
+int race_batch_candidate(int kind, int index, int *p0, int *p1, int *p2);
+int race_batch_use_candidate(race_batch b, int index);
+int race_batch_autotune(race_batch b, int rev, int reps, int *best, float *times, int cap);
+int race_batch_destroy(race_batch b);
+
 /* ---- SELL-C-sigma helpers ----
  * Entry j of row r of a SELL-C-sigma matrix sits at
  * chunk_start[r / C] + r % C + j * C; row_len[r] is the row length,

@@ -107,7 +107,7 @@ void *or_ws_create(int n)
     w->queue = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
     w->touched = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
     w->perm = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
-    w->cnt = (int *)malloc(sizeof(int) * (size_t)(2 * n + 4));
+    w->cnt = (int *)malloc(sizeof(int) * (size_t)(3 * (size_t)n + 4));
     for (int i = 0; i < n; ++i) { w->dist[i] = -1; w->parent[i] = -1; }
     return w;
 }
@@ -133,7 +133,10 @@ static inline void ws_visit(or_ws *w, int c, int par)
  * the graph stays in original numbering; u -> totalPerm[u] on reads,
  * child -> totalInvPerm[child]).
  *
- * levelRow/levelNnz: caller buffers of at least 2*(hi-lo)+2 ints.
+ * levelRow/levelNnz: caller buffers of at least 3*(hi-lo)+2 ints (with
+ * distance-2 bridges an isolated row of the range can cost three levels:
+ * the restart's empty level, its own, and the one that expands its
+ * bridges).
  * distOut (optional, may be NULL): receives dist for rows [lo,hi) in the
  * index space BEFORE the permutation is applied.
  * Returns totalLevel, or -1 when rows go missing (the reference exits).

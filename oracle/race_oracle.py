@@ -185,7 +185,7 @@ class Graph:
     def traverse(self, dist, lo, hi, want_dist=False):
         """Traverse(graph,dist,lo,hi).calculateDistance(); returns
         (levelRow, levelNnz[, dist]) and composes the permutation."""
-        cap = 2 * (hi - lo) + 2
+        cap = 3 * (hi - lo) + 2  # distance-2: up to three levels per row
         lr = np.zeros(cap, np.int32)
         ln = np.zeros(cap, np.int32)
         dd = np.zeros(max(hi - lo, 1), np.int32)

@@ -6,9 +6,9 @@ RACE::Interface over that ABI, the synthetic matrix generators and the build
 script.
 """
 from .interface import (BF16, FP32, FP64, K_GS, K_KACZ, K_MPK, K_SPMV, K_SYMMSPMV, LB_NNZ, LB_ROW, ONE,
-                        POWER, TWO, TWO_BLOCK, DeviceArray, Interface, RaceError, device_count, lb_split,
+                        POWER, TWO, TWO_BLOCK, Batch, DeviceArray, Interface, RaceError, device_count, lb_split,
                         simdify_resolve, simdifyD1)
 
-__all__ = ["Interface", "DeviceArray", "RaceError", "lb_split", "device_count", "simdify_resolve", "simdifyD1", "ONE", "TWO", "POWER",
+__all__ = ["Interface", "Batch", "DeviceArray", "RaceError", "lb_split", "device_count", "simdify_resolve", "simdifyD1", "ONE", "TWO", "POWER",
            "K_SYMMSPMV", "K_GS", "K_KACZ", "K_SPMV", "K_MPK", "FP64", "FP32", "BF16", "LB_NNZ", "LB_ROW",
            "TWO_BLOCK"]

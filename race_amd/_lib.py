@@ -88,6 +88,21 @@ PROTOTYPES = {
     "race_lb_split": (ctypes.c_int, [i32p, i32p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                      ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, i32p, i32p,
                                      i32p, i32p, i32p]),
+    "race_batch_create": (ctypes.c_int, [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_int]),
+    "race_batch_add": (ctypes.c_int, [vp, vp, ctypes.POINTER(KernelDesc)]),
+    "race_batch_set_option": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_int]),
+    "race_batch_get_option": (ctypes.c_int, [vp, ctypes.c_char_p, i32p]),
+    "race_batch_build": (ctypes.c_int, [vp]),
+    "race_batch_execute": (ctypes.c_int, [vp, ctypes.c_int]),
+    "race_batch_execute_async": (ctypes.c_int, [vp, ctypes.c_int, vp]),
+    "race_batch_time": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_float)]),
+    "race_batch_candidates": (ctypes.c_int, [ctypes.c_int, i32p]),
+    "race_batch_candidate": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, i32p, i32p, i32p]),
+    "race_batch_use_candidate": (ctypes.c_int, [vp, ctypes.c_int]),
+    "race_batch_autotune": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, i32p,
+                                           ctypes.POINTER(ctypes.c_float), ctypes.c_int]),
+    "race_batch_destroy": (ctypes.c_int, [vp]),
     "race_sell_c_sigmize": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, i32p, i32p, i32p, i32p]),
     "race_simdify": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, i32p, i32p, i32p, i32p, vp,
                                     ctypes.c_int, ctypes.c_int]),

@@ -83,18 +83,20 @@ BundleTable buildBundles(const Schedule& s, int kind, int n, const int* rp, cons
     t.wave.push_back(0);
     t.extPtr.push_back(0);
 
-    std::vector<int> curLeaves, rows, cnt, extCols;
+    // a bundle is filled with row ranges: whole leaves, or segments of a
+    // leaf too large for one bundle
+    std::vector<std::pair<int, int>> cur;
+    std::vector<int> rows, extCols;
     int curRows = 0, curLocal = 0;
     int bundleId = 0;  // id of the bundle being filled
 
-    // local columns a leaf adds to the bundle tagged `tag`
-    auto countNew = [&](int l, int tag) {
+    // local columns the rows [lo, hi) add to the bundle tagged `tag`
+    auto countNew = [&](int lo, int hi, int tag) {
         int add = 0;
-        const int lo = s.leafLo[l], hi = s.leafHi[l];
         for (int r = lo; r < hi; ++r)
             for (int j = rp[r]; j < rp[r + 1]; ++j) {
                 const int c = col[j];
-                if (gs && c >= lo && c < hi) continue;  // a row of this leaf
+                if (gs && c >= lo && c < hi) continue;  // one of these rows
                 if (seen[c] != tag) {
                     seen[c] = tag;
                     ++add;
@@ -102,39 +104,32 @@ BundleTable buildBundles(const Schedule& s, int kind, int n, const int* rp, cons
             }
         return add + (gs ? hi - lo : 0);
     };
-    // drop the tags of a leaf that did not fit, so it can be recounted
-    auto untag = [&](int l, int tag) {
-        for (int r = s.leafLo[l]; r < s.leafHi[l]; ++r)
+    // drop the tags of rows that did not fit, so they can be recounted
+    auto untag = [&](int lo, int hi, int tag) {
+        for (int r = lo; r < hi; ++r)
             for (int j = rp[r]; j < rp[r + 1]; ++j)
                 if (seen[col[j]] == tag) seen[col[j]] = -2;
     };
 
     auto flush = [&]() {
-        if (curLeaves.empty()) return;
+        if (cur.empty()) return;
         const int b = bundleId;
         const int s0 = t.slot.back();
         rows.clear();
-        int maxD = 0;
-        for (int l : curLeaves)
-            for (int r = s.leafLo[l]; r < s.leafHi[l]; ++r) {
-                rows.push_back(r);
-                maxD = std::max(maxD, depth[r]);
-            }
-        // slots ordered by (depth, row)
-        std::sort(rows.begin(), rows.end());
-        cnt.assign(maxD + 2, 0);
-        for (int r : rows) cnt[depth[r] + 1]++;
-        for (int d = 1; d <= maxD + 1; ++d) cnt[d] += cnt[d - 1];
-        std::vector<int> ordered(rows.size());
-        for (int r : rows) ordered[cnt[depth[r]]++] = r;
+        for (const auto& rg : cur)
+            for (int r = rg.first; r < rg.second; ++r) rows.push_back(r);
+        // slots ordered by (depth, row); one wave per depth present
+        std::sort(rows.begin(), rows.end(), [&](int a_, int b_) {
+            return depth[a_] != depth[b_] ? depth[a_] < depth[b_] : a_ < b_;
+        });
+        const std::vector<int>& ordered = rows;
         const int nb = (int)ordered.size();
-        {
-            int i = 0;
-            for (int d = 1; d <= maxD; ++d) {
+        int nw = 0;
+        for (int i = 0; i < nb; ++i)
+            if (i == 0 || depth[ordered[i]] != depth[ordered[i - 1]]) {
                 t.waveSlot.push_back(s0 + i);
-                while (i < nb && depth[ordered[i]] == d) ++i;
+                ++nw;
             }
-        }
         t.wave.push_back((int)t.waveSlot.size());
         // local column space (sorted outside columns: coalesced gathers)
         const int tagL = -(b + 3);  // distinct from the packing tags
@@ -208,40 +203,169 @@ BundleTable buildBundles(const Schedule& s, int kind, int n, const int* rp, cons
         t.slot.push_back(s0 + nb);
         t.phRows.back() = std::max(t.phRows.back(), nb);
         t.phLocal.back() = std::max(t.phLocal.back(), local);
-        t.phWaves.back() = std::max(t.phWaves.back(), maxD);
-        curLeaves.clear();
+        t.phWaves.back() = std::max(t.phWaves.back(), nw);
+        cur.clear();
         curRows = 0;
         curLocal = 0;
         ++bundleId;
     };
-
-    for (int p = 0; p < s.numPhases(); ++p) {
+    auto beginPhase = [&]() {
         t.phaseBundle.push_back((int)t.slot.size() - 1);
         t.phRows.push_back(0);
         t.phLocal.push_back(0);
         t.phWaves.push_back(0);
-        for (int l = s.phasePtr[p]; l < s.phasePtr[p + 1]; ++l) {
-            const int rowsL = s.leafHi[l] - s.leafLo[l];
-            if (rowsL > cap) {
-                t.ok = false;
-                return t;
-            }
-            int add = countNew(l, bundleId);
-            if (!curLeaves.empty() && (curRows + rowsL > cap || curLocal + add > localCap)) {
-                untag(l, bundleId);
-                flush();
-                add = countNew(l, bundleId);
-            }
-            if (add > localCap) {
-                t.ok = false;
-                return t;
-            }
-            curLeaves.push_back(l);
-            curRows += rowsL;
-            curLocal += add;
-        }
+    };
+    auto endPhase = [&]() {
         flush();
         t.critical += t.phWaves.back();
+    };
+    // add [lo, hi) to the bundle being filled, flushing it first when the
+    // rows or local columns would exceed the limits; false if [lo, hi)
+    // alone does not fit
+    auto place = [&](int lo, int hi) {
+        const int rowsL = hi - lo;
+        if (rowsL > cap) return false;
+        int add = countNew(lo, hi, bundleId);
+        if (!cur.empty() && (curRows + rowsL > cap || curLocal + add > localCap)) {
+            untag(lo, hi, bundleId);
+            flush();
+            add = countNew(lo, hi, bundleId);
+        }
+        if (add > localCap) {
+            untag(lo, hi, bundleId);
+            return false;
+        }
+        cur.push_back({lo, hi});
+        curRows += rowsL;
+        curLocal += add;
+        return true;
+    };
+
+    std::vector<int> big;
+    std::vector<std::vector<std::pair<int, int>>> segs;
+    for (int p = 0; p < s.numPhases(); ++p) {
+        beginPhase();
+        big.clear();
+        for (int l = s.phasePtr[p]; l < s.phasePtr[p + 1]; ++l)
+            if (!place(s.leafLo[l], s.leafHi[l])) big.push_back(l);
+        endPhase();
+        if (big.empty()) continue;
+        // Leaves too large for one bundle: consecutive segments in the
+        // leaf's execution order (forward lo..hi-1, reverse hi-1..lo), each
+        // segment as large as a bundle holds; segment k of every such leaf
+        // runs in sub-phase k, after segment k - 1.
+        segs.assign(big.size(), {});
+        size_t maxSeg = 0;
+        for (size_t q = 0; q < big.size(); ++q) {
+            const int lo = s.leafLo[big[q]], hi = s.leafHi[big[q]];
+            int a = s.rev ? hi : lo;  // next unplaced row boundary
+            while (s.rev ? a > lo : a < hi) {
+                // grow the segment one row at a time up to the limits
+                int len = 0, local = 0;
+                const int tag = -(int)(1 << 30) + (int)q;  // private tags for the sizing pass
+                for (;;) {
+                    const int r = s.rev ? a - len - 1 : a + len;
+                    if (s.rev ? r < lo : r >= hi) break;
+                    if (len + 1 > cap) break;
+                    int add = gs ? 1 : 0;
+                    for (int j = rp[r]; j < rp[r + 1]; ++j) {
+                        const int c = col[j];
+                        if (gs && c == r) continue;
+                        if (seen[c] != tag) ++add;
+                    }
+                    if (local + add > localCap) break;
+                    for (int j = rp[r]; j < rp[r + 1]; ++j) seen[col[j]] = tag;
+                    if (gs) seen[r] = tag;
+                    local += add;
+                    ++len;
+                }
+                if (len == 0) {
+                    t.ok = false;  // a single row exceeds the local column space
+                    return t;
+                }
+                for (int k = 0; k < len; ++k) {
+                    const int r = s.rev ? a - k - 1 : a + k;
+                    for (int j = rp[r]; j < rp[r + 1]; ++j) seen[col[j]] = -2;
+                    seen[r] = -2;
+                }
+                if (s.rev) {
+                    segs[q].push_back({a - len, a});
+                    a -= len;
+                } else {
+                    segs[q].push_back({a, a + len});
+                    a += len;
+                }
+            }
+            maxSeg = std::max(maxSeg, segs[q].size());
+        }
+        for (size_t k = 0; k < maxSeg; ++k) {
+            beginPhase();
+            for (size_t q = 0; q < big.size(); ++q)
+                if (k < segs[q].size() && !place(segs[q][k].first, segs[q][k].second)) {
+                    t.ok = false;
+                    return t;
+                }
+            endPhase();
+        }
+    }
+    t.phaseBundle.push_back((int)t.slot.size() - 1);
+    t.waveSlot.push_back(t.slot.back());
+    return t;
+}
+
+BundleTable concatBundles(const std::vector<const BundleTable*>& tabs, std::vector<int>* bundleTable) {
+    BundleTable t;
+    bundleTable->clear();
+    if (tabs.empty()) return t;
+    t.kind = tabs[0]->kind;
+    t.tpr = tabs[0]->tpr;
+    t.cap = 0;
+    int P = 0;
+    for (const BundleTable* a : tabs) {
+        P = std::max(P, a->numPhases());
+        t.cap = std::max(t.cap, a->cap);
+        t.maxLen = std::max(t.maxLen, a->maxLen);
+        t.paddedEntries += a->paddedEntries;
+        t.ok = t.ok && a->ok;
+    }
+    const bool gs = t.kind != RACE_K_KACZ;
+    t.slot.push_back(0);
+    t.wave.push_back(0);
+    t.extPtr.push_back(0);
+    for (int p = 0; p < P; ++p) {
+        t.phaseBundle.push_back((int)t.slot.size() - 1);
+        int rows = 0, local = 0, waves = 0;
+        for (size_t m = 0; m < tabs.size(); ++m) {
+            const BundleTable& a = *tabs[m];
+            if (p >= a.numPhases()) continue;
+            rows = std::max(rows, a.phRows[p]);
+            local = std::max(local, a.phLocal[p]);
+            waves = std::max(waves, a.phWaves[p]);
+            for (int q = a.phaseBundle[p]; q < a.phaseBundle[p + 1]; ++q) {
+                const int s0 = a.slot[q], nb = a.slot[q + 1] - s0;
+                const int newS0 = t.slot.back();
+                for (int w = a.wave[q]; w < a.wave[q + 1]; ++w) t.waveSlot.push_back(newS0 + a.waveSlot[w] - s0);
+                t.wave.push_back((int)t.waveSlot.size());
+                t.rowId.insert(t.rowId.end(), a.rowId.begin() + s0, a.rowId.begin() + s0 + nb);
+                if (gs)
+                    t.diag.insert(t.diag.end(), a.diag.begin() + s0, a.diag.begin() + s0 + nb);
+                else
+                    t.rowNorm.insert(t.rowNorm.end(), a.rowNorm.begin() + s0, a.rowNorm.begin() + s0 + nb);
+                t.slot.push_back(newS0 + nb);
+                const long block = (long)a.len[q] * nb * a.tpr;
+                t.off.push_back((long)t.sidx.size());
+                t.len.push_back(a.len[q]);
+                t.sidx.insert(t.sidx.end(), a.sidx.begin() + a.off[q], a.sidx.begin() + a.off[q] + block);
+                t.sval.insert(t.sval.end(), a.sval.begin() + a.off[q], a.sval.begin() + a.off[q] + block);
+                t.ext.insert(t.ext.end(), a.ext.begin() + a.extPtr[q], a.ext.begin() + a.extPtr[q + 1]);
+                t.extPtr.push_back((int)t.ext.size());
+                bundleTable->push_back((int)m);
+            }
+        }
+        t.phRows.push_back(rows);
+        t.phLocal.push_back(local);
+        t.phWaves.push_back(waves);
+        t.critical += waves;
     }
     t.phaseBundle.push_back((int)t.slot.size() - 1);
     t.waveSlot.push_back(t.slot.back());

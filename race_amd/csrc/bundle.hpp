@@ -56,4 +56,11 @@ struct BundleTable {
 BundleTable buildBundles(const Schedule& s, int kind, int n, const int* rowPtr, const int* col,
                          const double* val, int cap, int localCap, int tpr);
 
+// Bundle tables of several matrices joined phase by phase: fused phase p
+// holds the bundles of phase p of every table that has one, in table order.
+// Slots, waves, entry blocks and outside-column lists are renumbered; row
+// ids and column ids stay those of their own matrix (each bundle of a batch
+// carries its own vectors).  bundleTable[k] = table of fused bundle k.
+BundleTable concatBundles(const std::vector<const BundleTable*>& tabs, std::vector<int>* bundleTable);
+
 }  // namespace race

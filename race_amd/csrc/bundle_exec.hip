@@ -54,13 +54,35 @@ __device__ __forceinline__ int bundleOfBlock(int nbnd, int xcd) {
     return q < nbnd ? q : -1;
 }
 
-template <int BS, int TPR, int E, typename VT, typename XT, bool ONE_RHS>
+// Vectors of a bundle: the launch's x, b and n, or, for a batch of matrices
+// (BATCH), the bundle's own (each matrix keeps its vectors).
+template <bool BATCH, typename XT>
+struct BundleVec {
+    XT* x;
+    const XT* b;
+    int n;
+    __device__ __forceinline__ BundleVec(XT* x0, const XT* b0, int n0, XT* const* bx, const XT* const* bb,
+                                         const int* bn, int bnd) {
+        if (BATCH) {
+            x = bx[bnd];
+            b = bb[bnd];
+            n = bn[bnd];
+        } else {
+            x = x0;
+            b = b0;
+            n = n0;
+        }
+    }
+};
+
+template <int BS, int TPR, int E, typename VT, typename XT, bool ONE_RHS, bool BATCH>
 __global__ void __launch_bounds__(BS) k_gs_bundle(
     const int* __restrict__ bSlot, const int* __restrict__ bWave, const long* __restrict__ bOff,
     const int* __restrict__ bLen, const int* __restrict__ bExt, const int* __restrict__ waveSlot,
     const int* __restrict__ rowId, const XT* __restrict__ diag, const int* __restrict__ ext,
-    const uint16_t* __restrict__ sidx, const VT* __restrict__ sval, XT* __restrict__ x,
-    const XT* __restrict__ b, int n, int nrhs, int bundle0, int nbnd, int xcd, int ldsLocal) {
+    const uint16_t* __restrict__ sidx, const VT* __restrict__ sval, XT* __restrict__ x0,
+    const XT* __restrict__ b0, int n0, int nrhs, int bundle0, int nbnd, int xcd, int ldsLocal,
+    XT* const* __restrict__ bX, const XT* const* __restrict__ bB, const int* __restrict__ bN) {
     extern __shared__ __align__(16) unsigned char smem[];
     XT* xl = reinterpret_cast<XT*>(smem);
     int* ws = reinterpret_cast<int*>(xl + ldsLocal);
@@ -68,6 +90,10 @@ __global__ void __launch_bounds__(BS) k_gs_bundle(
     const int q = bundleOfBlock(nbnd, xcd);
     if (q < 0) return;
     const int bnd = bundle0 + q;
+    const BundleVec<BATCH, XT> vec(x0, b0, n0, bX, bB, bN, bnd);
+    XT* __restrict__ x = vec.x;
+    const XT* __restrict__ b = vec.b;
+    const int n = vec.n;
     const int s0 = bSlot[bnd], nb = bSlot[bnd + 1] - s0;
     const long off = bOff[bnd];
     const int wb = bLen[bnd];
@@ -119,13 +145,14 @@ __global__ void __launch_bounds__(BS) k_gs_bundle(
 
 // Kaczmarz: every column the bundle touches lives in LDS; rows of one wave
 // share no column.  The row norms stay FP64.
-template <int BS, int TPR, int E, typename VT, typename XT>
+template <int BS, int TPR, int E, typename VT, typename XT, bool BATCH>
 __global__ void __launch_bounds__(BS) k_kacz_bundle(
     const int* __restrict__ bSlot, const int* __restrict__ bWave, const long* __restrict__ bOff,
     const int* __restrict__ bLen, const int* __restrict__ bExt, const int* __restrict__ waveSlot,
     const int* __restrict__ rowId, const double* __restrict__ rowNorm, const int* __restrict__ ext,
-    const uint16_t* __restrict__ sidx, const VT* __restrict__ sval, XT* __restrict__ x,
-    const XT* __restrict__ b, int n, int nrhs, int bundle0, int nbnd, int xcd, int ldsLocal) {
+    const uint16_t* __restrict__ sidx, const VT* __restrict__ sval, XT* __restrict__ x0,
+    const XT* __restrict__ b0, int n0, int nrhs, int bundle0, int nbnd, int xcd, int ldsLocal,
+    XT* const* __restrict__ bX, const XT* const* __restrict__ bB, const int* __restrict__ bN) {
     extern __shared__ __align__(16) unsigned char smem[];
     XT* xl = reinterpret_cast<XT*>(smem);
     int* ws = reinterpret_cast<int*>(xl + ldsLocal);
@@ -133,6 +160,10 @@ __global__ void __launch_bounds__(BS) k_kacz_bundle(
     const int q = bundleOfBlock(nbnd, xcd);
     if (q < 0) return;
     const int bnd = bundle0 + q;
+    const BundleVec<BATCH, XT> vec(x0, b0, n0, bX, bB, bN, bnd);
+    XT* __restrict__ x = vec.x;
+    const XT* __restrict__ b = vec.b;
+    const int n = vec.n;
     const int s0 = bSlot[bnd], nb = bSlot[bnd + 1] - s0;
     const long off = bOff[bnd];
     const int wb = bLen[bnd];
@@ -192,63 +223,127 @@ uint16_t toBf16(double v) {
     return (uint16_t)(u >> 16);
 }
 
-// (lanes per row, register width) pairs with a kernel instantiation
+// (lanes per row, register width) pairs with a kernel instantiation: short
+// rows 1 or 2 lanes, long rows (graphs with hub vertices) up to a whole
+// wavefront of 64 lanes x 16 entries
 int pickE(int tpr, int need) {
-    static const int e1[] = {4, 8, 16}, e2[] = {8, 13, 16}, e4[] = {8};
-    const int* c = tpr == 1 ? e1 : tpr == 2 ? e2 : e4;
-    const int nc = tpr == 1 ? 3 : tpr == 2 ? 3 : 1;
+    static const int e1[] = {4, 8, 16}, e2[] = {8, 13, 16}, e4[] = {8, 16}, eL[] = {16};
+    const int* c = tpr == 1 ? e1 : tpr == 2 ? e2 : tpr == 4 ? e4 : eL;
+    const int nc = tpr == 1 ? 3 : tpr == 2 ? 3 : tpr == 4 ? 2 : 1;
+    if (tpr != 1 && tpr != 2 && tpr != 4 && tpr != 8 && tpr != 16 && tpr != 64) return -1;
     for (int q = 0; q < nc; ++q)
         if (c[q] >= need) return c[q];
     return -1;
 }
 
 template <typename VT, typename XT>
+struct BundleDev {
+    BundleTable h;
+    DevBuf<int> slot, wave, len, ext, extPtr, waveSlot, rowId;
+    DevBuf<long> off;
+    DevBuf<uint16_t> sidx;
+    DevBuf<VT> sval;
+    DevBuf<XT> diag;
+    DevBuf<double> rowNorm;
+    int load(BundleTable&& t) {
+        h = std::move(t);
+        int e;
+        if ((e = upload(slot, h.slot.data(), h.slot.size()))) return e;
+        if ((e = upload(wave, h.wave.data(), h.wave.size()))) return e;
+        if ((e = upload(len, h.len.data(), h.len.size()))) return e;
+        if ((e = upload(off, h.off.data(), h.off.size()))) return e;
+        if ((e = upload(ext, h.ext.data(), h.ext.size()))) return e;
+        if ((e = upload(extPtr, h.extPtr.data(), h.extPtr.size()))) return e;
+        if ((e = upload(waveSlot, h.waveSlot.data(), h.waveSlot.size()))) return e;
+        if ((e = upload(rowId, h.rowId.data(), h.rowId.size()))) return e;
+        if ((e = upload(sidx, h.sidx.data(), h.sidx.size()))) return e;
+        std::vector<VT> v(h.sval.size());
+        for (size_t q = 0; q < v.size(); ++q) v[q] = conv(h.sval[q]);
+        if ((e = upload(sval, v.data(), v.size()))) return e;
+        if (h.kind == RACE_K_KACZ) {
+            if ((e = upload(rowNorm, h.rowNorm.data(), h.rowNorm.size()))) return e;
+        } else {
+            std::vector<XT> dg(h.diag.begin(), h.diag.end());
+            if ((e = upload(diag, dg.data(), dg.size()))) return e;
+        }
+        // the host copies of the big arrays are no longer needed
+        std::vector<uint16_t>().swap(h.sidx);
+        std::vector<double>().swap(h.sval);
+        return OK;
+    }
+    static VT conv(double v) {
+        if constexpr (sizeof(VT) == 2)
+            return toBf16(v);
+        else
+            return (VT)v;
+    }
+    size_t lds(int p) const {
+        return (size_t)h.phLocal[p] * sizeof(XT) + (size_t)(h.phWaves[p] + 1) * sizeof(int) + 32;
+    }
+};
+
+// one fused or plain launch of the bundles [b0, b0 + nbnd) of phase p
+template <int BS, int TPR, int EE, typename VT, typename XT, bool BATCH>
+void launchBundles(int kind, int nrhs, const BundleDev<VT, XT>& d, int b0, int nbnd, int p, int xcd, XT* x,
+                   const XT* b, int n, XT* const* bX, const XT* const* bB, const int* bN, hipStream_t st) {
+    const unsigned grid = xcd ? 8u * (unsigned)((nbnd + 7) / 8) : (unsigned)nbnd;
+    if (kind == RACE_K_KACZ)
+        hipLaunchKernelGGL((k_kacz_bundle<BS, TPR, EE, VT, XT, BATCH>), dim3(grid), dim3(BS), d.lds(p), st,
+                           d.slot.p, d.wave.p, d.off.p, d.len.p, d.extPtr.p, d.waveSlot.p, d.rowId.p,
+                           d.rowNorm.p, d.ext.p, d.sidx.p, d.sval.p, x, b, n, nrhs, b0, nbnd, xcd,
+                           d.h.phLocal[p], bX, bB, bN);
+    else if (nrhs == 1)
+        hipLaunchKernelGGL((k_gs_bundle<BS, TPR, EE, VT, XT, true, BATCH>), dim3(grid), dim3(BS), d.lds(p), st,
+                           d.slot.p, d.wave.p, d.off.p, d.len.p, d.extPtr.p, d.waveSlot.p, d.rowId.p, d.diag.p,
+                           d.ext.p, d.sidx.p, d.sval.p, x, b, n, nrhs, b0, nbnd, xcd, d.h.phLocal[p], bX, bB, bN);
+    else
+        hipLaunchKernelGGL((k_gs_bundle<BS, TPR, EE, VT, XT, false, BATCH>), dim3(grid), dim3(BS), d.lds(p),
+                           st, d.slot.p, d.wave.p, d.off.p, d.len.p, d.extPtr.p, d.waveSlot.p, d.rowId.p,
+                           d.diag.p, d.ext.p, d.sidx.p, d.sval.p, x, b, n, nrhs, b0, nbnd, xcd, d.h.phLocal[p],
+                           bX, bB, bN);
+}
+
+template <typename VT, typename XT, bool BATCH>
+void launchPhase(int bs, int tpr, int E, int kind, int nrhs, const BundleDev<VT, XT>& d, int b0, int nbnd, int p,
+                 int xcd, XT* x, const XT* b, int n, XT* const* bX, const XT* const* bB, const int* bN,
+                 hipStream_t st) {
+#define RACE_BUNDLE_CASE(BS_)                                                                              \
+    switch (tpr * 100 + E) {                                                                               \
+        case 104: launchBundles<BS_, 1, 4, VT, XT, BATCH>(kind, nrhs, d, b0, nbnd, p, xcd, x, b, n, bX, bB, bN, st); break;   \
+        case 108: launchBundles<BS_, 1, 8, VT, XT, BATCH>(kind, nrhs, d, b0, nbnd, p, xcd, x, b, n, bX, bB, bN, st); break;   \
+        case 116: launchBundles<BS_, 1, 16, VT, XT, BATCH>(kind, nrhs, d, b0, nbnd, p, xcd, x, b, n, bX, bB, bN, st); break;  \
+        case 208: launchBundles<BS_, 2, 8, VT, XT, BATCH>(kind, nrhs, d, b0, nbnd, p, xcd, x, b, n, bX, bB, bN, st); break;   \
+        case 213: launchBundles<BS_, 2, 13, VT, XT, BATCH>(kind, nrhs, d, b0, nbnd, p, xcd, x, b, n, bX, bB, bN, st); break;  \
+        case 216: launchBundles<BS_, 2, 16, VT, XT, BATCH>(kind, nrhs, d, b0, nbnd, p, xcd, x, b, n, bX, bB, bN, st); break;  \
+        case 416: launchBundles<BS_, 4, 16, VT, XT, BATCH>(kind, nrhs, d, b0, nbnd, p, xcd, x, b, n, bX, bB, bN, st); break;  \
+        case 816: launchBundles<BS_, 8, 16, VT, XT, BATCH>(kind, nrhs, d, b0, nbnd, p, xcd, x, b, n, bX, bB, bN, st); break;  \
+        case 1616: launchBundles<BS_, 16, 16, VT, XT, BATCH>(kind, nrhs, d, b0, nbnd, p, xcd, x, b, n, bX, bB, bN, st); break; \
+        case 6416: launchBundles<BS_, 64, 16, VT, XT, BATCH>(kind, nrhs, d, b0, nbnd, p, xcd, x, b, n, bX, bB, bN, st); break; \
+        default: launchBundles<BS_, 4, 8, VT, XT, BATCH>(kind, nrhs, d, b0, nbnd, p, xcd, x, b, n, bX, bB, bN, st); break;   \
+    }
+    switch (bs) {
+        case 128: RACE_BUNDLE_CASE(128) break;
+        case 512: RACE_BUNDLE_CASE(512) break;
+        case 1024: RACE_BUNDLE_CASE(1024) break;
+        default: RACE_BUNDLE_CASE(256) break;
+    }
+#undef RACE_BUNDLE_CASE
+}
+
+// lanes per row and register width for the longest row (diagonal excluded
+// for Gauss-Seidel); E < 0 if no kernel instantiation fits
+void pickShape(int kind, int maxRowLen, int forcedTpr, int* tpr, int* E) {
+    int m = maxRowLen - (kind != RACE_K_KACZ ? 1 : 0);
+    m = std::max(m, 1);
+    *tpr = forcedTpr > 0 ? forcedTpr
+                        : (m <= 16 ? 1 : m <= 32 ? 2 : m <= 64 ? 4 : m <= 128 ? 8 : m <= 256 ? 16 : 64);
+    *E = pickE(*tpr, (m + *tpr - 1) / *tpr);
+}
+
+template <typename VT, typename XT>
 class BundleFn : public DeviceFn {
   public:
-    struct Dev {
-        BundleTable h;
-        DevBuf<int> slot, wave, len, ext, extPtr, waveSlot, rowId;
-        DevBuf<long> off;
-        DevBuf<uint16_t> sidx;
-        DevBuf<VT> sval;
-        DevBuf<XT> diag;
-        DevBuf<double> rowNorm;
-        int load(BundleTable&& t) {
-            h = std::move(t);
-            int e;
-            if ((e = upload(slot, h.slot.data(), h.slot.size()))) return e;
-            if ((e = upload(wave, h.wave.data(), h.wave.size()))) return e;
-            if ((e = upload(len, h.len.data(), h.len.size()))) return e;
-            if ((e = upload(off, h.off.data(), h.off.size()))) return e;
-            if ((e = upload(ext, h.ext.data(), h.ext.size()))) return e;
-            if ((e = upload(extPtr, h.extPtr.data(), h.extPtr.size()))) return e;
-            if ((e = upload(waveSlot, h.waveSlot.data(), h.waveSlot.size()))) return e;
-            if ((e = upload(rowId, h.rowId.data(), h.rowId.size()))) return e;
-            if ((e = upload(sidx, h.sidx.data(), h.sidx.size()))) return e;
-            std::vector<VT> v(h.sval.size());
-            for (size_t q = 0; q < v.size(); ++q) v[q] = conv(h.sval[q]);
-            if ((e = upload(sval, v.data(), v.size()))) return e;
-            if (h.kind == RACE_K_KACZ) {
-                if ((e = upload(rowNorm, h.rowNorm.data(), h.rowNorm.size()))) return e;
-            } else {
-                std::vector<XT> dg(h.diag.begin(), h.diag.end());
-                if ((e = upload(diag, dg.data(), dg.size()))) return e;
-            }
-            // the host copies of the big arrays are no longer needed
-            std::vector<uint16_t>().swap(h.sidx);
-            std::vector<double>().swap(h.sval);
-            return OK;
-        }
-        static VT conv(double v) {
-            if constexpr (sizeof(VT) == 2)
-                return toBf16(v);
-            else
-                return (VT)v;
-        }
-        size_t lds(int p) const {
-            return (size_t)h.phLocal[p] * sizeof(XT) + (size_t)(h.phWaves[p] + 1) * sizeof(int) + 32;
-        }
-    };
+    using Dev = BundleDev<VT, XT>;
     Dev df, dr;
     int nrhs = 1, n = 0, bs = 256, tpr = 2, E = 16, xcd = 1;
     std::unique_ptr<DeviceFn> fallback;
@@ -263,10 +358,7 @@ class BundleFn : public DeviceFn {
         xcd = ctx.xcdOrder;
         int maxLen = 0;
         for (int r = 0; r < n; ++r) maxLen = std::max(maxLen, d.rowPtr[r + 1] - d.rowPtr[r]);
-        if (kind != RACE_K_KACZ) maxLen -= 1;  // the diagonal is kept apart
-        maxLen = std::max(maxLen, 1);
-        tpr = ctx.bundleTpr > 0 ? ctx.bundleTpr : (maxLen <= 16 ? 1 : maxLen <= 32 ? 2 : 4);
-        E = pickE(tpr, (maxLen + tpr - 1) / tpr);
+        pickShape(kind, maxLen, ctx.bundleTpr, &tpr, &E);
         const int rowsCap = std::min(ctx.bundleRows, bs / tpr);
         const bool fp64 = std::is_same<VT, double>::value && std::is_same<XT, double>::value;
         auto fallBack = [&]() -> int {
@@ -290,37 +382,6 @@ class BundleFn : public DeviceFn {
         return OK;
     }
 
-    template <int BS, int TPR, int EE>
-    void launch(const Dev& d, int b0, int nbnd, int p, hipStream_t st) {
-        const unsigned grid = xcd ? 8u * (unsigned)((nbnd + 7) / 8) : (unsigned)nbnd;
-        if (kind == RACE_K_KACZ)
-            hipLaunchKernelGGL((k_kacz_bundle<BS, TPR, EE, VT, XT>), dim3(grid), dim3(BS), d.lds(p), st,
-                               d.slot.p, d.wave.p, d.off.p, d.len.p, d.extPtr.p, d.waveSlot.p,
-                               d.rowId.p, d.rowNorm.p, d.ext.p, d.sidx.p, d.sval.p, (XT*)x,
-                               (const XT*)b, n, nrhs, b0, nbnd, xcd, d.h.phLocal[p]);
-        else if (nrhs == 1)
-            hipLaunchKernelGGL((k_gs_bundle<BS, TPR, EE, VT, XT, true>), dim3(grid), dim3(BS), d.lds(p),
-                               st, d.slot.p, d.wave.p, d.off.p, d.len.p, d.extPtr.p, d.waveSlot.p,
-                               d.rowId.p, d.diag.p, d.ext.p, d.sidx.p, d.sval.p, (XT*)x, (const XT*)b,
-                               n, nrhs, b0, nbnd, xcd, d.h.phLocal[p]);
-        else
-            hipLaunchKernelGGL((k_gs_bundle<BS, TPR, EE, VT, XT, false>), dim3(grid), dim3(BS), d.lds(p),
-                               st, d.slot.p, d.wave.p, d.off.p, d.len.p, d.extPtr.p, d.waveSlot.p,
-                               d.rowId.p, d.diag.p, d.ext.p, d.sidx.p, d.sval.p, (XT*)x, (const XT*)b,
-                               n, nrhs, b0, nbnd, xcd, d.h.phLocal[p]);
-    }
-    template <int BS>
-    void launchBs(const Dev& d, int b0, int nbnd, int p, hipStream_t st) {
-        switch (tpr * 100 + E) {
-            case 104: launch<BS, 1, 4>(d, b0, nbnd, p, st); break;
-            case 108: launch<BS, 1, 8>(d, b0, nbnd, p, st); break;
-            case 116: launch<BS, 1, 16>(d, b0, nbnd, p, st); break;
-            case 208: launch<BS, 2, 8>(d, b0, nbnd, p, st); break;
-            case 213: launch<BS, 2, 13>(d, b0, nbnd, p, st); break;
-            case 216: launch<BS, 2, 16>(d, b0, nbnd, p, st); break;
-            default: launch<BS, 4, 8>(d, b0, nbnd, p, st); break;
-        }
-    }
     int waves(bool rev) const override {
         if (fallback) return fallback->waves(rev);
         return (int)(rev ? dr.h.critical : df.h.critical);
@@ -336,19 +397,155 @@ class BundleFn : public DeviceFn {
         for (int p = 0; p + 1 < (int)pb.size(); ++p) {
             const int b0 = pb[p], nbnd = pb[p + 1] - pb[p];
             if (nbnd <= 0) continue;
-            switch (bs) {
-                case 128: launchBs<128>(d, b0, nbnd, p, st); break;
-                case 512: launchBs<512>(d, b0, nbnd, p, st); break;
-                case 1024: launchBs<1024>(d, b0, nbnd, p, st); break;
-                default: launchBs<256>(d, b0, nbnd, p, st); break;
-            }
+            launchPhase<VT, XT, false>(bs, tpr, E, kind, nrhs, d, b0, nbnd, p, xcd, (XT*)x, (const XT*)b, n,
+                                       nullptr, nullptr, nullptr, st);
         }
         RACE_HIP(hipGetLastError());
         return OK;
     }
 };
 
+// Gauss-Seidel or Kaczmarz on a batch of matrices, each with its own
+// colouring and vectors: one fused launch per phase index runs the bundles
+// of that phase of every matrix (concatBundles), so a batch of small
+// matrices fills the device instead of issuing a few tiny launches per
+// matrix and phase.  All matrices share the kernel shape (lanes per row,
+// register width, workgroup size), chosen from the longest row.
+template <typename VT, typename XT>
+class BundleBatchFn : public DeviceFn {
+  public:
+    using Dev = BundleDev<VT, XT>;
+    // matrices with the same kernel shape share fused tables
+    struct Group {
+        int tpr = 1, E = 4;
+        Dev dev[2];
+        DevBuf<XT*> bx[2];
+        DevBuf<const XT*> bb[2];
+        DevBuf<int> bn[2];
+    };
+    std::vector<std::unique_ptr<Group>> groups;
+    int nrhs = 1, bs = 256, xcd = 0, maxPhases[2] = {0, 0};
+    long critical[2] = {0, 0};
+
+    int init(const std::vector<BatchItem>& items, const ExecContext& ctx) {
+        if (items.empty()) return ERR_INVALID_ARG;
+        kind = items[0].d.kind;
+        nrhs = items[0].d.nrhs > 0 ? items[0].d.nrhs : 1;
+        bs = ctx.bundleThreads;
+        xcd = ctx.xcdOrder;
+        std::vector<int> shape(items.size());
+        std::vector<std::pair<int, int>> shapes;
+        for (size_t m = 0; m < items.size(); ++m) {
+            const BatchItem& it = items[m];
+            int maxLen = 0;
+            for (int r = 0; r < it.d.nrow; ++r) maxLen = std::max(maxLen, it.d.rowPtr[r + 1] - it.d.rowPtr[r]);
+            int tpr, E;
+            pickShape(kind, maxLen, ctx.bundleTpr, &tpr, &E);
+            if (E < 0 && ctx.bundleTpr > 0) pickShape(kind, maxLen, 0, &tpr, &E);  // row too long for the forced width
+            if (E < 0 || std::min(ctx.bundleRows, bs / tpr) < 1) return ERR_INCOMPATIBILITY;
+            const std::pair<int, int> key(tpr, E);
+            auto f = std::find(shapes.begin(), shapes.end(), key);
+            shape[m] = (int)(f - shapes.begin());
+            if (f == shapes.end()) shapes.push_back(key);
+        }
+        for (size_t g = 0; g < shapes.size(); ++g) {
+            auto grp = std::make_unique<Group>();
+            grp->tpr = shapes[g].first;
+            grp->E = shapes[g].second;
+            const int rowsCap = std::min(ctx.bundleRows, bs / grp->tpr);
+            for (int dir = 0; dir < 2; ++dir) {
+                std::vector<int> members;
+                for (size_t m = 0; m < items.size(); ++m)
+                    if (shape[m] == (int)g) members.push_back((int)m);
+                std::vector<BundleTable> tabs(members.size());
+                std::vector<const BundleTable*> ptrs;
+                for (size_t q = 0; q < members.size(); ++q) {
+                    const BatchItem& it = items[members[q]];
+                    tabs[q] = buildBundles(dir ? *it.rev : *it.fwd, kind, it.d.nrow, it.d.rowPtr, it.d.col,
+                                           it.d.val, rowsCap, ctx.bundleLocal, grp->tpr);
+                    if (!tabs[q].ok) return ERR_INCOMPATIBILITY;
+                    ptrs.push_back(&tabs[q]);
+                }
+                std::vector<int> owner;
+                BundleTable fused = concatBundles(ptrs, &owner);
+                std::vector<BundleTable>().swap(tabs);
+                std::vector<XT*> hx(owner.size());
+                std::vector<const XT*> hb(owner.size());
+                std::vector<int> hn(owner.size());
+                for (size_t k = 0; k < owner.size(); ++k) {
+                    const race_kernel_desc& d = items[members[owner[k]]].d;
+                    hx[k] = (XT*)d.x;
+                    hb[k] = (const XT*)d.b;
+                    hn[k] = d.nrow;
+                }
+                int e;
+                if ((e = upload(grp->bx[dir], hx.data(), hx.size()))) return e;
+                if ((e = upload(grp->bb[dir], hb.data(), hb.size()))) return e;
+                if ((e = upload(grp->bn[dir], hn.data(), hn.size()))) return e;
+                if ((e = grp->dev[dir].load(std::move(fused)))) return e;
+                for (int p = 0; p < grp->dev[dir].h.numPhases(); ++p)
+                    if (grp->dev[dir].lds(p) > kMaxLds) return ERR_INCOMPATIBILITY;
+                maxPhases[dir] = std::max(maxPhases[dir], grp->dev[dir].h.numPhases());
+            }
+            groups.push_back(std::move(grp));
+        }
+        for (int dir = 0; dir < 2; ++dir)
+            for (int p = 0; p < maxPhases[dir]; ++p) {
+                int w = 0;
+                for (const auto& g : groups)
+                    if (p < g->dev[dir].h.numPhases()) w = std::max(w, g->dev[dir].h.phWaves[p]);
+                critical[dir] += w;
+            }
+        return OK;
+    }
+    int waves(bool rev) const override { return (int)critical[rev ? 1 : 0]; }
+    int run(bool rev, hipStream_t st) override {
+        const int dir = rev ? 1 : 0;
+        // phase p of every group before phase p + 1 of any (stream order)
+        for (int p = 0; p < maxPhases[dir]; ++p)
+            for (const auto& g : groups) {
+                const Dev& d = g->dev[dir];
+                if (p >= d.h.numPhases()) continue;
+                const int b0 = d.h.phaseBundle[p], nbnd = d.h.phaseBundle[p + 1] - b0;
+                if (nbnd <= 0) continue;
+                launchPhase<VT, XT, true>(bs, g->tpr, g->E, kind, nrhs, d, b0, nbnd, p, xcd, nullptr, nullptr, 0,
+                                          g->bx[dir].p, g->bb[dir].p, g->bn[dir].p, st);
+            }
+        RACE_HIP(hipGetLastError());
+        return OK;
+    }
+};
+
 }  // namespace
+
+int makeBundleBatchFn(const std::vector<BatchItem>& items, const ExecContext& ctx, std::unique_ptr<DeviceFn>* out) {
+    if (items.empty()) return ERR_INVALID_ARG;
+    if (ctx.bundleThreads != 128 && ctx.bundleThreads != 256 && ctx.bundleThreads != 512 &&
+        ctx.bundleThreads != 1024)
+        return ERR_INVALID_ARG;
+    const race_kernel_desc& d0 = items[0].d;
+    for (const BatchItem& it : items)
+        if (it.d.kind != d0.kind || it.d.val_precision != d0.val_precision ||
+            it.d.vec_precision != d0.vec_precision || it.d.nrhs != d0.nrhs || !it.fwd || !it.rev)
+            return ERR_INVALID_ARG;
+    if (d0.kind != RACE_K_GS && d0.kind != RACE_K_KACZ) return ERR_INVALID_ARG;
+    int e;
+    if (d0.vec_precision == RACE_FP64) {
+        if (d0.val_precision != RACE_FP64) return ERR_NOT_IMPLEMENTED;
+        auto f = std::make_unique<BundleBatchFn<double, double>>();
+        e = f->init(items, ctx);
+        *out = std::move(f);
+    } else if (d0.val_precision == RACE_BF16) {
+        auto f = std::make_unique<BundleBatchFn<uint16_t, float>>();
+        e = f->init(items, ctx);
+        *out = std::move(f);
+    } else {
+        auto f = std::make_unique<BundleBatchFn<float, float>>();
+        e = f->init(items, ctx);
+        *out = std::move(f);
+    }
+    return e;
+}
 
 int makeBundleFn(const race_kernel_desc& d, const ExecContext& ctx, std::unique_ptr<DeviceFn>* out) {
     if (ctx.bundleThreads != 128 && ctx.bundleThreads != 256 && ctx.bundleThreads != 512 &&

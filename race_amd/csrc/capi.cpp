@@ -99,6 +99,76 @@ struct race_handle_s {
 
 static int checkHandle(race_handle h) { return h ? OK : ERR_INVALID_ARG; }
 
+// A batch of functions on many matrices (race_batch_*): host copies of each
+// matrix, its handle (colour schedules) and vectors, the layout options and
+// the built fused device function.
+struct race_batch_s {
+    int kind = 0, valPrec = RACE_FP64, vecPrec = RACE_FP64, nrhs = 1;
+    struct Mat {
+        race_handle h = nullptr;
+        std::vector<int> rp, col;
+        std::vector<double> val;
+        void* x = nullptr;
+        void* b = nullptr;
+    };
+    std::vector<Mat> mats;
+    ExecContext ctx;
+    std::unique_ptr<DeviceFn> fn;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    ~race_batch_s() {
+        fn.reset();
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+// the tuner's candidate layouts (12 per kernel family):
+//   GS / KACZ bundles: (workgroup threads, rows per bundle, lanes per row;
+//                       0 = from the longest row)
+//   SpMV / SymmSpMV SELL: (rows per chunk C, sorting window sigma)
+static const int kBundleCand[12][3] = {{512, 256, 0}, {256, 256, 0}, {1024, 256, 0}, {256, 64, 0},
+                                       {512, 128, 0}, {128, 128, 0}, {512, 256, 1},  {512, 256, 2},
+                                       {512, 256, 4}, {256, 128, 2}, {256, 128, 4},  {1024, 512, 2}};
+static const int kSellCand[12][2] = {{64, 1},   {32, 1},  {16, 1},  {8, 1},  {64, 32}, {32, 32},
+                                     {16, 32},  {8, 32},  {64, 1024}, {32, 1024}, {16, 1024}, {8, 1024}};
+
+static bool bundleKind(int k) { return k == RACE_K_GS || k == RACE_K_KACZ; }
+
+static void applyCandidate(race_batch b, int i) {
+    if (bundleKind(b->kind)) {
+        b->ctx.bundleThreads = kBundleCand[i][0];
+        b->ctx.bundleRows = kBundleCand[i][1];
+        b->ctx.bundleTpr = kBundleCand[i][2];
+    } else {
+        b->ctx.sellC = kSellCand[i][0];
+        b->ctx.sellSigma = kSellCand[i][1];
+    }
+}
+
+static int buildBatch(race_batch b) {
+    b->fn.reset();
+    std::vector<BatchItem> items(b->mats.size());
+    for (size_t m = 0; m < b->mats.size(); ++m) {
+        race_batch_s::Mat& M = b->mats[m];
+        BatchItem& it = items[m];
+        it.fwd = &M.h->fwd;
+        it.rev = &M.h->rev;
+        it.d.kind = b->kind;
+        it.d.nrow = (int)M.rp.size() - 1;
+        it.d.rowPtr = M.rp.data();
+        it.d.col = M.col.data();
+        it.d.val = M.val.data();
+        it.d.val_precision = b->valPrec;
+        it.d.vec_precision = b->vecPrec;
+        it.d.nrhs = b->nrhs;
+        it.d.x = M.x;
+        it.d.b = M.b;
+    }
+    return bundleKind(b->kind) ? makeBundleBatchFn(items, b->ctx, &b->fn) : makeSellBatchFn(items, b->ctx, &b->fn);
+}
+
 extern "C" {
 
 const char* race_error_string(int s) {
@@ -337,7 +407,8 @@ int race_set_option(race_handle h, const char* key, int value) {
         if (value != 128 && value != 256 && value != 512 && value != 1024) return ERR_INVALID_ARG;
         h->optBundleThreads = value;
     } else if (k == "bundle_tpr") {
-        if (value != 0 && value != 1 && value != 2 && value != 4) return ERR_INVALID_ARG;
+        if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8 && value != 16 && value != 64)
+            return ERR_INVALID_ARG;
         h->optBundleTpr = value;
     } else if (k == "bundle_local") {
         if (value < 1 || value > 65535) return ERR_INVALID_ARG;
@@ -615,6 +686,178 @@ int race_simdify_d1(int simd_width, int C, int nrows, int* col, const int* chunk
         return simdifyD1<float>(simd_width, C, nrows, col, chunk_start, chunk_len, (float*)val,
                                 dist2_compatible != 0);
     return ERR_INVALID_ARG;
+}
+
+int race_batch_create(race_batch* out, int kind, int val_precision, int vec_precision, int nrhs) {
+    if (!out) return ERR_INVALID_ARG;
+    *out = nullptr;
+    if (kind != RACE_K_GS && kind != RACE_K_KACZ && kind != RACE_K_SYMMSPMV && kind != RACE_K_SPMV)
+        return ERR_INVALID_ARG;
+    if (nrhs < 1 || val_precision < RACE_FP64 || val_precision > RACE_BF16 || vec_precision < RACE_FP64 ||
+        vec_precision > RACE_FP32)
+        return ERR_INVALID_ARG;
+    auto b = std::make_unique<race_batch_s>();
+    b->kind = kind;
+    b->valPrec = val_precision;
+    b->vecPrec = vec_precision;
+    b->nrhs = nrhs;
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return ERR_INTERNAL;
+    *out = b.release();
+    return OK;
+}
+
+int race_batch_add(race_batch b, race_handle h, const race_kernel_desc* d) {
+    if (!b || checkHandle(h) || !d || !d->rowPtr || !d->col || !d->val || !d->x) return ERR_INVALID_ARG;
+    if (d->kind != b->kind || d->val_precision != b->valPrec || d->vec_precision != b->vecPrec ||
+        (d->nrhs > 0 ? d->nrhs : 1) != b->nrhs || d->nrow != h->nrow)
+        return ERR_INVALID_ARG;
+    if (bundleKind(b->kind) && (h->dist == DIST_POWER || !h->colored)) return ERR_INVALID_ARG;
+    if (!d->b) return ERR_INVALID_ARG;
+    race_batch_s::Mat m;
+    m.h = h;
+    m.rp.assign(d->rowPtr, d->rowPtr + d->nrow + 1);
+    m.col.assign(d->col, d->col + m.rp.back());
+    m.val.assign(d->val, d->val + m.rp.back());
+    m.x = d->x;
+    m.b = d->b;
+    b->mats.push_back(std::move(m));
+    b->fn.reset();
+    return OK;
+}
+
+int race_batch_set_option(race_batch b, const char* key, int value) {
+    if (!b || !key) return ERR_INVALID_ARG;
+    const std::string k(key);
+    if (k == "bundle_threads") {
+        if (value != 128 && value != 256 && value != 512 && value != 1024) return ERR_INVALID_ARG;
+        b->ctx.bundleThreads = value;
+    } else if (k == "bundle_rows") {
+        if (value < 1 || value > 8192) return ERR_INVALID_ARG;
+        b->ctx.bundleRows = value;
+    } else if (k == "bundle_tpr") {
+        if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8 && value != 16 && value != 64)
+            return ERR_INVALID_ARG;
+        b->ctx.bundleTpr = value;
+    } else if (k == "bundle_local") {
+        if (value < 1 || value > 65535) return ERR_INVALID_ARG;
+        b->ctx.bundleLocal = value;
+    } else if (k == "sell_c") {
+        if (value != 8 && value != 16 && value != 32 && value != 64) return ERR_INVALID_ARG;
+        b->ctx.sellC = value;
+    } else if (k == "sell_sigma") {
+        if (value < 1) return ERR_INVALID_ARG;
+        b->ctx.sellSigma = value;
+    } else {
+        return ERR_INVALID_ARG;
+    }
+    b->fn.reset();
+    return OK;
+}
+
+int race_batch_get_option(race_batch b, const char* key, int* value) {
+    if (!b || !key || !value) return ERR_INVALID_ARG;
+    const std::string k(key);
+    if (k == "bundle_threads") *value = b->ctx.bundleThreads;
+    else if (k == "bundle_rows") *value = b->ctx.bundleRows;
+    else if (k == "bundle_tpr") *value = b->ctx.bundleTpr;
+    else if (k == "bundle_local") *value = b->ctx.bundleLocal;
+    else if (k == "sell_c") *value = b->ctx.sellC;
+    else if (k == "sell_sigma") *value = b->ctx.sellSigma;
+    else return ERR_INVALID_ARG;
+    return OK;
+}
+
+int race_batch_build(race_batch b) {
+    if (!b || b->mats.empty()) return ERR_INVALID_ARG;
+    return buildBatch(b);
+}
+
+int race_batch_execute_async(race_batch b, int rev, void* stream) {
+    if (!b) return ERR_INVALID_ARG;
+    if (!b->fn) {
+        int e = race_batch_build(b);
+        if (e) return e;
+    }
+    return b->fn->run(rev != 0, stream ? (hipStream_t)stream : b->stream);
+}
+
+int race_batch_execute(race_batch b, int rev) {
+    int e = race_batch_execute_async(b, rev, nullptr);
+    if (e) return e;
+    return hipStreamSynchronize(b->stream) == hipSuccess ? OK : ERR_INTERNAL;
+}
+
+int race_batch_time(race_batch b, int rev, int reps, float* ms) {
+    if (!b || !ms || reps < 1) return ERR_INVALID_ARG;
+    if (!b->ev0 && hipEventCreate(&b->ev0) != hipSuccess) return ERR_INTERNAL;
+    if (!b->ev1 && hipEventCreate(&b->ev1) != hipSuccess) return ERR_INTERNAL;
+    int e = race_batch_execute_async(b, rev, nullptr);  // warm-up (and build)
+    if (e) return e;
+    if (hipEventRecord(b->ev0, b->stream) != hipSuccess) return ERR_INTERNAL;
+    for (int r = 0; r < reps; ++r)
+        if ((e = b->fn->run(rev != 0, b->stream))) return e;
+    if (hipEventRecord(b->ev1, b->stream) != hipSuccess) return ERR_INTERNAL;
+    if (hipEventSynchronize(b->ev1) != hipSuccess) return ERR_INTERNAL;
+    float t = 0;
+    if (hipEventElapsedTime(&t, b->ev0, b->ev1) != hipSuccess) return ERR_INTERNAL;
+    *ms = t / reps;
+    return OK;
+}
+
+int race_batch_candidates(int kind, int* count) {
+    if (!count) return ERR_INVALID_ARG;
+    *count = 12;
+    (void)kind;
+    return OK;
+}
+
+int race_batch_candidate(int kind, int index, int* p0, int* p1, int* p2) {
+    if (index < 0 || index >= 12 || !p0 || !p1 || !p2) return ERR_INVALID_ARG;
+    if (bundleKind(kind)) {
+        *p0 = kBundleCand[index][0];
+        *p1 = kBundleCand[index][1];
+        *p2 = kBundleCand[index][2];
+    } else {
+        *p0 = kSellCand[index][0];
+        *p1 = kSellCand[index][1];
+        *p2 = 0;
+    }
+    return OK;
+}
+
+int race_batch_use_candidate(race_batch b, int index) {
+    if (!b || index < 0 || index >= 12) return ERR_INVALID_ARG;
+    applyCandidate(b, index);
+    b->fn.reset();
+    return OK;
+}
+
+int race_batch_autotune(race_batch b, int rev, int reps, int* best, float* times, int cap) {
+    if (!b || b->mats.empty() || reps < 1) return ERR_INVALID_ARG;
+    int bi = -1;
+    float bt = 0;
+    for (int i = 0; i < 12; ++i) {
+        applyCandidate(b, i);
+        float t = -1;
+        int e = buildBatch(b);
+        if (e == OK) e = race_batch_time(b, rev, reps, &t);
+        if (e == ERR_INTERNAL) return e;
+        if (e != OK) t = -1;  // layout not possible for these matrices
+        if (times && i < cap) times[i] = t;
+        if (t >= 0 && (bi < 0 || t < bt)) {
+            bi = i;
+            bt = t;
+        }
+    }
+    if (bi < 0) return ERR_INCOMPATIBILITY;
+    applyCandidate(b, bi);
+    if (best) *best = bi;
+    return buildBatch(b);
+}
+
+int race_batch_destroy(race_batch b) {
+    delete b;
+    return OK;
 }
 
 int race_lb_split(const int* level_row, const int* level_nnz, int nlevels, int nthreads,

@@ -48,9 +48,23 @@ struct ExecContext {
     int bundleThreads = 512;  // workgroup size of the bundle kernels (128 ... 1024)
     int bundleTpr = 0;        // lanes per row in the bundle kernels (0: from the row length)
     int symmMode = 0;         // SymmSpMV: 0 auto, 1 colour phases, 2 SELL atomics, 3 expanded
+    int sellC = 64;           // batched SELL kernels: rows per chunk (8, 16, 32, 64)
+    int sellSigma = 1;        // batched SELL kernels: row-sorting window (1 = unsorted)
 };
 
 int makeDeviceFn(const race_kernel_desc& d, const ExecContext& ctx, std::unique_ptr<DeviceFn>* out);
+
+// one matrix of a batch: its colour schedules and its descriptor (host CRS
+// and device vectors)
+struct BatchItem {
+    const Schedule* fwd = nullptr;
+    const Schedule* rev = nullptr;
+    race_kernel_desc d{};
+};
+// GS / KACZ on a batch of matrices, fused launches per phase (bundle_exec.hip)
+int makeBundleBatchFn(const std::vector<BatchItem>& items, const ExecContext& ctx, std::unique_ptr<DeviceFn>* out);
+// SymmSpMV / SpMV on a batch of matrices, one launch (sell.hip)
+int makeSellBatchFn(const std::vector<BatchItem>& items, const ExecContext& ctx, std::unique_ptr<DeviceFn>* out);
 // Gauss-Seidel / Kaczmarz over leaf bundles (bundle_exec.hip)
 int makeBundleFn(const race_kernel_desc& d, const ExecContext& ctx, std::unique_ptr<DeviceFn>* out);
 // FP64 iterative refinement around an inner GS/KACZ function (refine.hip)

@@ -321,7 +321,218 @@ class MpkFn : public DeviceFn {
     }
 };
 
+// ---------------------------------------------------------------------------
+// Generalised SELL-C-sigma for batches and the layout tuner: chunk height
+// C in {8, 16, 32, 64} (a wavefront holds 64 / C chunks), rows sorted by
+// length (longest first) inside windows of sigma rows so that a chunk holds
+// rows of similar length, and every chunk tagged with its matrix, whose
+// vectors it uses.  Entry j of slot i of a chunk at off + j C + i.
+struct SellGChunk {
+    int rows, width, slot0, mat;
+    long off;
+};
+
+template <int C, bool PERM, typename XT, bool ACC>
+__global__ void __launch_bounds__(kBlock) k_sellg_spmv(const SellGChunk* __restrict__ chunks, int nChunk,
+                                                       const int* __restrict__ rowOf, const int* __restrict__ col,
+                                                       const XT* __restrict__ val, const XT* const* __restrict__ mx,
+                                                       XT* const* __restrict__ my, const int* __restrict__ mn,
+                                                       int nrhs) {
+    const int lane = threadIdx.x % 64;
+    const int c = (int)((blockIdx.x * (long)blockDim.x + threadIdx.x) / 64) * (64 / C) + lane / C;
+    const int i = lane % C;
+    if (c >= nChunk) return;
+    const SellGChunk k = chunks[c];
+    if (i >= k.rows) return;
+    const int row = PERM ? rowOf[k.slot0 + i] : k.slot0 + i;
+    const XT* __restrict__ x = mx[k.mat];
+    XT* __restrict__ y = my[k.mat];
+    const long n = mn[k.mat];
+    const int* __restrict__ cc = col + k.off + i;
+    const XT* __restrict__ vv = val + k.off + i;
+    for (int q = 0; q < nrhs; ++q) {
+        const XT* __restrict__ xq = x + q * n;
+        XT t = XT(0);
+        int j = 0;
+        for (; j + 2 <= k.width; j += 2) {
+            const int c0 = cc[j * C], c1 = cc[(j + 1) * C];
+            const XT v0 = vv[j * C], v1 = vv[(j + 1) * C];
+            t += v0 * xq[c0];
+            t += v1 * xq[c1];
+        }
+        if (j < k.width) t += vv[j * C] * xq[cc[j * C]];
+        if (ACC)
+            y[q * n + row] += t;
+        else
+            y[q * n + row] = t;
+    }
+}
+
+// full CRS of a symmetric matrix from its upper triangle (diagonal first):
+// row r holds its upper entries, then (c, r) of the rows c < r
+static void expandUpper(const race_kernel_desc& d, std::vector<int>& rp, std::vector<int>& fc,
+                        std::vector<double>& fv) {
+    const int n = d.nrow;
+    std::vector<int> cnt(n + 1, 0);
+    for (int r = 0; r < n; ++r)
+        for (int j = d.rowPtr[r]; j < d.rowPtr[r + 1]; ++j) {
+            ++cnt[r + 1];
+            if (d.col[j] != r) ++cnt[d.col[j] + 1];
+        }
+    for (int r = 0; r < n; ++r) cnt[r + 1] += cnt[r];
+    rp = cnt;
+    fc.assign(cnt[n], 0);
+    fv.assign(cnt[n], 0.0);
+    std::vector<int> pos(cnt.begin(), cnt.end() - 1);
+    for (int r = 0; r < n; ++r)
+        for (int j = d.rowPtr[r]; j < d.rowPtr[r + 1]; ++j) {
+            fc[pos[r]] = d.col[j];
+            fv[pos[r]++] = d.val[j];
+        }
+    for (int r = 0; r < n; ++r)
+        for (int j = d.rowPtr[r]; j < d.rowPtr[r + 1]; ++j) {
+            const int c = d.col[j];
+            if (c == r) continue;
+            fc[pos[c]] = r;
+            fv[pos[c]++] = d.val[j];
+        }
+}
+
+// SpMV (b = A x) or SymmSpMV (b += A x, expanded from the upper triangle)
+// on a batch of matrices in one launch.
+template <typename XT>
+class SellBatchFn : public DeviceFn {
+  public:
+    int C = 64, sigma = 1, nrhs = 1, nChunk = 0;
+    bool perm = false;
+    DevBuf<SellGChunk> chunks;
+    DevBuf<int> rowOf, col, mn;
+    DevBuf<XT> val;
+    DevBuf<const XT*> mx;
+    DevBuf<XT*> my;
+
+    int init(const std::vector<BatchItem>& items, const ExecContext& ctx) {
+        kind = items[0].d.kind;
+        nrhs = items[0].d.nrhs > 0 ? items[0].d.nrhs : 1;
+        C = ctx.sellC;
+        sigma = std::max(1, ctx.sellSigma);
+        perm = sigma > 1;
+        if (C != 8 && C != 16 && C != 32 && C != 64) return ERR_INVALID_ARG;
+        std::vector<SellGChunk> hc;
+        std::vector<int> hrow, hcol, hn;
+        std::vector<XT> hval;
+        std::vector<const XT*> hx;
+        std::vector<XT*> hy;
+        std::vector<int> rp, fc, order;
+        std::vector<double> fv;
+        for (size_t m = 0; m < items.size(); ++m) {
+            const race_kernel_desc& d = items[m].d;
+            const int n = d.nrow;
+            const int* R = d.rowPtr;
+            const int* Cc = d.col;
+            const double* V = d.val;
+            if (kind == RACE_K_SYMMSPMV) {
+                expandUpper(d, rp, fc, fv);
+                R = rp.data();
+                Cc = fc.data();
+                V = fv.data();
+            }
+            order.resize(n);
+            for (int r = 0; r < n; ++r) order[r] = r;
+            if (perm)
+                for (int w0 = 0; w0 < n; w0 += sigma) {
+                    const int w1 = std::min(n, w0 + sigma);
+                    std::stable_sort(order.begin() + w0, order.begin() + w1, [&](int a, int b) {
+                        return R[a + 1] - R[a] > R[b + 1] - R[b];
+                    });
+                }
+            for (int r0 = 0; r0 < n; r0 += C) {
+                const int rows = std::min(C, n - r0);
+                int w = 0;
+                for (int i = 0; i < rows; ++i) w = std::max(w, R[order[r0 + i] + 1] - R[order[r0 + i]]);
+                const long off = (long)hcol.size();
+                const int slot0 = perm ? (int)hrow.size() : r0;
+                hc.push_back(SellGChunk{rows, w, slot0, (int)m, off});
+                hcol.resize(off + (long)w * C);
+                hval.resize(off + (long)w * C);
+                for (int i = 0; i < C; ++i) {
+                    const int r = order[r0 + std::min(i, rows - 1)];
+                    const int len = i < rows ? R[r + 1] - R[r] : 0;
+                    for (int j = 0; j < w; ++j) {
+                        const long pos = off + (long)j * C + i;
+                        hcol[pos] = j < len ? Cc[R[r] + j] : r;
+                        hval[pos] = j < len ? (XT)V[R[r] + j] : XT(0);
+                    }
+                    if (perm && i < rows) hrow.push_back(r);
+                }
+            }
+            hx.push_back((const XT*)d.x);
+            hy.push_back((XT*)d.b);
+            hn.push_back(n);
+        }
+        nChunk = (int)hc.size();
+        int e;
+        if ((e = upload(chunks, hc.data(), hc.size()))) return e;
+        if ((e = upload(rowOf, hrow.data(), hrow.size()))) return e;
+        if ((e = upload(col, hcol.data(), hcol.size()))) return e;
+        if ((e = upload(val, hval.data(), hval.size()))) return e;
+        if ((e = upload(mx, hx.data(), hx.size()))) return e;
+        if ((e = upload(my, hy.data(), hy.size()))) return e;
+        if ((e = upload(mn, hn.data(), hn.size()))) return e;
+        return OK;
+    }
+    template <int CC, bool P>
+    void launch(hipStream_t st) {
+        const long waves = ((long)nChunk + (64 / CC) - 1) / (64 / CC);
+        const unsigned g = (unsigned)std::max(1l, (waves + kWavesPerBlock - 1) / kWavesPerBlock);
+        if (kind == RACE_K_SYMMSPMV)
+            hipLaunchKernelGGL((k_sellg_spmv<CC, P, XT, true>), dim3(g), dim3(kBlock), 0, st, chunks.p, nChunk,
+                               rowOf.p, col.p, val.p, mx.p, my.p, mn.p, nrhs);
+        else
+            hipLaunchKernelGGL((k_sellg_spmv<CC, P, XT, false>), dim3(g), dim3(kBlock), 0, st, chunks.p, nChunk,
+                               rowOf.p, col.p, val.p, mx.p, my.p, mn.p, nrhs);
+    }
+    int run(bool, hipStream_t st) override {
+        switch (C * 2 + (perm ? 1 : 0)) {
+            case 16: launch<8, false>(st); break;
+            case 17: launch<8, true>(st); break;
+            case 32: launch<16, false>(st); break;
+            case 33: launch<16, true>(st); break;
+            case 64: launch<32, false>(st); break;
+            case 65: launch<32, true>(st); break;
+            case 128: launch<64, false>(st); break;
+            default: launch<64, true>(st); break;
+        }
+        RACE_HIP(hipGetLastError());
+        return OK;
+    }
+};
+
 }  // namespace
+
+int makeSellBatchFn(const std::vector<BatchItem>& items, const ExecContext& ctx, std::unique_ptr<DeviceFn>* out) {
+    if (items.empty()) return ERR_INVALID_ARG;
+    const race_kernel_desc& d0 = items[0].d;
+    for (const BatchItem& it : items)
+        if (it.d.kind != d0.kind || it.d.val_precision != d0.val_precision ||
+            it.d.vec_precision != d0.vec_precision || it.d.nrhs != d0.nrhs)
+            return ERR_INVALID_ARG;
+    if (d0.kind != RACE_K_SYMMSPMV && d0.kind != RACE_K_SPMV) return ERR_INVALID_ARG;
+    if (d0.val_precision != d0.vec_precision) return ERR_NOT_IMPLEMENTED;
+    int e;
+    if (d0.vec_precision == RACE_FP64) {
+        auto f = std::make_unique<SellBatchFn<double>>();
+        e = f->init(items, ctx);
+        *out = std::move(f);
+    } else if (d0.vec_precision == RACE_FP32) {
+        auto f = std::make_unique<SellBatchFn<float>>();
+        e = f->init(items, ctx);
+        *out = std::move(f);
+    } else {
+        return ERR_NOT_IMPLEMENTED;
+    }
+    return e;
+}
 
 int makeSellSpmvFn(const race_kernel_desc& d, std::unique_ptr<DeviceFn>* out) {
     auto f = std::make_unique<SellSpmvFn>();

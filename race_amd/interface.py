@@ -306,6 +306,83 @@ class Interface:
         return ms.value, n.value
 
 
+class Batch:
+    """One GS / KACZ / SpMV / SymmSpMV function on many matrices, run as
+    fused launches (race_batch_* in the C ABI).  Each matrix is added with
+    its coloured Interface, its (permuted) CRS and its device vectors."""
+
+    def __init__(self, kind, val_precision=FP64, vec_precision=FP64, nrhs=1):
+        self._L = L.load()
+        self.kind = int(kind)
+        h = ctypes.c_void_p()
+        L.check(self._L.race_batch_create(ctypes.byref(h), self.kind, int(val_precision), int(vec_precision),
+                                          int(nrhs)), "race_batch_create")
+        self.h = h
+        self.val_precision, self.vec_precision, self.nrhs = int(val_precision), int(vec_precision), int(nrhs)
+        self._keep = []
+
+    def add(self, iface, rowPtr, col, val, x, b):
+        rp, c = _i32(rowPtr), _i32(col)
+        v = np.ascontiguousarray(val, dtype=np.float64)
+        d = L.KernelDesc(kind=self.kind, nrow=iface.nrow, rowPtr=_ip(rp), col=_ip(c),
+                         val=v.ctypes.data_as(L.f64p), val_precision=self.val_precision,
+                         vec_precision=self.vec_precision, nrhs=self.nrhs,
+                         x=x.ptr if isinstance(x, DeviceArray) else x,
+                         b=b.ptr if isinstance(b, DeviceArray) else b, power=1)
+        L.check(self._L.race_batch_add(self.h, iface.h, ctypes.byref(d)), "race_batch_add")
+        self._keep.append((iface, x, b))
+
+    def setOption(self, key, value):
+        L.check(self._L.race_batch_set_option(self.h, key.encode(), int(value)), "race_batch_set_option")
+
+    def getOption(self, key):
+        v = ctypes.c_int()
+        L.check(self._L.race_batch_get_option(self.h, key.encode(), ctypes.byref(v)), "race_batch_get_option")
+        return v.value
+
+    def build(self):
+        L.check(self._L.race_batch_build(self.h), "race_batch_build")
+
+    def execute(self, rev=False):
+        L.check(self._L.race_batch_execute(self.h, int(bool(rev))), "race_batch_execute")
+
+    def executeAsync(self, rev=False, stream=None):
+        L.check(self._L.race_batch_execute_async(self.h, int(bool(rev)), stream), "race_batch_execute_async")
+
+    def time(self, rev=False, reps=5):
+        ms = ctypes.c_float()
+        L.check(self._L.race_batch_time(self.h, int(bool(rev)), int(reps), ctypes.byref(ms)), "race_batch_time")
+        return float(ms.value)
+
+    def useCandidate(self, i):
+        L.check(self._L.race_batch_use_candidate(self.h, int(i)), "race_batch_use_candidate")
+
+    def autotune(self, rev=False, reps=3):
+        best = ctypes.c_int()
+        t = (ctypes.c_float * 12)()
+        L.check(self._L.race_batch_autotune(self.h, int(bool(rev)), int(reps), ctypes.byref(best), t, 12),
+                "race_batch_autotune")
+        return best.value, [float(v) for v in t]
+
+    @staticmethod
+    def candidate(kind, i):
+        a, b_, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        L.check(L.load().race_batch_candidate(int(kind), int(i), ctypes.byref(a), ctypes.byref(b_),
+                                              ctypes.byref(c)), "race_batch_candidate")
+        return a.value, b_.value, c.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._L.race_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def _sell_args(col, chunkStart, clp, val):
     """Checked views for the in-place SELL-C-sigma helpers."""
     for a, dt in ((col, np.int32), (chunkStart, np.int32), (clp, np.int32)):

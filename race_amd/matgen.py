@@ -201,3 +201,85 @@ def to_sell(rowPtr, col, val, C, pad_col=0):
             scol[base + j * C] = col[rowPtr[r] + j]
             sval[base + j * C] = val[rowPtr[r] + j]
     return cs, rl, clp, scol, sval
+
+
+def _symmetric_csr(n, rows, cols):
+    """Sorted, duplicate-free symmetric pattern with the diagonal, from an
+    edge list (both directions are added)."""
+    r = np.concatenate([rows, cols, np.arange(n)]).astype(np.int64)
+    c = np.concatenate([cols, rows, np.arange(n)]).astype(np.int64)
+    key = np.unique(r * n + c)
+    r, c = key // n, key % n
+    rp = np.zeros(n + 1, np.int64)
+    np.add.at(rp, r + 1, 1)
+    return np.cumsum(rp).astype(np.int32), c.astype(np.int32)
+
+
+def banded_random(n, mean_nnz, seed, band_frac=0.01):
+    """Random banded symmetric matrix: about mean_nnz entries per row (half of
+    them drawn per row, the rest from the transpose) at column offsets
+    uniform in [-bw, bw], bw = ceil(band_frac * n); off-diagonals U[-1,0)
+    (symmetric), diagonal 1 + sum |off| (strictly diagonally dominant)."""
+    rng = np.random.default_rng(seed)
+    bw = max(1, int(np.ceil(band_frac * n)))
+    k = max(1, int(round((mean_nnz - 1) / 2)))
+    rows = np.repeat(np.arange(n), k)
+    off = rng.integers(1, bw + 1, size=rows.size) * rng.choice([-1, 1], size=rows.size)
+    cols = rows + off
+    keep = (cols >= 0) & (cols < n)
+    rp, col = _symmetric_csr(n, rows[keep], cols[keep])
+    return rp, col, _dominant_values(rp, col, seed)
+
+
+def chung_lu_laplacian(n, mean_deg, seed, gamma=2.3, max_deg=None):
+    """Chung-Lu random graph with power-law expected degrees (exponent
+    gamma, mean mean_deg; expected degree capped at max_deg, default
+    min(sqrt(sum w), 400)), returned as the pattern of L + I with L the graph
+    Laplacian: off-diagonals -1, diagonal degree + 1."""
+    rng = np.random.default_rng(seed)
+    i = np.arange(n, dtype=np.float64)
+    w = (i + 1.0) ** (-1.0 / (gamma - 1.0))
+    w *= mean_deg * n / w.sum()
+    cap = max_deg if max_deg is not None else min(np.sqrt(w.sum()), 400.0)
+    w = np.minimum(w, cap)
+    w *= mean_deg * n / w.sum()
+    w = np.minimum(w, cap)
+    m = int(w.sum() / 2)
+    p = w / w.sum()
+    a = rng.choice(n, size=m, p=p)
+    b = rng.choice(n, size=m, p=p)
+    keep = a != b
+    perm = rng.permutation(n)  # hubs spread over the numbering
+    rp, col = _symmetric_csr(n, perm[a[keep]], perm[b[keep]])
+    r = row_of(rp)
+    deg = np.diff(rp) - 1
+    val = np.where(col == r, deg[r] + 1.0, -1.0)
+    return rp, col, val.astype(np.float64)
+
+
+def _dominant_values(rp, col, seed):
+    n = len(rp) - 1
+    r = row_of(rp)
+    u = pair_uniform(r, col, n, seed)
+    off = np.where(col == r, 0.0, -u)
+    s = np.zeros(n)
+    np.add.at(s, r, np.abs(off))
+    return np.where(col == r, 1.0 + s[r], off)
+
+
+def batch_matrices(count, seed, n_lo=2000, n_hi=200000):
+    """The batched workload: count matrices, n log-uniform in [n_lo, n_hi],
+    even indices random banded (band 1 % of n), odd indices Chung-Lu
+    Laplacians (exponent 2.3); mean entries per row uniform in [5, 40].
+    Yields (kind, n, mean, rowPtr, col, val)."""
+    rng = np.random.default_rng(seed)
+    for k in range(count):
+        n = int(np.exp(rng.uniform(np.log(n_lo), np.log(n_hi))))
+        mean = float(rng.uniform(5, 40))
+        s = int(rng.integers(1 << 30))
+        if k % 2 == 0:
+            rp, col, val = banded_random(n, mean, s)
+            yield "banded", n, mean, rp, col, val
+        else:
+            rp, col, val = chung_lu_laplacian(n, mean, s)
+            yield "chung-lu", n, mean, rp, col, val

@@ -30,6 +30,11 @@ def graphs():
     rp2 = np.concatenate([rp1, rp1[1:] + rp1[-1], [2 * rp1[-1] + 1]]).astype(np.int32)
     c2 = np.concatenate([c1, c1 + n1, [2 * n1]]).astype(np.int32)
     out.append(("islands", rp2, c2))
+    # 40 rows, each joined only to a private row: with distance 2 the range
+    # [0, 40) needs three levels per row
+    from .test_oracle import private_bridge_graph
+    rp3, c3 = private_bridge_graph(40)
+    out.append(("private-bridges", rp3, c3))
     return out
 
 
@@ -109,8 +114,12 @@ def _prepared(rp, col, val, T, dist, oracle):
     return ce, (prp, pcol, pval), zt
 
 
-@pytest.mark.parametrize("T,layout", [(1, 1), (4, 1), (16, 1), (64, 1), (64, 0), (256, 2), (1024, 2)])
-def test_gs_matches_race_order(oracle, T, layout):
+# (T, layout, bundle_rows, bundle_local): the last three force leaves larger
+# than a bundle, run as consecutive segments in sub-phases
+@pytest.mark.parametrize("T,layout,rows,local", [(1, 1, 0, 0), (4, 1, 0, 0), (16, 1, 0, 0), (64, 1, 0, 0),
+                                                 (64, 0, 0, 0), (256, 2, 128, 0), (1024, 2, 128, 0),
+                                                 (1, 2, 64, 512), (4, 2, 32, 256), (16, 2, 16, 128)])
+def test_gs_matches_race_order(oracle, T, layout, rows, local):
     rp, col = M.stencil27(12)
     val = M.gs_values(rp, col, seed=7)
     n = len(rp) - 1
@@ -120,8 +129,10 @@ def test_gs_matches_race_order(oracle, T, layout):
     dx = R.DeviceArray.from_host(x0)
     db = R.DeviceArray.from_host(b)
     ce.setOption("layout", layout)
-    if layout == 2:
-        ce.setOption("bundle_rows", 128)
+    if rows:
+        ce.setOption("bundle_rows", rows)
+    if local:
+        ce.setOption("bundle_local", local)
     fid = ce.registerDevice(R.K_GS, prp, pcol, pval, dx, db)
     xo = x0.copy()
     for rev in (False, True, False):
@@ -131,8 +142,9 @@ def test_gs_matches_race_order(oracle, T, layout):
         assert relerr(dx.download(), xo) <= TOL
 
 
-@pytest.mark.parametrize("T,layout", [(1, 1), (8, 1), (32, 1), (32, 0), (256, 2)])
-def test_kacz_matches_race_order(oracle, T, layout):
+@pytest.mark.parametrize("T,layout,rows,local", [(1, 1, 0, 0), (8, 1, 0, 0), (32, 1, 0, 0), (32, 0, 0, 0),
+                                                 (256, 2, 256, 2048), (1, 2, 64, 512), (8, 2, 32, 256)])
+def test_kacz_matches_race_order(oracle, T, layout, rows, local):
     rp, col = M.stencil7(14)
     val = M.jittered_values(rp, col, 0.1, seed=3)
     n = len(rp) - 1
@@ -141,9 +153,10 @@ def test_kacz_matches_race_order(oracle, T, layout):
     dx = R.DeviceArray.from_host(np.zeros(n))
     db = R.DeviceArray.from_host(b)
     ce.setOption("layout", layout)
-    if layout == 2:
-        ce.setOption("bundle_rows", 256)
-        ce.setOption("bundle_local", 2048)
+    if rows:
+        ce.setOption("bundle_rows", rows)
+    if local:
+        ce.setOption("bundle_local", local)
     fid = ce.registerDevice(R.K_KACZ, prp, pcol, pval, dx, db)
     xo = np.zeros(n)
     for rev in (False, True):

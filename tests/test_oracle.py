@@ -118,6 +118,39 @@ def test_bfs_distance2_bridges_through_rows_outside_the_range(oracle):
     assert d1.tolist() == [0, 1, 3, 5]
 
 
+def private_bridge_graph(k):
+    """Rows 0..k-1 (the range) each joined only to its own row k+i outside
+    the range."""
+    n = 2 * k
+    A = sp.lil_matrix((n, n))
+    for i in range(k):
+        A[i, k + i] = A[k + i, i] = 1
+    for r in range(n):
+        A[r, r] = 1
+    A = A.tocsr()
+    A.sort_indices()
+    return A.indptr.astype(np.int32), A.indices.astype(np.int32)
+
+
+def test_bfs_distance2_three_levels_per_isolated_row(oracle):
+    """Hand-traced: every row of the range [0,k) is an island whose only
+    neighbour is a private row outside the range.  Distance 2: the root's
+    level, one level expanding to its bridge, one expanding the bridge
+    (nothing new), then the next island restarts one empty level later --
+    row i at level 3i, 3k-1 levels in all (more than 2k+2)."""
+    k = 40
+    rp, col = private_bridge_graph(k)
+    G = oracle.Graph(rp, col, symm_hint=True)
+    lr, ln, d = G.traverse(oracle.TWO, 0, k, want_dist=True)
+    assert d.tolist() == [3 * i for i in range(k)]
+    assert len(lr) == 3 * k - 1
+    assert lr.tolist() == [1 if L % 3 == 0 else 0 for L in range(3 * k - 1)]
+    # distance 1 ignores the outside rows: islands two levels apart
+    G = oracle.Graph(rp, col, symm_hint=True)
+    lr, ln, d = G.traverse(oracle.ONE, 0, k, want_dist=True)
+    assert d.tolist() == [2 * i for i in range(k)]
+
+
 def test_subrange_traversal_composes_permutation(oracle):
     rp, col = M.stencil7(6)
     G = oracle.Graph(rp, col, symm_hint=True)
