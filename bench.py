@@ -131,6 +131,131 @@ def bench_mpk(R, M, g=256, k=8, cache_mb=512.0, reps=10):
     return out
 
 
+def bench_batched(args, R, M):
+    """Batched workload: args.batch_count matrices (n log-uniform in
+    [2000, 200000], half random banded, half Chung-Lu Laplacians), each
+    coloured by RACE (distance 2); one step = SymmSpMV + args.batch_sweeps
+    Kaczmarz sweeps (alternating forward / backward) on all of them, FP32
+    values and vectors, FP64 row norms, 8 right-hand sides.  Layouts are
+    tuned on the first 64 matrices and applied to the whole batch."""
+    nrhs, sweeps, count = 8, args.batch_sweeps, args.batch_count
+    tune_n = min(64, count)
+    kacz = R.Batch(R.K_KACZ, R.FP32, R.FP32, nrhs)
+    symm = R.Batch(R.K_SYMMSPMV, R.FP32, R.FP32, nrhs)
+    tk = R.Batch(R.K_KACZ, R.FP32, R.FP32, nrhs)
+    ts = R.Batch(R.K_SYMMSPMV, R.FP32, R.FP32, nrhs)
+    bytes_symm = bytes_kacz = 0
+    tot_n = tot_nnz = 0
+    kinds = {}
+    singles = []
+    cpu_sample = []
+    t0 = time.time()
+    gen = ((kd, n, mean, rp, col, val) for kd, n, mean, rp, col, val in M.batch_matrices(count, args.batch_seed)
+           if args.batch_kinds in ("both", kd))
+    for k, (kind, n, mean, rp, col, val) in enumerate(gen):
+        T = max(4, min(1024, n // 256))
+        ce = R.Interface(n, T, R.TWO, rp, col, symm_hint=True)
+        ce.RACEColor()
+        perm, inv = ce.getPerm(), ce.getInvPerm()
+        prp, pcol, pval = M.permute(rp, col, val, perm, inv)
+        prp, pcol, pval = M.make_diag_first(prp, pcol, pval)
+        urp, ucol, uval = M.upper_triangle(prp, pcol, pval)
+        nnz, nnzu = int(prp[-1]), int(urp[-1])
+        bvec = M.uniform_vec(n, 1000 + k, nrhs=nrhs).astype(np.float32)
+        xvec = M.uniform_vec(n, 5000 + k, nrhs=nrhs).astype(np.float32)
+        dx, db = R.DeviceArray.from_host(np.zeros(nrhs * n, np.float32)), R.DeviceArray.from_host(bvec)
+        sx, sb = R.DeviceArray.from_host(xvec), R.DeviceArray.from_host(np.zeros(nrhs * n, np.float32))
+        kacz.add(ce, prp, pcol, pval, dx, db)
+        symm.add(ce, urp, ucol, uval, sx, sb)
+        if k < tune_n:
+            tk.add(ce, prp, pcol, pval, dx, db)
+            ts.add(ce, urp, ucol, uval, sx, sb)
+            singles.append((ce, prp, pcol, pval, n))
+        if k < 8:
+            cpu_sample.append((ce, prp, pcol, pval, n, bvec.astype(np.float64)))
+        bytes_symm += nnzu * 8 + n * (4 + nrhs * 3 * 4)
+        bytes_kacz += nnz * 8 + n * (4 + 8 + nrhs * 3 * 4)
+        tot_n += n
+        tot_nnz += nnz
+        kinds[kind] = kinds.get(kind, 0) + 1
+        if (k + 1) % 32 == 0:
+            log(f"batched: {k + 1} matrices prepared ({time.time() - t0:.0f}s, {tot_nnz / 1e6:.0f}M nnz)")
+    count = k + 1
+    tune_n = min(tune_n, count)
+    t_prep = time.time() - t0
+    t0 = time.time()
+    bk, times_k = tk.autotune(False, reps=2)
+    bs, times_s = ts.autotune(False, reps=3)
+    log(f"tuned on {tune_n} matrices in {time.time() - t0:.1f}s: KACZ layout {bk} "
+        f"{R.Batch.candidate(R.K_KACZ, bk)}, SymmSpMV layout {bs} {R.Batch.candidate(R.K_SYMMSPMV, bs)}")
+    # batched against per-matrix launches on the tuning sample (forward sweep)
+    t_batch_sample = tk.time(False, reps=3)
+    t_single = 0.0
+    for ce, prp, pcol, pval, n in singles:
+        wx = R.DeviceArray(nrhs * n, np.float32)
+        wb = R.DeviceArray.from_host(np.ones(nrhs * n, np.float32))
+        fid = ce.registerDevice(R.K_KACZ, prp, pcol, pval, wx, wb, nrhs=nrhs, val_precision=R.FP32,
+                                vec_precision=R.FP32)
+        ce.executeFunction(fid)
+        ce.timerBegin()
+        for _ in range(3):
+            ce.executeAsync(fid)
+        t_single += ce.timerEnd() / 3
+    del tk, ts
+    t0 = time.time()
+    kacz.useCandidate(bk)
+    symm.useCandidate(bs)
+    kacz.build()
+    symm.build()
+    t_build = time.time() - t0
+    t_s = symm.time(False, reps=5)
+    t_f = kacz.time(False, reps=3)
+    t_b = kacz.time(True, reps=3)
+    ms = t_s + sweeps / 2 * (t_f + t_b)
+    total = bytes_symm + sweeps * bytes_kacz
+    gbs = total / (ms * 1e-3) / 1e9
+    log(f"batched step: SymmSpMV {t_s:.3f} ms + {sweeps} x KACZ ({t_f:.3f} / {t_b:.3f} ms) = {ms:.2f} ms, "
+        f"{gbs:.1f} GB/s; per-matrix launches on the sample: {t_single:.2f} ms vs batched {t_batch_sample:.2f} ms")
+    cpu = None
+    if not args.no_cpu:
+        from oracle import race_oracle as O
+        O.build()
+        cb = ct = 0.0
+        for ce, prp, pcol, pval, n, bv in cpu_sample:
+            pp, lo, hi = ce.schedule(False)
+            calls = np.stack([lo, hi], axis=1).reshape(-1)
+            xo = np.zeros(nrhs * n)
+            t1 = time.time()
+            O.run_calls(O.K_KACZ, prp, pcol, pval, xo, bv, calls, nrhs=nrhs)
+            ct += time.time() - t1
+            cb += int(prp[-1]) * 8 + n * (4 + 8 + nrhs * 3 * 4)
+        cpu = {"value": round(cb / ct / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
+               "sample": f"one forward KACZ sweep, 8 right-hand sides, on the first {len(cpu_sample)} matrices "
+                         f"of the batch (oracle C restatement in FP64, 1 thread, {ct * 1e3:.0f} ms)"}
+    return {
+        "metric": "batched SymmSpMV + 20 KACZ sweeps GB/s (FP32, 8 RHS)",
+        "value": round(gbs, 3), "unit": "GB/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"{count} matrices (seed {args.batch_seed}; {kinds}), n log-uniform in "
+                               f"[2000, 200000], total n={tot_n}, nnz={tot_nnz}; SymmSpMV + {sweeps} KACZ sweeps, "
+                               f"distance-2 colouring, FP32 values/vectors, FP64 row norms, {nrhs} RHS",
+                   "matrices": count, "rows": tot_n, "nnz": tot_nnz},
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(gbs / HBM_PEAK_GBS, 5), "traffic": None},
+        "cpu_baseline": cpu,
+        "batched": {"symmspmv_ms": round(t_s, 4), "kacz_fwd_ms": round(t_f, 4), "kacz_bwd_ms": round(t_b, 4),
+                    "tuned_kacz_layout": list(R.Batch.candidate(R.K_KACZ, bk)),
+                    "tuned_symmspmv_layout": list(R.Batch.candidate(R.K_SYMMSPMV, bs)),
+                    "tuner_kacz_ms": [round(t, 4) for t in times_k],
+                    "tuner_symmspmv_ms": [round(t, 4) for t in times_s],
+                    "sample_matrices": tune_n, "sample_per_matrix_launches_ms": round(t_single, 4),
+                    "sample_batched_ms": round(t_batch_sample, 4),
+                    "batching_speedup": round(t_single / t_batch_sample, 3),
+                    "prepare_s": round(t_prep, 1), "build_s": round(t_build, 1)},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -143,6 +268,13 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--inner-sweeps", type=int, default=4)
+    ap.add_argument("--workload", choices=["gs", "batched"], default="gs",
+                    help="gs: the Gauss-Seidel line (default); batched: many matrices at once")
+    ap.add_argument("--batch-count", type=int, default=512)
+    ap.add_argument("--batch-sweeps", type=int, default=20)
+    ap.add_argument("--batch-seed", type=int, default=2026)
+    ap.add_argument("--batch-kinds", choices=["both", "banded", "chung-lu"], default="both",
+                    help="keep only one matrix family of the batch")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the SymmSpMV 64^3 and MPK 256^3 measurements")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -155,6 +287,10 @@ def main():
 
     import race_amd as R
     from race_amd import matgen as M
+
+    if args.workload == "batched":
+        print(json.dumps(bench_batched(args, R, M)), flush=True)
+        return
 
     g = args.grid
     t0 = time.time()

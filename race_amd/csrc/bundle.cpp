@@ -209,14 +209,26 @@ BundleTable buildBundles(const Schedule& s, int kind, int n, const int* rp, cons
         curLocal = 0;
         ++bundleId;
     };
+    // every bundle starts a chain unless it continues a leaf's segments
+    bool continueChain = false;
+    auto markChain = [&]() {
+        // called after each flush: the bundle just written is slot.size()-2
+        if (!continueChain) t.chainPtr.push_back((int)t.slot.size() - 2);
+    };
     auto beginPhase = [&]() {
+        t.phaseChain.push_back((int)t.chainPtr.size());
         t.phaseBundle.push_back((int)t.slot.size() - 1);
         t.phRows.push_back(0);
         t.phLocal.push_back(0);
         t.phWaves.push_back(0);
     };
-    auto endPhase = [&]() {
+    auto flushMark = [&]() {
+        const bool had = !cur.empty();
         flush();
+        if (had) markChain();
+    };
+    auto endPhase = [&]() {
+        flushMark();
         t.critical += t.phWaves.back();
     };
     // add [lo, hi) to the bundle being filled, flushing it first when the
@@ -228,7 +240,7 @@ BundleTable buildBundles(const Schedule& s, int kind, int n, const int* rp, cons
         int add = countNew(lo, hi, bundleId);
         if (!cur.empty() && (curRows + rowsL > cap || curLocal + add > localCap)) {
             untag(lo, hi, bundleId);
-            flush();
+            flushMark();
             add = countNew(lo, hi, bundleId);
         }
         if (add > localCap) {
@@ -252,10 +264,10 @@ BundleTable buildBundles(const Schedule& s, int kind, int n, const int* rp, cons
         if (big.empty()) continue;
         // Leaves too large for one bundle: consecutive segments in the
         // leaf's execution order (forward lo..hi-1, reverse hi-1..lo), each
-        // segment as large as a bundle holds; segment k of every such leaf
-        // runs in sub-phase k, after segment k - 1.
+        // as large as a bundle holds, one bundle per segment.  The segments
+        // of a leaf form a chain that one workgroup runs in order; the
+        // chains of all such leaves of the phase run in one extra phase.
         segs.assign(big.size(), {});
-        size_t maxSeg = 0;
         for (size_t q = 0; q < big.size(); ++q) {
             const int lo = s.leafLo[big[q]], hi = s.leafHi[big[q]];
             int a = s.rev ? hi : lo;  // next unplaced row boundary
@@ -296,18 +308,30 @@ BundleTable buildBundles(const Schedule& s, int kind, int n, const int* rp, cons
                     a += len;
                 }
             }
-            maxSeg = std::max(maxSeg, segs[q].size());
         }
-        for (size_t k = 0; k < maxSeg; ++k) {
-            beginPhase();
-            for (size_t q = 0; q < big.size(); ++q)
-                if (k < segs[q].size() && !place(segs[q][k].first, segs[q][k].second)) {
+        beginPhase();
+        int deepest = 0;
+        for (size_t q = 0; q < big.size(); ++q) {
+            int chainWaves = 0;
+            for (size_t k = 0; k < segs[q].size(); ++k) {
+                continueChain = k > 0;
+                if (!place(segs[q][k].first, segs[q][k].second)) {
                     t.ok = false;
                     return t;
                 }
-            endPhase();
+                const int before = t.phWaves.back();
+                t.phWaves.back() = 0;
+                flushMark();
+                chainWaves += t.phWaves.back();
+                t.phWaves.back() = std::max(before, t.phWaves.back());
+            }
+            continueChain = false;
+            deepest = std::max(deepest, chainWaves);
         }
+        t.critical += deepest;  // a chain's waves run one after the other
     }
+    t.phaseChain.push_back((int)t.chainPtr.size());
+    t.chainPtr.push_back((int)t.slot.size() - 1);
     t.phaseBundle.push_back((int)t.slot.size() - 1);
     t.waveSlot.push_back(t.slot.back());
     return t;
@@ -334,6 +358,7 @@ BundleTable concatBundles(const std::vector<const BundleTable*>& tabs, std::vect
     t.extPtr.push_back(0);
     for (int p = 0; p < P; ++p) {
         t.phaseBundle.push_back((int)t.slot.size() - 1);
+        t.phaseChain.push_back((int)t.chainPtr.size());
         int rows = 0, local = 0, waves = 0;
         for (size_t m = 0; m < tabs.size(); ++m) {
             const BundleTable& a = *tabs[m];
@@ -341,6 +366,10 @@ BundleTable concatBundles(const std::vector<const BundleTable*>& tabs, std::vect
             rows = std::max(rows, a.phRows[p]);
             local = std::max(local, a.phLocal[p]);
             waves = std::max(waves, a.phWaves[p]);
+            // chains keep their bundles, renumbered
+            const int base = (int)t.slot.size() - 1;
+            for (int c = a.phaseChain[p]; c < a.phaseChain[p + 1]; ++c)
+                t.chainPtr.push_back(base + a.chainPtr[c] - a.phaseBundle[p]);
             for (int q = a.phaseBundle[p]; q < a.phaseBundle[p + 1]; ++q) {
                 const int s0 = a.slot[q], nb = a.slot[q + 1] - s0;
                 const int newS0 = t.slot.back();
@@ -367,6 +396,8 @@ BundleTable concatBundles(const std::vector<const BundleTable*>& tabs, std::vect
         t.phWaves.push_back(waves);
         t.critical += waves;
     }
+    t.phaseChain.push_back((int)t.chainPtr.size());
+    t.chainPtr.push_back((int)t.slot.size() - 1);
     t.phaseBundle.push_back((int)t.slot.size() - 1);
     t.waveSlot.push_back(t.slot.back());
     return t;

@@ -28,6 +28,10 @@ struct BundleTable {
     int cap = 128;                  // rows per bundle
     int tpr = 1;                    // lanes per row
     std::vector<int> phaseBundle;   // phase -> [first bundle, last bundle)
+    std::vector<int> phaseChain;    // phase -> [first chain, last chain)
+    std::vector<int> chainPtr;      // chain -> [first bundle, last bundle): run in order
+                                    // by one workgroup (a bundle alone, or the
+                                    // segments of a leaf too large for one bundle)
     std::vector<int> slot;          // bundle -> first slot
     std::vector<int> wave;          // bundle -> first wave
     std::vector<int> waveSlot;      // wave -> first slot (absolute)

@@ -10,8 +10,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdint>
 #include <cstring>
+#include <thread>
 #include <type_traits>
 
 #include "bundle.hpp"
@@ -81,64 +83,72 @@ __global__ void __launch_bounds__(BS) k_gs_bundle(
     const int* __restrict__ bLen, const int* __restrict__ bExt, const int* __restrict__ waveSlot,
     const int* __restrict__ rowId, const XT* __restrict__ diag, const int* __restrict__ ext,
     const uint16_t* __restrict__ sidx, const VT* __restrict__ sval, XT* __restrict__ x0,
-    const XT* __restrict__ b0, int n0, int nrhs, int bundle0, int nbnd, int xcd, int ldsLocal,
+    const XT* __restrict__ b0, int n0, int nrhs, const int* __restrict__ chainPtr, int chain0, int nchain, int xcd,
+    int ldsLocal,
     XT* const* __restrict__ bX, const XT* const* __restrict__ bB, const int* __restrict__ bN) {
     extern __shared__ __align__(16) unsigned char smem[];
     XT* xl = reinterpret_cast<XT*>(smem);
     int* ws = reinterpret_cast<int*>(xl + ldsLocal);
 
-    const int q = bundleOfBlock(nbnd, xcd);
+    const int q = bundleOfBlock(nchain, xcd);
     if (q < 0) return;
-    const int bnd = bundle0 + q;
-    const BundleVec<BATCH, XT> vec(x0, b0, n0, bX, bB, bN, bnd);
-    XT* __restrict__ x = vec.x;
-    const XT* __restrict__ b = vec.b;
-    const int n = vec.n;
-    const int s0 = bSlot[bnd], nb = bSlot[bnd + 1] - s0;
-    const long off = bOff[bnd];
-    const int wb = bLen[bnd];
-    const int w0 = bWave[bnd], nw = bWave[bnd + 1] - w0;
-    const int e0 = bExt[bnd], ne = bExt[bnd + 1] - e0;
-    const int t = threadIdx.x, i = t / TPR, l = t % TPR;
-    const bool active = i < nb;
-    const long lanes = (long)nb * TPR;
+    // a chain: one bundle, or the segments of a leaf too large for one bundle,
+    // run in order (each segment reads what the previous one wrote)
+    const int cLo = chainPtr[chain0 + q], cHi = chainPtr[chain0 + q + 1];
+    for (int bnd = cLo; bnd < cHi; ++bnd) {
+        const BundleVec<BATCH, XT> vec(x0, b0, n0, bX, bB, bN, bnd);
+        XT* __restrict__ x = vec.x;
+        const XT* __restrict__ b = vec.b;
+        const int n = vec.n;
+        const int s0 = bSlot[bnd], nb = bSlot[bnd + 1] - s0;
+        const long off = bOff[bnd];
+        const int wb = bLen[bnd];
+        const int w0 = bWave[bnd], nw = bWave[bnd + 1] - w0;
+        const int e0 = bExt[bnd], ne = bExt[bnd + 1] - e0;
+        const int t = threadIdx.x, i = t / TPR, l = t % TPR;
+        const bool active = i < nb;
+        const long lanes = (long)nb * TPR;
 
-    int li[E];
-    XT lv[E];
+        int li[E];
+        XT lv[E];
 #pragma unroll
-    for (int u = 0; u < E; ++u) {
-        const bool in = active && u < wb;
-        const long e = off + (long)u * lanes + t;
-        li[u] = in ? (int)sidx[e] : 0;
-        lv[u] = in ? loadVal<XT>(sval + e) : XT(0);
-    }
-    const int row = active ? rowId[s0 + i] : 0;
-    const XT dg = active ? diag[s0 + i] : XT(1);
-    for (int w = t; w <= nw; w += BS) ws[w] = waveSlot[w0 + w] - s0;
-    __syncthreads();
-    int myWave = -1;
-    if (active)
-        for (int w = 0; w < nw; ++w)
-            if (i >= ws[w] && i < ws[w + 1]) myWave = w;
-
-    const int nr = ONE_RHS ? 1 : nrhs;
-    for (int k = 0; k < nr; ++k) {
-        const size_t vo = (size_t)k * n;
-        for (int c = t; c < ne; c += BS) xl[nb + c] = x[vo + ext[e0 + c]];
-        if (active && l == 0) xl[i] = x[vo + row];
-        const XT br = (active && l == 0) ? b[vo + row] : XT(0);
+        for (int u = 0; u < E; ++u) {
+            const bool in = active && u < wb;
+            const long e = off + (long)u * lanes + t;
+            li[u] = in ? (int)sidx[e] : 0;
+            lv[u] = in ? loadVal<XT>(sval + e) : XT(0);
+        }
+        const int row = active ? rowId[s0 + i] : 0;
+        const XT dg = active ? diag[s0 + i] : XT(1);
+        for (int w = t; w <= nw; w += BS) ws[w] = waveSlot[w0 + w] - s0;
         __syncthreads();
-        for (int w = 0; w < nw; ++w) {
-            if (myWave == w) {
-                XT s = XT(0);
+        int myWave = -1;
+        if (active)
+            for (int w = 0; w < nw; ++w)
+                if (i >= ws[w] && i < ws[w + 1]) myWave = w;
+
+        // right-hand sides are independent: grid.y splits them over workgroups
+        const int k0 = ONE_RHS ? 0 : (int)blockIdx.y;
+        const int k1 = ONE_RHS ? 1 : (gridDim.y > 1 ? k0 + 1 : nrhs);
+        for (int k = k0; k < k1; ++k) {
+            const size_t vo = (size_t)k * n;
+            for (int c = t; c < ne; c += BS) xl[nb + c] = x[vo + ext[e0 + c]];
+            if (active && l == 0) xl[i] = x[vo + row];
+            const XT br = (active && l == 0) ? b[vo + row] : XT(0);
+            __syncthreads();
+            for (int w = 0; w < nw; ++w) {
+                if (myWave == w) {
+                    XT s = XT(0);
 #pragma unroll
-                for (int u = 0; u < E; ++u) s += lv[u] * xl[li[u]];
-                s = laneSum<TPR>(s);
-                if (l == 0) xl[i] = (br - s) / dg;
+                    for (int u = 0; u < E; ++u) s += lv[u] * xl[li[u]];
+                    s = laneSum<TPR>(s);
+                    if (l == 0) xl[i] = (br - s) / dg;
+                }
+                __syncthreads();
             }
+            if (active && l == 0) x[vo + row] = xl[i];
             __syncthreads();
         }
-        if (active && l == 0) x[vo + row] = xl[i];
         __syncthreads();
     }
 }
@@ -151,65 +161,74 @@ __global__ void __launch_bounds__(BS) k_kacz_bundle(
     const int* __restrict__ bLen, const int* __restrict__ bExt, const int* __restrict__ waveSlot,
     const int* __restrict__ rowId, const double* __restrict__ rowNorm, const int* __restrict__ ext,
     const uint16_t* __restrict__ sidx, const VT* __restrict__ sval, XT* __restrict__ x0,
-    const XT* __restrict__ b0, int n0, int nrhs, int bundle0, int nbnd, int xcd, int ldsLocal,
+    const XT* __restrict__ b0, int n0, int nrhs, const int* __restrict__ chainPtr, int chain0, int nchain, int xcd,
+    int ldsLocal,
     XT* const* __restrict__ bX, const XT* const* __restrict__ bB, const int* __restrict__ bN) {
     extern __shared__ __align__(16) unsigned char smem[];
     XT* xl = reinterpret_cast<XT*>(smem);
     int* ws = reinterpret_cast<int*>(xl + ldsLocal);
 
-    const int q = bundleOfBlock(nbnd, xcd);
+    const int q = bundleOfBlock(nchain, xcd);
     if (q < 0) return;
-    const int bnd = bundle0 + q;
-    const BundleVec<BATCH, XT> vec(x0, b0, n0, bX, bB, bN, bnd);
-    XT* __restrict__ x = vec.x;
-    const XT* __restrict__ b = vec.b;
-    const int n = vec.n;
-    const int s0 = bSlot[bnd], nb = bSlot[bnd + 1] - s0;
-    const long off = bOff[bnd];
-    const int wb = bLen[bnd];
-    const int w0 = bWave[bnd], nw = bWave[bnd + 1] - w0;
-    const int e0 = bExt[bnd], ne = bExt[bnd + 1] - e0;
-    const int t = threadIdx.x, i = t / TPR, l = t % TPR;
-    const bool active = i < nb;
-    const long lanes = (long)nb * TPR;
+    // a chain: one bundle, or the segments of a leaf too large for one bundle,
+    // run in order (each segment reads what the previous one wrote)
+    const int cLo = chainPtr[chain0 + q], cHi = chainPtr[chain0 + q + 1];
+    for (int bnd = cLo; bnd < cHi; ++bnd) {
+        const BundleVec<BATCH, XT> vec(x0, b0, n0, bX, bB, bN, bnd);
+        XT* __restrict__ x = vec.x;
+        const XT* __restrict__ b = vec.b;
+        const int n = vec.n;
+        const int s0 = bSlot[bnd], nb = bSlot[bnd + 1] - s0;
+        const long off = bOff[bnd];
+        const int wb = bLen[bnd];
+        const int w0 = bWave[bnd], nw = bWave[bnd + 1] - w0;
+        const int e0 = bExt[bnd], ne = bExt[bnd + 1] - e0;
+        const int t = threadIdx.x, i = t / TPR, l = t % TPR;
+        const bool active = i < nb;
+        const long lanes = (long)nb * TPR;
 
-    int li[E];
-    XT lv[E];
+        int li[E];
+        XT lv[E];
 #pragma unroll
-    for (int u = 0; u < E; ++u) {
-        const bool in = active && u < wb;
-        const long e = off + (long)u * lanes + t;
-        li[u] = in ? (int)sidx[e] : 0;
-        lv[u] = in ? loadVal<XT>(sval + e) : XT(0);
-    }
-    const int row = active ? rowId[s0 + i] : 0;
-    const double rn = active ? rowNorm[s0 + i] : 1.0;
-    for (int w = t; w <= nw; w += BS) ws[w] = waveSlot[w0 + w] - s0;
-    __syncthreads();
-    int myWave = -1;
-    if (active)
-        for (int w = 0; w < nw; ++w)
-            if (i >= ws[w] && i < ws[w + 1]) myWave = w;
-
-    for (int k = 0; k < nrhs; ++k) {
-        const size_t vo = (size_t)k * n;
-        for (int c = t; c < ne; c += BS) xl[c] = x[vo + ext[e0 + c]];
-        const XT br = active ? b[vo + row] : XT(0);
+        for (int u = 0; u < E; ++u) {
+            const bool in = active && u < wb;
+            const long e = off + (long)u * lanes + t;
+            li[u] = in ? (int)sidx[e] : 0;
+            lv[u] = in ? loadVal<XT>(sval + e) : XT(0);
+        }
+        const int row = active ? rowId[s0 + i] : 0;
+        const double rn = active ? rowNorm[s0 + i] : 1.0;
+        for (int w = t; w <= nw; w += BS) ws[w] = waveSlot[w0 + w] - s0;
         __syncthreads();
-        for (int w = 0; w < nw; ++w) {
-            if (myWave == w) {
-                XT dot = XT(0);
+        int myWave = -1;
+        if (active)
+            for (int w = 0; w < nw; ++w)
+                if (i >= ws[w] && i < ws[w + 1]) myWave = w;
+
+        // right-hand sides are independent: grid.y splits them over workgroups
+        const int k0 = (int)blockIdx.y;
+        const int k1 = gridDim.y > 1 ? k0 + 1 : nrhs;
+        for (int k = k0; k < k1; ++k) {
+            const size_t vo = (size_t)k * n;
+            for (int c = t; c < ne; c += BS) xl[c] = x[vo + ext[e0 + c]];
+            const XT br = active ? b[vo + row] : XT(0);
+            __syncthreads();
+            for (int w = 0; w < nw; ++w) {
+                if (myWave == w) {
+                    XT dot = XT(0);
 #pragma unroll
-                for (int u = 0; u < E; ++u) dot += lv[u] * xl[li[u]];
-                dot = laneSum<TPR>(dot);
-                const XT scale = (XT)(((double)dot - (double)br) / rn);
+                    for (int u = 0; u < E; ++u) dot += lv[u] * xl[li[u]];
+                    dot = laneSum<TPR>(dot);
+                    const XT scale = (XT)(((double)dot - (double)br) / rn);
 #pragma unroll
-                for (int u = 0; u < E; ++u)
-                    if (lv[u] != XT(0)) xl[li[u]] -= scale * lv[u];
+                    for (int u = 0; u < E; ++u)
+                        if (lv[u] != XT(0)) xl[li[u]] -= scale * lv[u];
+                }
+                __syncthreads();
             }
+            for (int c = t; c < ne; c += BS) x[vo + ext[e0 + c]] = xl[c];
             __syncthreads();
         }
-        for (int c = t; c < ne; c += BS) x[vo + ext[e0 + c]] = xl[c];
         __syncthreads();
     }
 }
@@ -239,7 +258,7 @@ int pickE(int tpr, int need) {
 template <typename VT, typename XT>
 struct BundleDev {
     BundleTable h;
-    DevBuf<int> slot, wave, len, ext, extPtr, waveSlot, rowId;
+    DevBuf<int> slot, wave, len, ext, extPtr, waveSlot, rowId, chainPtr;
     DevBuf<long> off;
     DevBuf<uint16_t> sidx;
     DevBuf<VT> sval;
@@ -255,6 +274,7 @@ struct BundleDev {
         if ((e = upload(ext, h.ext.data(), h.ext.size()))) return e;
         if ((e = upload(extPtr, h.extPtr.data(), h.extPtr.size()))) return e;
         if ((e = upload(waveSlot, h.waveSlot.data(), h.waveSlot.size()))) return e;
+        if ((e = upload(chainPtr, h.chainPtr.data(), h.chainPtr.size()))) return e;
         if ((e = upload(rowId, h.rowId.data(), h.rowId.size()))) return e;
         if ((e = upload(sidx, h.sidx.data(), h.sidx.size()))) return e;
         std::vector<VT> v(h.sval.size());
@@ -282,44 +302,45 @@ struct BundleDev {
     }
 };
 
-// one fused or plain launch of the bundles [b0, b0 + nbnd) of phase p
+// one fused or plain launch of the chains [c0, c0 + nchain) of phase p
 template <int BS, int TPR, int EE, typename VT, typename XT, bool BATCH>
-void launchBundles(int kind, int nrhs, const BundleDev<VT, XT>& d, int b0, int nbnd, int p, int xcd, XT* x,
+void launchBundles(int kind, int nrhs, const BundleDev<VT, XT>& d, int c0, int nchain, int p, int xcd, XT* x,
                    const XT* b, int n, XT* const* bX, const XT* const* bB, const int* bN, hipStream_t st) {
-    const unsigned grid = xcd ? 8u * (unsigned)((nbnd + 7) / 8) : (unsigned)nbnd;
+    const dim3 grid(xcd ? 8u * (unsigned)((nchain + 7) / 8) : (unsigned)nchain, (unsigned)std::max(1, nrhs));
     if (kind == RACE_K_KACZ)
-        hipLaunchKernelGGL((k_kacz_bundle<BS, TPR, EE, VT, XT, BATCH>), dim3(grid), dim3(BS), d.lds(p), st,
+        hipLaunchKernelGGL((k_kacz_bundle<BS, TPR, EE, VT, XT, BATCH>), grid, dim3(BS), d.lds(p), st,
                            d.slot.p, d.wave.p, d.off.p, d.len.p, d.extPtr.p, d.waveSlot.p, d.rowId.p,
-                           d.rowNorm.p, d.ext.p, d.sidx.p, d.sval.p, x, b, n, nrhs, b0, nbnd, xcd,
+                           d.rowNorm.p, d.ext.p, d.sidx.p, d.sval.p, x, b, n, nrhs, d.chainPtr.p, c0, nchain, xcd,
                            d.h.phLocal[p], bX, bB, bN);
     else if (nrhs == 1)
-        hipLaunchKernelGGL((k_gs_bundle<BS, TPR, EE, VT, XT, true, BATCH>), dim3(grid), dim3(BS), d.lds(p), st,
+        hipLaunchKernelGGL((k_gs_bundle<BS, TPR, EE, VT, XT, true, BATCH>), dim3(grid.x), dim3(BS), d.lds(p), st,
                            d.slot.p, d.wave.p, d.off.p, d.len.p, d.extPtr.p, d.waveSlot.p, d.rowId.p, d.diag.p,
-                           d.ext.p, d.sidx.p, d.sval.p, x, b, n, nrhs, b0, nbnd, xcd, d.h.phLocal[p], bX, bB, bN);
+                           d.ext.p, d.sidx.p, d.sval.p, x, b, n, nrhs, d.chainPtr.p, c0, nchain, xcd, d.h.phLocal[p], bX, bB,
+                           bN);
     else
-        hipLaunchKernelGGL((k_gs_bundle<BS, TPR, EE, VT, XT, false, BATCH>), dim3(grid), dim3(BS), d.lds(p),
+        hipLaunchKernelGGL((k_gs_bundle<BS, TPR, EE, VT, XT, false, BATCH>), grid, dim3(BS), d.lds(p),
                            st, d.slot.p, d.wave.p, d.off.p, d.len.p, d.extPtr.p, d.waveSlot.p, d.rowId.p,
-                           d.diag.p, d.ext.p, d.sidx.p, d.sval.p, x, b, n, nrhs, b0, nbnd, xcd, d.h.phLocal[p],
+                           d.diag.p, d.ext.p, d.sidx.p, d.sval.p, x, b, n, nrhs, d.chainPtr.p, c0, nchain, xcd, d.h.phLocal[p],
                            bX, bB, bN);
 }
 
 template <typename VT, typename XT, bool BATCH>
-void launchPhase(int bs, int tpr, int E, int kind, int nrhs, const BundleDev<VT, XT>& d, int b0, int nbnd, int p,
+void launchPhase(int bs, int tpr, int E, int kind, int nrhs, const BundleDev<VT, XT>& d, int c0, int nchain, int p,
                  int xcd, XT* x, const XT* b, int n, XT* const* bX, const XT* const* bB, const int* bN,
                  hipStream_t st) {
 #define RACE_BUNDLE_CASE(BS_)                                                                              \
     switch (tpr * 100 + E) {                                                                               \
-        case 104: launchBundles<BS_, 1, 4, VT, XT, BATCH>(kind, nrhs, d, b0, nbnd, p, xcd, x, b, n, bX, bB, bN, st); break;   \
-        case 108: launchBundles<BS_, 1, 8, VT, XT, BATCH>(kind, nrhs, d, b0, nbnd, p, xcd, x, b, n, bX, bB, bN, st); break;   \
-        case 116: launchBundles<BS_, 1, 16, VT, XT, BATCH>(kind, nrhs, d, b0, nbnd, p, xcd, x, b, n, bX, bB, bN, st); break;  \
-        case 208: launchBundles<BS_, 2, 8, VT, XT, BATCH>(kind, nrhs, d, b0, nbnd, p, xcd, x, b, n, bX, bB, bN, st); break;   \
-        case 213: launchBundles<BS_, 2, 13, VT, XT, BATCH>(kind, nrhs, d, b0, nbnd, p, xcd, x, b, n, bX, bB, bN, st); break;  \
-        case 216: launchBundles<BS_, 2, 16, VT, XT, BATCH>(kind, nrhs, d, b0, nbnd, p, xcd, x, b, n, bX, bB, bN, st); break;  \
-        case 416: launchBundles<BS_, 4, 16, VT, XT, BATCH>(kind, nrhs, d, b0, nbnd, p, xcd, x, b, n, bX, bB, bN, st); break;  \
-        case 816: launchBundles<BS_, 8, 16, VT, XT, BATCH>(kind, nrhs, d, b0, nbnd, p, xcd, x, b, n, bX, bB, bN, st); break;  \
-        case 1616: launchBundles<BS_, 16, 16, VT, XT, BATCH>(kind, nrhs, d, b0, nbnd, p, xcd, x, b, n, bX, bB, bN, st); break; \
-        case 6416: launchBundles<BS_, 64, 16, VT, XT, BATCH>(kind, nrhs, d, b0, nbnd, p, xcd, x, b, n, bX, bB, bN, st); break; \
-        default: launchBundles<BS_, 4, 8, VT, XT, BATCH>(kind, nrhs, d, b0, nbnd, p, xcd, x, b, n, bX, bB, bN, st); break;   \
+        case 104: launchBundles<BS_, 1, 4, VT, XT, BATCH>(kind, nrhs, d, c0, nchain, p, xcd, x, b, n, bX, bB, bN, st); break;   \
+        case 108: launchBundles<BS_, 1, 8, VT, XT, BATCH>(kind, nrhs, d, c0, nchain, p, xcd, x, b, n, bX, bB, bN, st); break;   \
+        case 116: launchBundles<BS_, 1, 16, VT, XT, BATCH>(kind, nrhs, d, c0, nchain, p, xcd, x, b, n, bX, bB, bN, st); break;  \
+        case 208: launchBundles<BS_, 2, 8, VT, XT, BATCH>(kind, nrhs, d, c0, nchain, p, xcd, x, b, n, bX, bB, bN, st); break;   \
+        case 213: launchBundles<BS_, 2, 13, VT, XT, BATCH>(kind, nrhs, d, c0, nchain, p, xcd, x, b, n, bX, bB, bN, st); break;  \
+        case 216: launchBundles<BS_, 2, 16, VT, XT, BATCH>(kind, nrhs, d, c0, nchain, p, xcd, x, b, n, bX, bB, bN, st); break;  \
+        case 416: launchBundles<BS_, 4, 16, VT, XT, BATCH>(kind, nrhs, d, c0, nchain, p, xcd, x, b, n, bX, bB, bN, st); break;  \
+        case 816: launchBundles<BS_, 8, 16, VT, XT, BATCH>(kind, nrhs, d, c0, nchain, p, xcd, x, b, n, bX, bB, bN, st); break;  \
+        case 1616: launchBundles<BS_, 16, 16, VT, XT, BATCH>(kind, nrhs, d, c0, nchain, p, xcd, x, b, n, bX, bB, bN, st); break; \
+        case 6416: launchBundles<BS_, 64, 16, VT, XT, BATCH>(kind, nrhs, d, c0, nchain, p, xcd, x, b, n, bX, bB, bN, st); break; \
+        default: launchBundles<BS_, 4, 8, VT, XT, BATCH>(kind, nrhs, d, c0, nchain, p, xcd, x, b, n, bX, bB, bN, st); break;   \
     }
     switch (bs) {
         case 128: RACE_BUNDLE_CASE(128) break;
@@ -393,11 +414,11 @@ class BundleFn : public DeviceFn {
             return fallback->run(rev, st);
         }
         const Dev& d = rev ? dr : df;
-        const std::vector<int>& pb = d.h.phaseBundle;
-        for (int p = 0; p + 1 < (int)pb.size(); ++p) {
-            const int b0 = pb[p], nbnd = pb[p + 1] - pb[p];
-            if (nbnd <= 0) continue;
-            launchPhase<VT, XT, false>(bs, tpr, E, kind, nrhs, d, b0, nbnd, p, xcd, (XT*)x, (const XT*)b, n,
+        const std::vector<int>& pc = d.h.phaseChain;
+        for (int p = 0; p + 1 < (int)pc.size(); ++p) {
+            const int c0 = pc[p], nchain = pc[p + 1] - pc[p];
+            if (nchain <= 0) continue;
+            launchPhase<VT, XT, false>(bs, tpr, E, kind, nrhs, d, c0, nchain, p, xcd, (XT*)x, (const XT*)b, n,
                                        nullptr, nullptr, nullptr, st);
         }
         RACE_HIP(hipGetLastError());
@@ -457,15 +478,27 @@ class BundleBatchFn : public DeviceFn {
                 std::vector<int> members;
                 for (size_t m = 0; m < items.size(); ++m)
                     if (shape[m] == (int)g) members.push_back((int)m);
+                // the matrices' tables are independent: build them on host threads
                 std::vector<BundleTable> tabs(members.size());
+                std::atomic<size_t> next{0};
+                std::atomic<bool> bad{false};
+                auto work = [&]() {
+                    for (size_t q; (q = next++) < members.size();) {
+                        const BatchItem& it = items[members[q]];
+                        tabs[q] = buildBundles(dir ? *it.rev : *it.fwd, kind, it.d.nrow, it.d.rowPtr, it.d.col,
+                                               it.d.val, rowsCap, ctx.bundleLocal, grp->tpr);
+                        if (!tabs[q].ok) bad = true;
+                    }
+                };
+                const size_t nth = std::min<size_t>(members.size(),
+                                                    std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
+                std::vector<std::thread> pool;
+                for (size_t t = 1; t < nth; ++t) pool.emplace_back(work);
+                work();
+                for (std::thread& th : pool) th.join();
+                if (bad) return ERR_INCOMPATIBILITY;
                 std::vector<const BundleTable*> ptrs;
-                for (size_t q = 0; q < members.size(); ++q) {
-                    const BatchItem& it = items[members[q]];
-                    tabs[q] = buildBundles(dir ? *it.rev : *it.fwd, kind, it.d.nrow, it.d.rowPtr, it.d.col,
-                                           it.d.val, rowsCap, ctx.bundleLocal, grp->tpr);
-                    if (!tabs[q].ok) return ERR_INCOMPATIBILITY;
-                    ptrs.push_back(&tabs[q]);
-                }
+                for (size_t q = 0; q < members.size(); ++q) ptrs.push_back(&tabs[q]);
                 std::vector<int> owner;
                 BundleTable fused = concatBundles(ptrs, &owner);
                 std::vector<BundleTable>().swap(tabs);
@@ -506,9 +539,9 @@ class BundleBatchFn : public DeviceFn {
             for (const auto& g : groups) {
                 const Dev& d = g->dev[dir];
                 if (p >= d.h.numPhases()) continue;
-                const int b0 = d.h.phaseBundle[p], nbnd = d.h.phaseBundle[p + 1] - b0;
-                if (nbnd <= 0) continue;
-                launchPhase<VT, XT, true>(bs, g->tpr, g->E, kind, nrhs, d, b0, nbnd, p, xcd, nullptr, nullptr, 0,
+                const int c0 = d.h.phaseChain[p], nchain = d.h.phaseChain[p + 1] - c0;
+                if (nchain <= 0) continue;
+                launchPhase<VT, XT, true>(bs, g->tpr, g->E, kind, nrhs, d, c0, nchain, p, xcd, nullptr, nullptr, 0,
                                           g->bx[dir].p, g->bb[dir].p, g->bn[dir].p, st);
             }
         RACE_HIP(hipGetLastError());
