@@ -220,6 +220,29 @@ int race_batch_use_candidate(race_batch b, int index);
 int race_batch_autotune(race_batch b, int rev, int reps, int *best, float *times, int cap);
 int race_batch_destroy(race_batch b);
 
+/* ---- the layout tuner's model (no counterpart in the reference) ----
+ * A 16-64-64-12 multilayer perceptron (ReLU hidden layers) mapping 16 matrix
+ * features to the predicted relative log-time of the 12 candidate layouts
+ * of race_batch_candidate.  Parameters: W1 [16][64], c1 [64], W2 [64][64],
+ * c2 [64], W3 [64][12], c3 [12], row-major, concatenated.  x: batch x 16,
+ * y/t/mask: batch x 12 (host arrays). */
+typedef struct race_mlp_s *race_mlp;
+int race_mlp_dims(int *n_in, int *n_hidden, int *n_out, int *n_params);
+int race_mlp_create(race_mlp *out, unsigned seed);
+int race_mlp_set_params(race_mlp m, const float *params, int count);
+int race_mlp_get_params(race_mlp m, float *params, int count);
+/* precision RACE_FP32 or RACE_BF16 (BF16 weights and activations on the
+ * matrix cores, FP32 accumulation) */
+int race_mlp_forward(race_mlp m, const float *x, int batch, float *y, int precision);
+/* gradients of sum(mask (y - t)^2) / sum(mask) with respect to the
+ * parameters (FP32), and that loss */
+int race_mlp_gradients(race_mlp m, const float *x, const float *t, const float *mask, int batch,
+                       float *grads, float *loss);
+/* epochs full-batch Adam steps (beta 0.9 / 0.999); loss of the last step */
+int race_mlp_train(race_mlp m, const float *x, const float *t, const float *mask, int batch, int epochs,
+                   float lr, float *loss);
+int race_mlp_destroy(race_mlp m);
+
 /* ---- SELL-C-sigma helpers ----
  * Entry j of row r of a SELL-C-sigma matrix sits at
  * chunk_start[r / C] + r % C + j * C; row_len[r] is the row length,

@@ -808,3 +808,49 @@ def simdify_d1(simdWidth, C, nrows, col, chunkStart, cl, val, dist2Compatible=Fa
                 row += 1
     return True
 
+
+# ---------------------------------------------------------------------------
+# The layout tuner's MLP (no counterpart in the reference): 16-64-64-12,
+# ReLU hidden layers, masked mean-squared error.  Parameters concatenated
+# W1 [16][64], c1, W2 [64][64], c2, W3 [64][12], c3 (row-major).
+
+def mlp_unpack(p, n_in=16, n_h=64, n_out=12):
+    p = np.asarray(p, np.float64)
+    o = 0
+    out = []
+    for shape in ((n_in, n_h), (n_h,), (n_h, n_h), (n_h,), (n_h, n_out), (n_out,)):
+        k = int(np.prod(shape))
+        out.append(p[o:o + k].reshape(shape))
+        o += k
+    assert o == len(p)
+    return out
+
+
+def mlp_forward(p, X):
+    W1, c1, W2, c2, W3, c3 = mlp_unpack(p)
+    h1 = np.maximum(np.asarray(X, np.float64) @ W1 + c1, 0.0)
+    h2 = np.maximum(h1 @ W2 + c2, 0.0)
+    return h2 @ W3 + c3
+
+
+def mlp_loss_grad(p, X, T, mask):
+    """sum(mask (y - t)^2) / sum(mask) and its gradient (back-propagation)."""
+    W1, c1, W2, c2, W3, c3 = mlp_unpack(p)
+    X = np.asarray(X, np.float64)
+    z1 = X @ W1 + c1
+    h1 = np.maximum(z1, 0.0)
+    z2 = h1 @ W2 + c2
+    h2 = np.maximum(z2, 0.0)
+    y = h2 @ W3 + c3
+    ms = float(np.sum(mask))
+    d = (y - T) * mask
+    loss = float(np.sum(d * d) / ms)
+    dy = 2.0 * d / ms
+    gW3, gc3 = h2.T @ dy, dy.sum(axis=0)
+    d2 = (dy @ W3.T) * (h2 > 0)
+    gW2, gc2 = h1.T @ d2, d2.sum(axis=0)
+    d1 = (d2 @ W2.T) * (h1 > 0)
+    gW1, gc1 = X.T @ d1, d1.sum(axis=0)
+    g = np.concatenate([a.reshape(-1) for a in (gW1, gc1, gW2, gc2, gW3, gc3)])
+    return loss, g
+

@@ -25,6 +25,7 @@ FP64, FP32, BF16 = 0, 1, 2
 
 i32p = ctypes.POINTER(ctypes.c_int)
 f64p = ctypes.POINTER(ctypes.c_double)
+f32p = ctypes.POINTER(ctypes.c_float)
 vp = ctypes.c_void_p
 
 
@@ -103,6 +104,16 @@ PROTOTYPES = {
     "race_batch_autotune": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, i32p,
                                            ctypes.POINTER(ctypes.c_float), ctypes.c_int]),
     "race_batch_destroy": (ctypes.c_int, [vp]),
+    "race_mlp_dims": (ctypes.c_int, [i32p, i32p, i32p, i32p]),
+    "race_mlp_create": (ctypes.c_int, [ctypes.POINTER(vp), ctypes.c_uint]),
+    "race_mlp_set_params": (ctypes.c_int, [vp, f32p, ctypes.c_int]),
+    "race_mlp_get_params": (ctypes.c_int, [vp, f32p, ctypes.c_int]),
+    "race_mlp_forward": (ctypes.c_int, [vp, f32p, ctypes.c_int, f32p, ctypes.c_int]),
+    "race_mlp_gradients": (ctypes.c_int, [vp, f32p, f32p, f32p, ctypes.c_int, f32p,
+                                          ctypes.POINTER(ctypes.c_float)]),
+    "race_mlp_train": (ctypes.c_int, [vp, f32p, f32p, f32p, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                      ctypes.POINTER(ctypes.c_float)]),
+    "race_mlp_destroy": (ctypes.c_int, [vp]),
     "race_sell_c_sigmize": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, i32p, i32p, i32p, i32p]),
     "race_simdify": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, i32p, i32p, i32p, i32p, vp,
                                     ctypes.c_int, ctypes.c_int]),

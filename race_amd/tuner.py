@@ -1,0 +1,296 @@
+"""Layout tuner: a 16-64-64-12 MLP that picks one of the 12 candidate layouts
+of a kernel family (race_batch_candidate) from 16 structural features of
+the matrix, instead of timing all 12 (race_batch_autotune).
+
+The model runs on the device through the C ABI (race_mlp_*): FP32 training
+(full-batch Adam), BF16 inference on the matrix cores.  Its target for a
+matrix is the relative log-time log(t_i / t_best) of every candidate i
+(masked where a layout does not fit); the prediction is the candidate with
+the smallest predicted value.
+
+    python -m race_amd.tuner sweep  --kind symmspmv --count 1200 --out data.npz
+    python -m race_amd.tuner train  --data data.npz --out model.npz
+    python -m race_amd.tuner score  --data data.npz --model model.npz
+"""
+import argparse
+import ctypes
+import json
+import time
+
+import numpy as np
+
+from . import _lib as L
+from . import matgen as M
+from .interface import BF16, FP32, FP64, K_GS, K_KACZ, K_SYMMSPMV, ONE, TWO, Batch, DeviceArray, Interface
+
+N_FEATURES, N_CANDIDATES = 16, 12
+KINDS = {"symmspmv": K_SYMMSPMV, "gs": K_GS, "kacz": K_KACZ}
+
+
+def _fp(a):
+    return a.ctypes.data_as(L.f32p)
+
+
+class MLP:
+    """The tuner's model (race_mlp_* in the C ABI)."""
+
+    def __init__(self, seed=0):
+        self._L = L.load()
+        h = ctypes.c_void_p()
+        L.check(self._L.race_mlp_create(ctypes.byref(h), ctypes.c_uint(seed)), "race_mlp_create")
+        self.h = h
+        d = [ctypes.c_int() for _ in range(4)]
+        L.check(self._L.race_mlp_dims(*[ctypes.byref(v) for v in d]), "race_mlp_dims")
+        self.n_in, self.n_hidden, self.n_out, self.n_params = (v.value for v in d)
+
+    def params(self):
+        p = np.empty(self.n_params, np.float32)
+        L.check(self._L.race_mlp_get_params(self.h, _fp(p), self.n_params), "race_mlp_get_params")
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        L.check(self._L.race_mlp_set_params(self.h, _fp(p), len(p)), "race_mlp_set_params")
+
+    def forward(self, x, precision=FP32):
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.empty((len(x), self.n_out), np.float32)
+        L.check(self._L.race_mlp_forward(self.h, _fp(x), len(x), _fp(y), int(precision)), "race_mlp_forward")
+        return y
+
+    def gradients(self, x, t, mask):
+        x, t, mask = (np.ascontiguousarray(a, np.float32) for a in (x, t, mask))
+        g = np.empty(self.n_params, np.float32)
+        loss = ctypes.c_float()
+        L.check(self._L.race_mlp_gradients(self.h, _fp(x), _fp(t), _fp(mask), len(x), _fp(g), ctypes.byref(loss)),
+                "race_mlp_gradients")
+        return g, float(loss.value)
+
+    def train(self, x, t, mask, epochs, lr):
+        x, t, mask = (np.ascontiguousarray(a, np.float32) for a in (x, t, mask))
+        loss = ctypes.c_float()
+        L.check(self._L.race_mlp_train(self.h, _fp(x), _fp(t), _fp(mask), len(x), int(epochs), ctypes.c_float(lr),
+                                       ctypes.byref(loss)), "race_mlp_train")
+        return float(loss.value)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._L.race_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+# features
+def bfs_levels(rowPtr, col):
+    """Level sizes of a breadth-first search from row 0 (islands restarted at
+    their smallest row), as numpy frontier expansion."""
+    n = len(rowPtr) - 1
+    rp = np.asarray(rowPtr, np.int64)
+    c = np.asarray(col, np.int64)
+    level = np.full(n, -1, np.int64)
+    sizes = []
+    nxt = 0
+    while True:
+        rest = np.nonzero(level < 0)[0]
+        if rest.size == 0:
+            break
+        front = rest[:1]
+        level[front] = nxt
+        while front.size:
+            sizes.append(front.size)
+            starts, lens = rp[front], rp[front + 1] - rp[front]
+            excl = np.cumsum(lens) - lens  # first position of each row's entries
+            idx = np.arange(lens.sum()) + np.repeat(starts - excl, lens)
+            nb = np.unique(c[idx])
+            nb = nb[level[nb] < 0]
+            nxt += 1
+            level[nb] = nxt
+            front = nb
+    return np.array(sizes, np.int64)
+
+
+def features(rowPtr, col):
+    """16 structural features of a square sparse pattern (CRS)."""
+    rp = np.asarray(rowPtr, np.int64)
+    c = np.asarray(col, np.int64)
+    n = len(rp) - 1
+    nnz = int(rp[-1])
+    rl = np.diff(rp).astype(np.float64)
+    mean = rl.mean()
+    r = np.repeat(np.arange(n), np.diff(rp))
+    dist = np.abs(c - r).astype(np.float64)
+    lv = bfs_levels(rp, c)
+    srt = np.sort(rl)
+    gini = 1.0 - 2.0 * np.sum(np.cumsum(srt)) / (n * srt.sum()) + 1.0 / n
+    return np.array([
+        np.log10(n), np.log10(max(nnz, 1)), mean, rl.std() / mean, np.log10(rl.max()), rl.min(),
+        np.mean(rl > 2 * mean), np.mean(rl > 32), dist.mean() / n, dist.max() / n, np.mean(dist <= 64),
+        np.log10(len(lv)), lv.max() / n, lv.mean() / lv.max(), gini, np.log10(nnz / len(lv)),
+    ], np.float64)
+
+
+# ---------------------------------------------------------------------------
+# sweeps (training data): every candidate timed on one matrix
+def tuner_matrices(count, seed, n_lo=2000, n_hi=100000):
+    """The batched workload's generators plus 3D 7- and 27-point stencils."""
+    rng = np.random.default_rng(seed)
+    gen = M.batch_matrices(count, seed + 1, n_lo, n_hi)
+    for k in range(count):
+        if k % 8 == 7:
+            g = int(rng.integers(14, 48))
+            rp, col = (M.stencil7 if k % 16 == 7 else M.stencil27)(g)
+            val = M.gs_values(rp, col, seed=k)
+            next(gen)
+            yield ("stencil7" if k % 16 == 7 else "stencil27"), rp, col, val
+        else:
+            name, n, mean, rp, col, val = next(gen)
+            yield name, rp, col, val
+
+
+def sweep_one(kind, rp, col, val, reps=3):
+    """(features, times[12]) of one matrix; times -1 where a layout does not
+    fit.  Matrix coloured by RACE with max(4, n / 256) threads."""
+    n = len(rp) - 1
+    T = max(4, min(1024, n // 256))
+    dist = ONE if kind == K_GS else TWO
+    ce = Interface(n, T, dist, rp, col, symm_hint=True)
+    ce.RACEColor()
+    perm, inv = ce.getPerm(), ce.getInvPerm()
+    prp, pcol, pval = M.permute(rp, col, val, perm, inv)
+    prp, pcol, pval = M.make_diag_first(prp, pcol, pval)
+    if kind == K_SYMMSPMV:
+        prp, pcol, pval = M.upper_triangle(prp, pcol, pval)
+    x = DeviceArray.from_host(M.uniform_vec(n, 7))
+    b = DeviceArray.from_host(M.uniform_vec(n, 8) if kind != K_SYMMSPMV else np.zeros(n))
+    bt = Batch(kind, FP64, FP64, 1)
+    bt.add(ce, prp, pcol, pval, x, b)
+    times = np.full(N_CANDIDATES, -1.0)
+    for i in range(N_CANDIDATES):
+        bt.useCandidate(i)
+        try:
+            times[i] = bt.time(False, reps)
+        except Exception:
+            times[i] = -1.0
+    return features(rp, col), times
+
+
+def targets(times):
+    """log(t_i / t_best) per matrix and the mask of the layouts that fit."""
+    t = np.atleast_2d(np.asarray(times, np.float64))
+    ok = t > 0
+    best = np.where(ok, t, np.inf).min(axis=1, keepdims=True)
+    best = np.where(np.isfinite(best), best, 1.0)
+    y = np.where(ok, np.log(np.where(ok, t, 1.0) / best), 0.0)
+    return y, ok.astype(np.float64)
+
+
+class Tuner:
+    """Feature normalisation + MLP; predict() picks a candidate index."""
+
+    def __init__(self, seed=0):
+        self.mlp = MLP(seed)
+        self.mu = np.zeros(N_FEATURES)
+        self.sd = np.ones(N_FEATURES)
+
+    def _norm(self, F):
+        return ((np.asarray(F, np.float64) - self.mu) / self.sd).astype(np.float32)
+
+    def fit(self, F, times, epochs=3000, lr=3e-3):
+        F = np.asarray(F, np.float64)
+        self.mu = F.mean(axis=0)
+        self.sd = F.std(axis=0) + 1e-9
+        y, m = targets(times)
+        return self.mlp.train(self._norm(F), y, m, epochs, lr)
+
+    def predict(self, F, precision=BF16):
+        y = self.mlp.forward(self._norm(np.atleast_2d(F)), precision)
+        return np.argmin(y, axis=1)
+
+    def save(self, path):
+        np.savez(path, params=self.mlp.params(), mu=self.mu, sd=self.sd)
+
+    def load(self, path):
+        z = np.load(path)
+        self.mlp.set_params(z["params"])
+        self.mu, self.sd = z["mu"], z["sd"]
+        return self
+
+
+def score(times, choice):
+    """Geometric mean over matrices of t_best / t_chosen (1 = always the
+    fastest layout)."""
+    t = np.asarray(times, np.float64)
+    best = np.where(t > 0, t, np.inf).min(axis=1)
+    chosen = t[np.arange(len(t)), choice]
+    chosen = np.where(chosen > 0, chosen, np.inf)
+    return float(np.exp(np.mean(np.log(best / chosen))))
+
+
+def _main(argv=None):
+    ap = argparse.ArgumentParser(prog="python -m race_amd.tuner")
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    s = sub.add_parser("sweep")
+    s.add_argument("--kind", choices=sorted(KINDS), default="symmspmv")
+    s.add_argument("--count", type=int, default=1200)
+    s.add_argument("--seed", type=int, default=11)
+    s.add_argument("--n-hi", type=int, default=100000)
+    s.add_argument("--out", required=True)
+    t = sub.add_parser("train")
+    t.add_argument("--data", required=True)
+    t.add_argument("--holdout", type=int, default=200)
+    t.add_argument("--epochs", type=int, default=3000)
+    t.add_argument("--out", required=True)
+    c = sub.add_parser("score")
+    c.add_argument("--data", required=True)
+    c.add_argument("--model", required=True)
+    c.add_argument("--holdout", type=int, default=200)
+    a = ap.parse_args(argv)
+    if a.cmd == "sweep":
+        kind = KINDS[a.kind]
+        F, T, names, secs = [], [], [], []
+        t0 = time.time()
+        for k, (name, rp, col, val) in enumerate(tuner_matrices(a.count, a.seed, n_hi=a.n_hi)):
+            t1 = time.time()
+            f, tm = sweep_one(kind, rp, col, val)
+            secs.append(time.time() - t1)
+            F.append(f)
+            T.append(tm)
+            names.append(name)
+            if (k + 1) % 50 == 0:
+                print(f"sweep {a.kind}: {k + 1}/{a.count} ({time.time() - t0:.0f}s)", flush=True)
+        np.savez(a.out, features=np.array(F), times=np.array(T), names=np.array(names), kind=a.kind,
+                 sweep_seconds=np.array(secs))
+    elif a.cmd == "train":
+        z = np.load(a.data)
+        F, T = z["features"], z["times"]
+        tr = slice(0, len(F) - a.holdout)
+        tu = Tuner(0)
+        t0 = time.time()
+        loss = tu.fit(F[tr], T[tr], epochs=a.epochs)
+        print(f"trained on {len(F) - a.holdout} matrices, loss {loss:.4f}, {time.time() - t0:.1f}s", flush=True)
+        tu.save(a.out)
+    else:
+        z = np.load(a.data)
+        F, T = z["features"], z["times"]
+        ho = slice(len(F) - a.holdout, len(F))
+        tu = Tuner(0).load(a.model)
+        t0 = time.time()
+        pick = tu.predict(F[ho])
+        t_pred = time.time() - t0
+        res = {"kind": str(z["kind"]), "heldout": int(len(F[ho])),
+               "score_mlp_bf16": round(score(T[ho], pick), 4),
+               "score_mlp_fp32": round(score(T[ho], tu.predict(F[ho], FP32)), 4),
+               "score_default_layout": round(score(T[ho], np.zeros(len(F[ho]), int)), 4),
+               "predict_seconds": round(t_pred, 4),
+               "autotune_seconds": round(float(z["sweep_seconds"][ho].sum()), 2)}
+        print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    _main()
