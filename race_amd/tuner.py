@@ -15,6 +15,7 @@ the smallest predicted value.
 import argparse
 import ctypes
 import json
+import os
 import time
 
 import numpy as np
@@ -153,9 +154,10 @@ def tuner_matrices(count, seed, n_lo=2000, n_hi=100000):
             yield name, rp, col, val
 
 
-def sweep_one(kind, rp, col, val, reps=3):
+def sweep_one(kind, rp, col, val, reps=3, stats=None):
     """(features, times[12]) of one matrix; times -1 where a layout does not
-    fit.  Matrix coloured by RACE with max(4, n / 256) threads."""
+    fit.  Matrix coloured by RACE with max(4, n / 256) threads.  stats, if
+    given, receives the seconds spent on features and on timing layouts."""
     n = len(rp) - 1
     T = max(4, min(1024, n // 256))
     dist = ONE if kind == K_GS else TWO
@@ -171,13 +173,19 @@ def sweep_one(kind, rp, col, val, reps=3):
     bt = Batch(kind, FP64, FP64, 1)
     bt.add(ce, prp, pcol, pval, x, b)
     times = np.full(N_CANDIDATES, -1.0)
+    t0 = time.time()
     for i in range(N_CANDIDATES):
         bt.useCandidate(i)
         try:
             times[i] = bt.time(False, reps)
         except Exception:
             times[i] = -1.0
-    return features(rp, col), times
+    t1 = time.time()
+    f = features(rp, col)
+    if stats is not None:
+        stats["autotune"] = t1 - t0
+        stats["features"] = time.time() - t1
+    return f, times
 
 
 def targets(times):
@@ -211,6 +219,13 @@ class Tuner:
     def predict(self, F, precision=BF16):
         y = self.mlp.forward(self._norm(np.atleast_2d(F)), precision)
         return np.argmin(y, axis=1)
+
+    @classmethod
+    def pretrained(cls, kind):
+        """The model shipped in race_amd/tuner_data for "symmspmv" or "gs"
+        (trained on the sweeps stored beside it)."""
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuner_data", f"{kind}_model.npz")
+        return cls(0).load(path)
 
     def save(self, path):
         np.savez(path, params=self.mlp.params(), mu=self.mu, sd=self.sd)
@@ -253,19 +268,20 @@ def _main(argv=None):
     a = ap.parse_args(argv)
     if a.cmd == "sweep":
         kind = KINDS[a.kind]
-        F, T, names, secs = [], [], [], []
+        F, T, names, secs, fsecs = [], [], [], [], []
         t0 = time.time()
         for k, (name, rp, col, val) in enumerate(tuner_matrices(a.count, a.seed, n_hi=a.n_hi)):
-            t1 = time.time()
-            f, tm = sweep_one(kind, rp, col, val)
-            secs.append(time.time() - t1)
+            st = {}
+            f, tm = sweep_one(kind, rp, col, val, stats=st)
+            secs.append(st["autotune"])
+            fsecs.append(st["features"])
             F.append(f)
             T.append(tm)
             names.append(name)
             if (k + 1) % 50 == 0:
                 print(f"sweep {a.kind}: {k + 1}/{a.count} ({time.time() - t0:.0f}s)", flush=True)
         np.savez(a.out, features=np.array(F), times=np.array(T), names=np.array(names), kind=a.kind,
-                 sweep_seconds=np.array(secs))
+                 sweep_seconds=np.array(secs), feature_seconds=np.array(fsecs))
     elif a.cmd == "train":
         z = np.load(a.data)
         F, T = z["features"], z["times"]
@@ -289,6 +305,10 @@ def _main(argv=None):
                "score_default_layout": round(score(T[ho], np.zeros(len(F[ho]), int)), 4),
                "predict_seconds": round(t_pred, 4),
                "autotune_seconds": round(float(z["sweep_seconds"][ho].sum()), 2)}
+        if "feature_seconds" in z:
+            fs = float(z["feature_seconds"][ho].sum())
+            res["feature_seconds"] = round(fs, 2)
+            res["cost_tuner_vs_autotune"] = round((fs + t_pred) / float(z["sweep_seconds"][ho].sum()), 4)
         print(json.dumps(res), flush=True)
 
 
