@@ -95,7 +95,7 @@ __global__ void __launch_bounds__(BS) k_gs_bundle(
     // a chain: one bundle, or the segments of a leaf too large for one bundle,
     // run in order (each segment reads what the previous one wrote)
     const int cLo = chainPtr[chain0 + q], cHi = chainPtr[chain0 + q + 1];
-    for (int bnd = cLo; bnd < cHi; ++bnd) {
+    auto runBundle = [&](const int bnd) {
         const BundleVec<BATCH, XT> vec(x0, b0, n0, bX, bB, bN, bnd);
         XT* __restrict__ x = vec.x;
         const XT* __restrict__ b = vec.b;
@@ -149,7 +149,15 @@ __global__ void __launch_bounds__(BS) k_gs_bundle(
             if (active && l == 0) x[vo + row] = xl[i];
             __syncthreads();
         }
-        __syncthreads();
+    };
+    // the common case, one bundle per workgroup, as its own straight-line copy
+    if (cHi == cLo + 1) {
+        runBundle(cLo);
+    } else {
+        for (int bnd = cLo; bnd < cHi; ++bnd) {
+            runBundle(bnd);
+            __syncthreads();
+        }
     }
 }
 
@@ -173,7 +181,7 @@ __global__ void __launch_bounds__(BS) k_kacz_bundle(
     // a chain: one bundle, or the segments of a leaf too large for one bundle,
     // run in order (each segment reads what the previous one wrote)
     const int cLo = chainPtr[chain0 + q], cHi = chainPtr[chain0 + q + 1];
-    for (int bnd = cLo; bnd < cHi; ++bnd) {
+    auto runBundle = [&](const int bnd) {
         const BundleVec<BATCH, XT> vec(x0, b0, n0, bX, bB, bN, bnd);
         XT* __restrict__ x = vec.x;
         const XT* __restrict__ b = vec.b;
@@ -229,7 +237,15 @@ __global__ void __launch_bounds__(BS) k_kacz_bundle(
             for (int c = t; c < ne; c += BS) x[vo + ext[e0 + c]] = xl[c];
             __syncthreads();
         }
-        __syncthreads();
+    };
+    // the common case, one bundle per workgroup, as its own straight-line copy
+    if (cHi == cLo + 1) {
+        runBundle(cLo);
+    } else {
+        for (int bnd = cLo; bnd < cHi; ++bnd) {
+            runBundle(bnd);
+            __syncthreads();
+        }
     }
 }
 
