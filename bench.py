@@ -39,11 +39,6 @@ def gs_bytes(n, nnz, sv=8, sx=8, si=4):
     return nnz * (sv + si) + n * (si + 3 * sx)
 
 
-def flush_copy(R, a, b):
-    """1 GiB device copy: evicts the L2s and the 256 MB Infinity Cache."""
-    R.interface.L.check(R.interface.L.load().race_device_copy(b.ptr, a.ptr, a.nbytes), "copy")
-
-
 def bench_symmspmv(R, M, g=64, threads=64, reps=200):
     """SymmSpMV b += A x, 7-pt g^3 (diag 6, off -1), upper triangle with the
     diagonal first, RACE distance-2 colouring.  Hot: back-to-back calls
@@ -70,8 +65,11 @@ def bench_symmspmv(R, M, g=64, threads=64, reps=200):
     hot = ce.timerEnd() / reps
     fa, fb = R.DeviceArray(1 << 27), R.DeviceArray(1 << 27)
     cold = []
+    st = ce.stream()
     for _ in range(20):
-        flush_copy(R, fa, fb)
+        # the 1 GiB copy on the handle's stream right before the call: the
+        # timer starts when the copy ends, with the call already queued
+        R.interface.L.check(R.interface.L.load().race_device_copy_async(fb.ptr, fa.ptr, fa.nbytes, st), "copy")
         ce.timerBegin()
         ce.executeAsync(fid)
         cold.append(ce.timerEnd())
@@ -80,6 +78,7 @@ def bench_symmspmv(R, M, g=64, threads=64, reps=200):
                        f"{threads} RACE threads, FP64",
            "bytes_per_call": byts, "hot_us": round(hot * 1e3, 2), "hot_gbs": round(byts / hot / 1e6, 1),
            "cold_us": round(cold * 1e3, 2), "cold_gbs": round(byts / cold / 1e6, 1),
+           "cold_method": "1 GiB device copy enqueued right before each timed call",
            "cold_frac_hbm": round(byts / cold / 1e6 / HBM_PEAK_GBS, 4)}
     log(f"SymmSpMV {g}^3: hot {out['hot_us']} us ({out['hot_gbs']} GB/s), cold {out['cold_us']} us "
         f"({out['cold_gbs']} GB/s)")

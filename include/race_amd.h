@@ -279,6 +279,10 @@ int race_memset_device(void *dst, int value, size_t bytes);
 int race_device_synchronize(void);
 /* streaming device-to-device copy by a plain HIP kernel (bytes % 16 == 0) */
 int race_device_copy(void *dst, const void *src, size_t bytes);
+/* the same copy enqueued on a HIP stream (NULL: the null stream), no wait */
+int race_device_copy_async(void *dst, const void *src, size_t bytes, void *stream);
+/* the handle's HIP stream (where race_execute and the timer run) */
+int race_get_stream(race_handle h, void **stream);
 int race_device_count(int *count);
 
 /* error text, RACE_error_string (src/error.cpp:26-60) */

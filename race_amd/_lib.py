@@ -104,6 +104,8 @@ PROTOTYPES = {
     "race_batch_autotune": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, i32p,
                                            ctypes.POINTER(ctypes.c_float), ctypes.c_int]),
     "race_batch_destroy": (ctypes.c_int, [vp]),
+    "race_device_copy_async": (ctypes.c_int, [vp, vp, ctypes.c_size_t, vp]),
+    "race_get_stream": (ctypes.c_int, [vp, ctypes.POINTER(vp)]),
     "race_mlp_dims": (ctypes.c_int, [i32p, i32p, i32p, i32p]),
     "race_mlp_create": (ctypes.c_int, [ctypes.POINTER(vp), ctypes.c_uint]),
     "race_mlp_set_params": (ctypes.c_int, [vp, f32p, ctypes.c_int]),

@@ -209,11 +209,17 @@ BundleTable buildBundles(const Schedule& s, int kind, int n, const int* rp, cons
         curLocal = 0;
         ++bundleId;
     };
-    // every bundle starts a chain unless it continues a leaf's segments
-    bool continueChain = false;
+    // Every bundle starts a chain, except inside a large leaf, whose
+    // bundles all belong to the chain started by its first one.
+    bool inLeafChain = false, chainOpen = false;
+    long chainWaves = 0;
     auto markChain = [&]() {
         // called after each flush: the bundle just written is slot.size()-2
-        if (!continueChain) t.chainPtr.push_back((int)t.slot.size() - 2);
+        if (!chainOpen) t.chainPtr.push_back((int)t.slot.size() - 2);
+        if (inLeafChain) {
+            chainOpen = true;
+            chainWaves += t.wave.back() - t.wave[t.wave.size() - 2];
+        }
     };
     auto beginPhase = [&]() {
         t.phaseChain.push_back((int)t.chainPtr.size());
@@ -254,7 +260,6 @@ BundleTable buildBundles(const Schedule& s, int kind, int n, const int* rp, cons
     };
 
     std::vector<int> big;
-    std::vector<std::vector<std::pair<int, int>>> segs;
     for (int p = 0; p < s.numPhases(); ++p) {
         beginPhase();
         big.clear();
@@ -262,70 +267,35 @@ BundleTable buildBundles(const Schedule& s, int kind, int n, const int* rp, cons
             if (!place(s.leafLo[l], s.leafHi[l])) big.push_back(l);
         endPhase();
         if (big.empty()) continue;
-        // Leaves too large for one bundle: consecutive segments in the
-        // leaf's execution order (forward lo..hi-1, reverse hi-1..lo), each
-        // as large as a bundle holds, one bundle per segment.  The segments
-        // of a leaf form a chain that one workgroup runs in order; the
-        // chains of all such leaves of the phase run in one extra phase.
-        segs.assign(big.size(), {});
+        // Leaves too large for one bundle: their rows in (depth, row) order
+        // (depth in the leaf's execution order, as for the waves), cut into
+        // bundles as large as the limits allow.  A row of depth d depends
+        // only on rows of smaller depth -- in an earlier bundle or an
+        // earlier wave of its own -- so running the bundles in this order
+        // reproduces the sequential leaf.  The bundles of a leaf form a
+        // chain that one workgroup runs in order; the chains of all such
+        // leaves of the phase run in one extra phase.
+        beginPhase();
+        long deepest = 0;
+        std::vector<int> ord;
         for (size_t q = 0; q < big.size(); ++q) {
             const int lo = s.leafLo[big[q]], hi = s.leafHi[big[q]];
-            int a = s.rev ? hi : lo;  // next unplaced row boundary
-            while (s.rev ? a > lo : a < hi) {
-                // grow the segment one row at a time up to the limits
-                int len = 0, local = 0;
-                const int tag = -(int)(1 << 30) + (int)q;  // private tags for the sizing pass
-                for (;;) {
-                    const int r = s.rev ? a - len - 1 : a + len;
-                    if (s.rev ? r < lo : r >= hi) break;
-                    if (len + 1 > cap) break;
-                    int add = gs ? 1 : 0;
-                    for (int j = rp[r]; j < rp[r + 1]; ++j) {
-                        const int c = col[j];
-                        if (gs && c == r) continue;
-                        if (seen[c] != tag) ++add;
-                    }
-                    if (local + add > localCap) break;
-                    for (int j = rp[r]; j < rp[r + 1]; ++j) seen[col[j]] = tag;
-                    if (gs) seen[r] = tag;
-                    local += add;
-                    ++len;
-                }
-                if (len == 0) {
+            ord.resize(hi - lo);
+            for (int r = lo; r < hi; ++r) ord[r - lo] = r;
+            std::sort(ord.begin(), ord.end(), [&](int a_, int b_) {
+                return depth[a_] != depth[b_] ? depth[a_] < depth[b_] : a_ < b_;
+            });
+            inLeafChain = true;
+            chainOpen = false;
+            chainWaves = 0;
+            for (int r : ord)
+                if (!place(r, r + 1)) {
                     t.ok = false;  // a single row exceeds the local column space
                     return t;
                 }
-                for (int k = 0; k < len; ++k) {
-                    const int r = s.rev ? a - k - 1 : a + k;
-                    for (int j = rp[r]; j < rp[r + 1]; ++j) seen[col[j]] = -2;
-                    seen[r] = -2;
-                }
-                if (s.rev) {
-                    segs[q].push_back({a - len, a});
-                    a -= len;
-                } else {
-                    segs[q].push_back({a, a + len});
-                    a += len;
-                }
-            }
-        }
-        beginPhase();
-        int deepest = 0;
-        for (size_t q = 0; q < big.size(); ++q) {
-            int chainWaves = 0;
-            for (size_t k = 0; k < segs[q].size(); ++k) {
-                continueChain = k > 0;
-                if (!place(segs[q][k].first, segs[q][k].second)) {
-                    t.ok = false;
-                    return t;
-                }
-                const int before = t.phWaves.back();
-                t.phWaves.back() = 0;
-                flushMark();
-                chainWaves += t.phWaves.back();
-                t.phWaves.back() = std::max(before, t.phWaves.back());
-            }
-            continueChain = false;
+            flushMark();
+            inLeafChain = false;
+            chainOpen = false;
             deepest = std::max(deepest, chainWaves);
         }
         t.critical += deepest;  // a chain's waves run one after the other

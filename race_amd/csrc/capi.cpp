@@ -905,6 +905,17 @@ int race_device_copy(void* dst, const void* src, size_t bytes) {
 
 int race_device_synchronize(void) { return hipDeviceSynchronize() == hipSuccess ? OK : ERR_INTERNAL; }
 
+int race_device_copy_async(void* dst, const void* src, size_t bytes, void* stream) {
+    if (!dst || !src) return ERR_INVALID_ARG;
+    return deviceCopy(dst, src, bytes, (hipStream_t)stream);
+}
+
+int race_get_stream(race_handle h, void** stream) {
+    if (checkHandle(h) || !stream) return ERR_INVALID_ARG;
+    *stream = (void*)h->stream;
+    return OK;
+}
+
 int race_device_count(int* count) {
     if (!count) return ERR_INVALID_ARG;
     int c = 0;

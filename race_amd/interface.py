@@ -269,6 +269,12 @@ class Interface:
         L.check(self._L.race_get_wave_count(self.h, fid, int(rev), ctypes.byref(v)), "waveCount")
         return v.value
 
+    def stream(self):
+        """the handle's HIP stream (as an integer handle)"""
+        v = ctypes.c_void_p()
+        L.check(self._L.race_get_stream(self.h, ctypes.byref(v)), "race_get_stream")
+        return v.value
+
     def timerBegin(self):
         L.check(self._L.race_timer_begin(self.h), "timerBegin")
 
