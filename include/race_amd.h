@@ -123,7 +123,9 @@ int race_register_device(race_handle h, const race_kernel_desc *desc, int *id);
  *   "bundle_local" LDS columns (rows + outside columns) per bundle (default 6144)
  *   "xcd_order"    1 = consecutive bundles of a phase run on the same XCD (default 0)
  *   "bundle_threads" workgroup size of the bundle kernels: 128, 256, 512 (default) or 1024
- *   "bundle_tpr"   lanes per matrix row in the bundle kernels: 1, 2, 4 (0 = auto)
+ *   "bundle_tpr"   lanes per matrix row in the bundle kernels: 1, 2, 4 (0 = auto:
+ *                  from the longest row; rows longer than 64 entries select
+ *                  variable lanes per row, 16 entries per lane)
  *   "symm_mode"    SymmSpMV: 1 = colour phases (CRS, one thread per row),
  *                  2 = one launch over a SELL-C upper triangle, FP64 atomics,
  *                  3 = one launch over the expanded full matrix (default, 0)

@@ -66,11 +66,18 @@ void leafDepths(const Schedule& s, int kind, int n, const int* rp, const int* co
 }  // namespace
 
 BundleTable buildBundles(const Schedule& s, int kind, int n, const int* rp, const int* col,
-                         const double* val, int cap, int localCap, int tpr) {
+                         const double* val, int cap, int localCap, int tpr, int laneCap) {
     BundleTable t;
     t.kind = kind;
     t.cap = cap;
     t.tpr = tpr;
+    const bool var = tpr == 0;
+    // lanes of a row in the variable layout (entries without the GS diagonal)
+    auto rowLanes = [&](int r) {
+        const int cnt = rp[r + 1] - rp[r] - (kind != RACE_K_KACZ ? 1 : 0);
+        return std::max(1, (cnt + kVarE - 1) / kVarE);
+    };
+    if (var) t.lanePtr.push_back(0);
     localCap = std::min(localCap, 65535);
     const bool gs = kind != RACE_K_KACZ;
     std::vector<int> depth;
@@ -87,7 +94,7 @@ BundleTable buildBundles(const Schedule& s, int kind, int n, const int* rp, cons
     // leaf too large for one bundle
     std::vector<std::pair<int, int>> cur;
     std::vector<int> rows, extCols;
-    int curRows = 0, curLocal = 0;
+    int curRows = 0, curLocal = 0, curLanes = 0;
     int bundleId = 0;  // id of the bundle being filled
 
     // local columns the rows [lo, hi) add to the bundle tagged `tag`
@@ -164,25 +171,37 @@ BundleTable buildBundles(const Schedule& s, int kind, int n, const int* rp, cons
         t.ext.insert(t.ext.end(), extCols.begin(), extCols.end());
         t.extPtr.push_back((int)t.ext.size());
         const int local = (gs ? nb : 0) + (int)extCols.size();
-        // entry block
-        int rowMax = 0;
-        for (int r : ordered) rowMax = std::max(rowMax, (rp[r + 1] - rp[r]) - (gs ? 1 : 0));
-        const int width = (rowMax + tpr - 1) / tpr;
+        // entry block: lane l of slot i at (lane0(i) + l), column-major
+        // over the bundle's lanes
+        std::vector<int> lane0(nb + 1, 0);
+        for (int i = 0; i < nb; ++i) lane0[i + 1] = lane0[i] + (var ? rowLanes(ordered[i]) : tpr);
+        const long lanes = lane0[nb];
+        int width = 0;
+        for (int i = 0; i < nb; ++i) {
+            const int cnt = (rp[ordered[i] + 1] - rp[ordered[i]]) - (gs ? 1 : 0);
+            const int nl = lane0[i + 1] - lane0[i];
+            width = std::max(width, (cnt + nl - 1) / nl);
+        }
         t.len.push_back(width);
         t.maxLen = std::max(t.maxLen, width);
         const long off = (long)t.sidx.size();
-        const long lanes = (long)nb * tpr;
         t.off.push_back(off);
         t.sidx.resize(off + (long)width * lanes);
         t.sval.resize(off + (long)width * lanes);
+        if (var) {
+            for (int i = 0; i < nb; ++i)
+                for (int l = lane0[i]; l < lane0[i + 1]; ++l) t.laneSlot.push_back((uint16_t)i);
+            t.lanePtr.push_back((int)t.laneSlot.size());
+        }
         for (int i = 0; i < nb; ++i) {
             const int r = ordered[i];
             t.rowId.push_back(r);
             const int j0 = gs ? rp[r] + 1 : rp[r];
             const int cntRow = rp[r + 1] - j0;
+            const int nl = lane0[i + 1] - lane0[i];
             double nrm = 0;
-            for (int k = 0; k < width * tpr; ++k) {
-                const long pos = off + (long)(k / tpr) * lanes + (long)i * tpr + (k % tpr);
+            for (int k = 0; k < width * nl; ++k) {
+                const long pos = off + (long)(k / nl) * lanes + lane0[i] + (k % nl);
                 if (k < cntRow) {
                     const int j = j0 + k;
                     t.sidx[pos] = (uint16_t)localOf[col[j]];
@@ -207,6 +226,7 @@ BundleTable buildBundles(const Schedule& s, int kind, int n, const int* rp, cons
         cur.clear();
         curRows = 0;
         curLocal = 0;
+        curLanes = 0;
         ++bundleId;
     };
     // Every bundle starts a chain, except inside a large leaf, whose
@@ -243,8 +263,14 @@ BundleTable buildBundles(const Schedule& s, int kind, int n, const int* rp, cons
     auto place = [&](int lo, int hi) {
         const int rowsL = hi - lo;
         if (rowsL > cap) return false;
+        int lanesL = 0;
+        if (var) {
+            for (int r = lo; r < hi; ++r) lanesL += rowLanes(r);
+            if (lanesL > laneCap) return false;
+        }
         int add = countNew(lo, hi, bundleId);
-        if (!cur.empty() && (curRows + rowsL > cap || curLocal + add > localCap)) {
+        if (!cur.empty() && (curRows + rowsL > cap || curLocal + add > localCap ||
+                             (var && curLanes + lanesL > laneCap))) {
             untag(lo, hi, bundleId);
             flushMark();
             add = countNew(lo, hi, bundleId);
@@ -256,6 +282,7 @@ BundleTable buildBundles(const Schedule& s, int kind, int n, const int* rp, cons
         cur.push_back({lo, hi});
         curRows += rowsL;
         curLocal += add;
+        curLanes += lanesL;
         return true;
     };
 
@@ -326,6 +353,7 @@ BundleTable concatBundles(const std::vector<const BundleTable*>& tabs, std::vect
     t.slot.push_back(0);
     t.wave.push_back(0);
     t.extPtr.push_back(0);
+    if (t.tpr == 0) t.lanePtr.push_back(0);
     for (int p = 0; p < P; ++p) {
         t.phaseBundle.push_back((int)t.slot.size() - 1);
         t.phaseChain.push_back((int)t.chainPtr.size());
@@ -351,9 +379,15 @@ BundleTable concatBundles(const std::vector<const BundleTable*>& tabs, std::vect
                 else
                     t.rowNorm.insert(t.rowNorm.end(), a.rowNorm.begin() + s0, a.rowNorm.begin() + s0 + nb);
                 t.slot.push_back(newS0 + nb);
-                const long block = (long)a.len[q] * nb * a.tpr;
+                const long lanesQ = a.tpr == 0 ? (long)(a.lanePtr[q + 1] - a.lanePtr[q]) : (long)nb * a.tpr;
+                const long block = (long)a.len[q] * lanesQ;
                 t.off.push_back((long)t.sidx.size());
                 t.len.push_back(a.len[q]);
+                if (a.tpr == 0) {
+                    t.laneSlot.insert(t.laneSlot.end(), a.laneSlot.begin() + a.lanePtr[q],
+                                      a.laneSlot.begin() + a.lanePtr[q + 1]);
+                    t.lanePtr.push_back((int)t.laneSlot.size());
+                }
                 t.sidx.insert(t.sidx.end(), a.sidx.begin() + a.off[q], a.sidx.begin() + a.off[q] + block);
                 t.sval.insert(t.sval.end(), a.sval.begin() + a.off[q], a.sval.begin() + a.off[q] + block);
                 t.ext.insert(t.ext.end(), a.ext.begin() + a.extPtr[q], a.ext.begin() + a.extPtr[q + 1]);

@@ -26,7 +26,7 @@ namespace race {
 struct BundleTable {
     int kind = 0;                   // RACE_K_GS or RACE_K_KACZ
     int cap = 128;                  // rows per bundle
-    int tpr = 1;                    // lanes per row
+    int tpr = 1;                    // lanes per row (0: variable, see lanePtr)
     std::vector<int> phaseBundle;   // phase -> [first bundle, last bundle)
     std::vector<int> phaseChain;    // phase -> [first chain, last chain)
     std::vector<int> chainPtr;      // chain -> [first bundle, last bundle): run in order
@@ -43,6 +43,11 @@ struct BundleTable {
     std::vector<int> rowId;         // slot -> matrix row
     std::vector<double> diag;       // slot -> diagonal value (GS)
     std::vector<double> rowNorm;    // slot -> sum of squares of the row (KACZ)
+    // variable lanes per row (tpr == 0): row of slot i gets max(1,
+    // ceil(len / kVarE)) lanes; entry k of the row goes to its lane k % nl
+    // at register k / nl; the bundle's block is column-major over its lanes
+    std::vector<int> lanePtr;       // bundle -> first lane
+    std::vector<uint16_t> laneSlot; // lane -> slot inside its bundle
     std::vector<uint16_t> sidx;     // entries: local column
     std::vector<double> sval;       // entries: value
     long critical = 0;              // sum over phases of the deepest bundle
@@ -57,8 +62,13 @@ struct BundleTable {
 // Build the bundle table of one sweep direction.  rowPtr/col/val: permuted
 // CRS with the diagonal first in each row.  localCap bounds the local column
 // space of a bundle (LDS entries, at most 65535).
+// register entries per lane in the variable-lanes layout
+constexpr int kVarE = 16;
+
+// tpr == 0 selects variable lanes per row; laneCap then bounds the lanes of
+// a bundle (the workgroup size) besides cap rows.
 BundleTable buildBundles(const Schedule& s, int kind, int n, const int* rowPtr, const int* col,
-                         const double* val, int cap, int localCap, int tpr);
+                         const double* val, int cap, int localCap, int tpr, int laneCap = 0);
 
 // Bundle tables of several matrices joined phase by phase: fused phase p
 // holds the bundles of phase p of every table that has one, in table order.

@@ -249,6 +249,199 @@ __global__ void __launch_bounds__(BS) k_kacz_bundle(
     }
 }
 
+// ---------------------------------------------------------------------------
+// Variable lanes per row (matrices with a few very long rows): the row of
+// slot i owns lanes [lane0(i), lane0(i+1)) of the bundle, each holding up to
+// kVarE entries in registers.  Every lane stores its partial sum in LDS and
+// the row's sum is read back in lane order (deterministic, no atomics), so
+// every wave takes two barriers.
+template <int BS, typename VT, typename XT, bool BATCH>
+__global__ void __launch_bounds__(BS) k_gs_bundle_vl(
+    const int* __restrict__ bSlot, const int* __restrict__ bWave, const long* __restrict__ bOff,
+    const int* __restrict__ bLen, const int* __restrict__ bExt, const int* __restrict__ waveSlot,
+    const int* __restrict__ rowId, const XT* __restrict__ diag, const int* __restrict__ ext,
+    const uint16_t* __restrict__ sidx, const VT* __restrict__ sval, XT* __restrict__ x0,
+    const XT* __restrict__ b0, int n0, int nrhs, const int* __restrict__ chainPtr, int chain0, int nchain, int xcd,
+    int ldsLocal, int ldsRows, const int* __restrict__ lanePtr, const uint16_t* __restrict__ laneSlot,
+    XT* const* __restrict__ bX, const XT* const* __restrict__ bB, const int* __restrict__ bN) {
+    constexpr int E = kVarE;
+    extern __shared__ __align__(16) unsigned char smem[];
+    XT* xl = reinterpret_cast<XT*>(smem);
+    XT* part = xl + ldsLocal;                               // BS partial sums
+    int* rl0 = reinterpret_cast<int*>(part + BS);           // row -> first lane
+    int* ws = rl0 + ldsRows + 1;
+
+    const int q = bundleOfBlock(nchain, xcd);
+    if (q < 0) return;
+    const int cLo = chainPtr[chain0 + q], cHi = chainPtr[chain0 + q + 1];
+    auto runBundle = [&](const int bnd) {
+        const BundleVec<BATCH, XT> vec(x0, b0, n0, bX, bB, bN, bnd);
+        XT* __restrict__ x = vec.x;
+        const XT* __restrict__ b = vec.b;
+        const int n = vec.n;
+        const int s0 = bSlot[bnd], nb = bSlot[bnd + 1] - s0;
+        const long off = bOff[bnd];
+        const int wb = bLen[bnd];
+        const int w0 = bWave[bnd], nw = bWave[bnd + 1] - w0;
+        const int e0 = bExt[bnd], ne = bExt[bnd + 1] - e0;
+        const int lp = lanePtr[bnd], nl = lanePtr[bnd + 1] - lp;
+        const int t = threadIdx.x;
+        const bool active = t < nl;
+        const int i = active ? (int)laneSlot[lp + t] : 0;
+        const bool first = active && (t == 0 || (int)laneSlot[lp + t - 1] != i);
+
+        int li[E];
+        XT lv[E];
+#pragma unroll
+        for (int u = 0; u < E; ++u) {
+            const bool in = active && u < wb;
+            const long e = off + (long)u * nl + t;
+            li[u] = in ? (int)sidx[e] : 0;
+            lv[u] = in ? loadVal<XT>(sval + e) : XT(0);
+        }
+        const int row = active ? rowId[s0 + i] : 0;
+        const XT dg = active ? diag[s0 + i] : XT(1);
+        for (int w = t; w <= nw; w += BS) ws[w] = waveSlot[w0 + w] - s0;
+        if (first) rl0[i] = t;
+        if (t == 0) rl0[nb] = nl;
+        __syncthreads();
+        const int rowL0 = active ? rl0[i] : 0, rowL1 = active ? rl0[i + 1] : 0;
+        int myWave = -1;
+        if (active)
+            for (int w = 0; w < nw; ++w)
+                if (i >= ws[w] && i < ws[w + 1]) myWave = w;
+
+        const int k0 = (int)blockIdx.y;
+        const int k1 = gridDim.y > 1 ? k0 + 1 : nrhs;
+        for (int k = k0; k < k1; ++k) {
+            const size_t vo = (size_t)k * n;
+            for (int c = t; c < ne; c += BS) xl[nb + c] = x[vo + ext[e0 + c]];
+            if (first) xl[i] = x[vo + row];
+            const XT br = first ? b[vo + row] : XT(0);
+            __syncthreads();
+            for (int w = 0; w < nw; ++w) {
+                if (myWave == w) {
+                    XT sum = XT(0);
+#pragma unroll
+                    for (int u = 0; u < E; ++u) sum += lv[u] * xl[li[u]];
+                    part[t] = sum;
+                }
+                __syncthreads();
+                if (myWave == w && first) {
+                    XT s = XT(0);
+                    for (int l = rowL0; l < rowL1; ++l) s += part[l];
+                    xl[i] = (br - s) / dg;
+                }
+                __syncthreads();
+            }
+            if (first) x[vo + row] = xl[i];
+            __syncthreads();
+        }
+    };
+    if (cHi == cLo + 1) {
+        runBundle(cLo);
+    } else {
+        for (int bnd = cLo; bnd < cHi; ++bnd) {
+            runBundle(bnd);
+            __syncthreads();
+        }
+    }
+}
+
+template <int BS, typename VT, typename XT, bool BATCH>
+__global__ void __launch_bounds__(BS) k_kacz_bundle_vl(
+    const int* __restrict__ bSlot, const int* __restrict__ bWave, const long* __restrict__ bOff,
+    const int* __restrict__ bLen, const int* __restrict__ bExt, const int* __restrict__ waveSlot,
+    const int* __restrict__ rowId, const double* __restrict__ rowNorm, const int* __restrict__ ext,
+    const uint16_t* __restrict__ sidx, const VT* __restrict__ sval, XT* __restrict__ x0,
+    const XT* __restrict__ b0, int n0, int nrhs, const int* __restrict__ chainPtr, int chain0, int nchain, int xcd,
+    int ldsLocal, int ldsRows, const int* __restrict__ lanePtr, const uint16_t* __restrict__ laneSlot,
+    XT* const* __restrict__ bX, const XT* const* __restrict__ bB, const int* __restrict__ bN) {
+    constexpr int E = kVarE;
+    extern __shared__ __align__(16) unsigned char smem[];
+    XT* xl = reinterpret_cast<XT*>(smem);
+    XT* part = xl + ldsLocal;                               // BS partial sums
+    int* rl0 = reinterpret_cast<int*>(part + BS);           // row -> first lane
+    int* ws = rl0 + ldsRows + 1;
+
+    const int q = bundleOfBlock(nchain, xcd);
+    if (q < 0) return;
+    const int cLo = chainPtr[chain0 + q], cHi = chainPtr[chain0 + q + 1];
+    auto runBundle = [&](const int bnd) {
+        const BundleVec<BATCH, XT> vec(x0, b0, n0, bX, bB, bN, bnd);
+        XT* __restrict__ x = vec.x;
+        const XT* __restrict__ b = vec.b;
+        const int n = vec.n;
+        const int s0 = bSlot[bnd], nb = bSlot[bnd + 1] - s0;
+        const long off = bOff[bnd];
+        const int wb = bLen[bnd];
+        const int w0 = bWave[bnd], nw = bWave[bnd + 1] - w0;
+        const int e0 = bExt[bnd], ne = bExt[bnd + 1] - e0;
+        const int lp = lanePtr[bnd], nl = lanePtr[bnd + 1] - lp;
+        const int t = threadIdx.x;
+        const bool active = t < nl;
+        const int i = active ? (int)laneSlot[lp + t] : 0;
+
+        int li[E];
+        XT lv[E];
+#pragma unroll
+        for (int u = 0; u < E; ++u) {
+            const bool in = active && u < wb;
+            const long e = off + (long)u * nl + t;
+            li[u] = in ? (int)sidx[e] : 0;
+            lv[u] = in ? loadVal<XT>(sval + e) : XT(0);
+        }
+        const int row = active ? rowId[s0 + i] : 0;
+        const double rn = active ? rowNorm[s0 + i] : 1.0;
+        for (int w = t; w <= nw; w += BS) ws[w] = waveSlot[w0 + w] - s0;
+        if (active && (t == 0 || (int)laneSlot[lp + t - 1] != i)) rl0[i] = t;
+        if (t == 0) rl0[nb] = nl;
+        __syncthreads();
+        const int rowL0 = active ? rl0[i] : 0, rowL1 = active ? rl0[i + 1] : 0;
+        int myWave = -1;
+        if (active)
+            for (int w = 0; w < nw; ++w)
+                if (i >= ws[w] && i < ws[w + 1]) myWave = w;
+
+        const int k0 = (int)blockIdx.y;
+        const int k1 = gridDim.y > 1 ? k0 + 1 : nrhs;
+        for (int k = k0; k < k1; ++k) {
+            const size_t vo = (size_t)k * n;
+            for (int c = t; c < ne; c += BS) xl[c] = x[vo + ext[e0 + c]];
+            const XT br = active ? b[vo + row] : XT(0);
+            __syncthreads();
+            for (int w = 0; w < nw; ++w) {
+                if (myWave == w) {
+                    XT dot = XT(0);
+#pragma unroll
+                    for (int u = 0; u < E; ++u) dot += lv[u] * xl[li[u]];
+                    part[t] = dot;
+                }
+                __syncthreads();
+                if (myWave == w) {
+                    XT rowDot = XT(0);
+                    for (int l = rowL0; l < rowL1; ++l) rowDot += part[l];
+                    const XT scale = (XT)(((double)rowDot - (double)br) / rn);
+#pragma unroll
+                    for (int u = 0; u < E; ++u)
+                        if (lv[u] != XT(0)) xl[li[u]] -= scale * lv[u];
+                }
+                __syncthreads();
+            }
+            for (int c = t; c < ne; c += BS) x[vo + ext[e0 + c]] = xl[c];
+            __syncthreads();
+        }
+    };
+    if (cHi == cLo + 1) {
+        runBundle(cLo);
+    } else {
+        for (int bnd = cLo; bnd < cHi; ++bnd) {
+            runBundle(bnd);
+            __syncthreads();
+        }
+    }
+}
+
 uint16_t toBf16(double v) {
     float f = (float)v;
     uint32_t u;
@@ -258,14 +451,13 @@ uint16_t toBf16(double v) {
     return (uint16_t)(u >> 16);
 }
 
-// (lanes per row, register width) pairs with a kernel instantiation: short
-// rows 1 or 2 lanes, long rows (graphs with hub vertices) up to a whole
-// wavefront of 64 lanes x 16 entries
+// (lanes per row, register width) pairs with a uniform kernel
+// instantiation; longer rows use variable lanes per row
 int pickE(int tpr, int need) {
-    static const int e1[] = {4, 8, 16}, e2[] = {8, 13, 16}, e4[] = {8, 16}, eL[] = {16};
-    const int* c = tpr == 1 ? e1 : tpr == 2 ? e2 : tpr == 4 ? e4 : eL;
-    const int nc = tpr == 1 ? 3 : tpr == 2 ? 3 : tpr == 4 ? 2 : 1;
-    if (tpr != 1 && tpr != 2 && tpr != 4 && tpr != 8 && tpr != 16 && tpr != 64) return -1;
+    static const int e1[] = {4, 8, 16}, e2[] = {8, 13, 16}, e4[] = {8, 16};
+    if (tpr != 1 && tpr != 2 && tpr != 4) return -1;
+    const int* c = tpr == 1 ? e1 : tpr == 2 ? e2 : e4;
+    const int nc = tpr == 4 ? 2 : 3;
     for (int q = 0; q < nc; ++q)
         if (c[q] >= need) return c[q];
     return -1;
@@ -274,7 +466,8 @@ int pickE(int tpr, int need) {
 template <typename VT, typename XT>
 struct BundleDev {
     BundleTable h;
-    DevBuf<int> slot, wave, len, ext, extPtr, waveSlot, rowId, chainPtr;
+    DevBuf<int> slot, wave, len, ext, extPtr, waveSlot, rowId, chainPtr, lanePtr;
+    DevBuf<uint16_t> laneSlot;
     DevBuf<long> off;
     DevBuf<uint16_t> sidx;
     DevBuf<VT> sval;
@@ -291,6 +484,10 @@ struct BundleDev {
         if ((e = upload(extPtr, h.extPtr.data(), h.extPtr.size()))) return e;
         if ((e = upload(waveSlot, h.waveSlot.data(), h.waveSlot.size()))) return e;
         if ((e = upload(chainPtr, h.chainPtr.data(), h.chainPtr.size()))) return e;
+        if (h.tpr == 0) {
+            if ((e = upload(lanePtr, h.lanePtr.data(), h.lanePtr.size()))) return e;
+            if ((e = upload(laneSlot, h.laneSlot.data(), h.laneSlot.size()))) return e;
+        }
         if ((e = upload(rowId, h.rowId.data(), h.rowId.size()))) return e;
         if ((e = upload(sidx, h.sidx.data(), h.sidx.size()))) return e;
         std::vector<VT> v(h.sval.size());
@@ -313,8 +510,12 @@ struct BundleDev {
         else
             return (VT)v;
     }
-    size_t lds(int p) const {
-        return (size_t)h.phLocal[p] * sizeof(XT) + (size_t)(h.phWaves[p] + 1) * sizeof(int) + 32;
+    // LDS bytes of phase p for workgroups of bs threads: the local column
+    // space, the wave table and, with variable lanes, a partial sum per lane
+    // and the first lane of every row
+    size_t lds(int p, int bs) const {
+        const size_t var = h.tpr == 0 ? (size_t)bs * sizeof(XT) + (size_t)(h.phRows[p] + 1) * sizeof(int) : 0;
+        return (size_t)h.phLocal[p] * sizeof(XT) + var + (size_t)(h.phWaves[p] + 1) * sizeof(int) + 32;
     }
 };
 
@@ -324,20 +525,37 @@ void launchBundles(int kind, int nrhs, const BundleDev<VT, XT>& d, int c0, int n
                    const XT* b, int n, XT* const* bX, const XT* const* bB, const int* bN, hipStream_t st) {
     const dim3 grid(xcd ? 8u * (unsigned)((nchain + 7) / 8) : (unsigned)nchain, (unsigned)std::max(1, nrhs));
     if (kind == RACE_K_KACZ)
-        hipLaunchKernelGGL((k_kacz_bundle<BS, TPR, EE, VT, XT, BATCH>), grid, dim3(BS), d.lds(p), st,
+        hipLaunchKernelGGL((k_kacz_bundle<BS, TPR, EE, VT, XT, BATCH>), grid, dim3(BS), d.lds(p, BS), st,
                            d.slot.p, d.wave.p, d.off.p, d.len.p, d.extPtr.p, d.waveSlot.p, d.rowId.p,
                            d.rowNorm.p, d.ext.p, d.sidx.p, d.sval.p, x, b, n, nrhs, d.chainPtr.p, c0, nchain, xcd,
                            d.h.phLocal[p], bX, bB, bN);
     else if (nrhs == 1)
-        hipLaunchKernelGGL((k_gs_bundle<BS, TPR, EE, VT, XT, true, BATCH>), dim3(grid.x), dim3(BS), d.lds(p), st,
+        hipLaunchKernelGGL((k_gs_bundle<BS, TPR, EE, VT, XT, true, BATCH>), dim3(grid.x), dim3(BS), d.lds(p, BS), st,
                            d.slot.p, d.wave.p, d.off.p, d.len.p, d.extPtr.p, d.waveSlot.p, d.rowId.p, d.diag.p,
                            d.ext.p, d.sidx.p, d.sval.p, x, b, n, nrhs, d.chainPtr.p, c0, nchain, xcd, d.h.phLocal[p], bX, bB,
                            bN);
     else
-        hipLaunchKernelGGL((k_gs_bundle<BS, TPR, EE, VT, XT, false, BATCH>), grid, dim3(BS), d.lds(p),
+        hipLaunchKernelGGL((k_gs_bundle<BS, TPR, EE, VT, XT, false, BATCH>), grid, dim3(BS), d.lds(p, BS),
                            st, d.slot.p, d.wave.p, d.off.p, d.len.p, d.extPtr.p, d.waveSlot.p, d.rowId.p,
                            d.diag.p, d.ext.p, d.sidx.p, d.sval.p, x, b, n, nrhs, d.chainPtr.p, c0, nchain, xcd, d.h.phLocal[p],
                            bX, bB, bN);
+}
+
+// the variable-lanes kernels (tpr == 0)
+template <int BS, typename VT, typename XT, bool BATCH>
+void launchBundlesVL(int kind, int nrhs, const BundleDev<VT, XT>& d, int c0, int nchain, int p, int xcd, XT* x,
+                     const XT* b, int n, XT* const* bX, const XT* const* bB, const int* bN, hipStream_t st) {
+    const dim3 grid(xcd ? 8u * (unsigned)((nchain + 7) / 8) : (unsigned)nchain, (unsigned)std::max(1, nrhs));
+    if (kind == RACE_K_KACZ)
+        hipLaunchKernelGGL((k_kacz_bundle_vl<BS, VT, XT, BATCH>), grid, dim3(BS), d.lds(p, BS), st, d.slot.p, d.wave.p,
+                           d.off.p, d.len.p, d.extPtr.p, d.waveSlot.p, d.rowId.p, d.rowNorm.p, d.ext.p, d.sidx.p,
+                           d.sval.p, x, b, n, nrhs, d.chainPtr.p, c0, nchain, xcd, d.h.phLocal[p], d.h.phRows[p],
+                           d.lanePtr.p, d.laneSlot.p, bX, bB, bN);
+    else
+        hipLaunchKernelGGL((k_gs_bundle_vl<BS, VT, XT, BATCH>), grid, dim3(BS), d.lds(p, BS), st, d.slot.p, d.wave.p,
+                           d.off.p, d.len.p, d.extPtr.p, d.waveSlot.p, d.rowId.p, d.diag.p, d.ext.p, d.sidx.p,
+                           d.sval.p, x, b, n, nrhs, d.chainPtr.p, c0, nchain, xcd, d.h.phLocal[p], d.h.phRows[p],
+                           d.lanePtr.p, d.laneSlot.p, bX, bB, bN);
 }
 
 template <typename VT, typename XT, bool BATCH>
@@ -353,9 +571,7 @@ void launchPhase(int bs, int tpr, int E, int kind, int nrhs, const BundleDev<VT,
         case 213: launchBundles<BS_, 2, 13, VT, XT, BATCH>(kind, nrhs, d, c0, nchain, p, xcd, x, b, n, bX, bB, bN, st); break;  \
         case 216: launchBundles<BS_, 2, 16, VT, XT, BATCH>(kind, nrhs, d, c0, nchain, p, xcd, x, b, n, bX, bB, bN, st); break;  \
         case 416: launchBundles<BS_, 4, 16, VT, XT, BATCH>(kind, nrhs, d, c0, nchain, p, xcd, x, b, n, bX, bB, bN, st); break;  \
-        case 816: launchBundles<BS_, 8, 16, VT, XT, BATCH>(kind, nrhs, d, c0, nchain, p, xcd, x, b, n, bX, bB, bN, st); break;  \
-        case 1616: launchBundles<BS_, 16, 16, VT, XT, BATCH>(kind, nrhs, d, c0, nchain, p, xcd, x, b, n, bX, bB, bN, st); break; \
-        case 6416: launchBundles<BS_, 64, 16, VT, XT, BATCH>(kind, nrhs, d, c0, nchain, p, xcd, x, b, n, bX, bB, bN, st); break; \
+        case 16: launchBundlesVL<BS_, VT, XT, BATCH>(kind, nrhs, d, c0, nchain, p, xcd, x, b, n, bX, bB, bN, st); break;        \
         default: launchBundles<BS_, 4, 8, VT, XT, BATCH>(kind, nrhs, d, c0, nchain, p, xcd, x, b, n, bX, bB, bN, st); break;   \
     }
     switch (bs) {
@@ -368,14 +584,24 @@ void launchPhase(int bs, int tpr, int E, int kind, int nrhs, const BundleDev<VT,
 }
 
 // lanes per row and register width for the longest row (diagonal excluded
-// for Gauss-Seidel); E < 0 if no kernel instantiation fits
+// for Gauss-Seidel); E < 0 if no kernel instantiation fits.  Rows longer
+// than 64 entries select variable lanes per row (tpr 0, kVarE registers
+// per lane), so a few long rows do not widen every row of the matrix.
 void pickShape(int kind, int maxRowLen, int forcedTpr, int* tpr, int* E) {
     int m = maxRowLen - (kind != RACE_K_KACZ ? 1 : 0);
     m = std::max(m, 1);
-    *tpr = forcedTpr > 0 ? forcedTpr
-                        : (m <= 16 ? 1 : m <= 32 ? 2 : m <= 64 ? 4 : m <= 128 ? 8 : m <= 256 ? 16 : 64);
+    if (forcedTpr <= 0 && m > 64) {
+        *tpr = 0;
+        *E = kVarE;
+        return;
+    }
+    *tpr = forcedTpr > 0 ? forcedTpr : (m <= 16 ? 1 : m <= 32 ? 2 : 4);
     *E = pickE(*tpr, (m + *tpr - 1) / *tpr);
 }
+
+// rows per bundle: the workgroup's lanes divided by the lanes per row, or
+// (variable lanes) the row limit alone, lanes checked by the builder
+int rowsCapFor(int bundleRows, int bs, int tpr) { return tpr > 0 ? std::min(bundleRows, bs / tpr) : bundleRows; }
 
 template <typename VT, typename XT>
 class BundleFn : public DeviceFn {
@@ -396,7 +622,7 @@ class BundleFn : public DeviceFn {
         int maxLen = 0;
         for (int r = 0; r < n; ++r) maxLen = std::max(maxLen, d.rowPtr[r + 1] - d.rowPtr[r]);
         pickShape(kind, maxLen, ctx.bundleTpr, &tpr, &E);
-        const int rowsCap = std::min(ctx.bundleRows, bs / tpr);
+        const int rowsCap = rowsCapFor(ctx.bundleRows, bs, tpr);
         const bool fp64 = std::is_same<VT, double>::value && std::is_same<XT, double>::value;
         auto fallBack = [&]() -> int {
             if (ctx.layout == 2 || !fp64) return ERR_INCOMPATIBILITY;  // bundles required
@@ -404,18 +630,18 @@ class BundleFn : public DeviceFn {
         };
         if (E < 0 || rowsCap < 1) return fallBack();
         BundleTable tf = buildBundles(*ctx.fwd, kind, n, d.rowPtr, d.col, d.val, rowsCap,
-                                      ctx.bundleLocal, tpr);
+                                      ctx.bundleLocal, tpr, bs);
         if (!tf.ok) return fallBack();
         BundleTable tr = buildBundles(*ctx.rev, kind, n, d.rowPtr, d.col, d.val, rowsCap,
-                                      ctx.bundleLocal, tpr);
+                                      ctx.bundleLocal, tpr, bs);
         if (!tr.ok) return fallBack();
         int e;
         if ((e = df.load(std::move(tf)))) return e;
         if ((e = dr.load(std::move(tr)))) return e;
         for (int p = 0; p < df.h.numPhases(); ++p)
-            if (df.lds(p) > kMaxLds) return fallBack();
+            if (df.lds(p, bs) > kMaxLds) return fallBack();
         for (int p = 0; p < dr.h.numPhases(); ++p)
-            if (dr.lds(p) > kMaxLds) return fallBack();
+            if (dr.lds(p, bs) > kMaxLds) return fallBack();
         return OK;
     }
 
@@ -479,7 +705,7 @@ class BundleBatchFn : public DeviceFn {
             int tpr, E;
             pickShape(kind, maxLen, ctx.bundleTpr, &tpr, &E);
             if (E < 0 && ctx.bundleTpr > 0) pickShape(kind, maxLen, 0, &tpr, &E);  // row too long for the forced width
-            if (E < 0 || std::min(ctx.bundleRows, bs / tpr) < 1) return ERR_INCOMPATIBILITY;
+            if (E < 0 || rowsCapFor(ctx.bundleRows, bs, tpr) < 1) return ERR_INCOMPATIBILITY;
             const std::pair<int, int> key(tpr, E);
             auto f = std::find(shapes.begin(), shapes.end(), key);
             shape[m] = (int)(f - shapes.begin());
@@ -489,7 +715,7 @@ class BundleBatchFn : public DeviceFn {
             auto grp = std::make_unique<Group>();
             grp->tpr = shapes[g].first;
             grp->E = shapes[g].second;
-            const int rowsCap = std::min(ctx.bundleRows, bs / grp->tpr);
+            const int rowsCap = rowsCapFor(ctx.bundleRows, bs, grp->tpr);
             for (int dir = 0; dir < 2; ++dir) {
                 std::vector<int> members;
                 for (size_t m = 0; m < items.size(); ++m)
@@ -502,7 +728,7 @@ class BundleBatchFn : public DeviceFn {
                     for (size_t q; (q = next++) < members.size();) {
                         const BatchItem& it = items[members[q]];
                         tabs[q] = buildBundles(dir ? *it.rev : *it.fwd, kind, it.d.nrow, it.d.rowPtr, it.d.col,
-                                               it.d.val, rowsCap, ctx.bundleLocal, grp->tpr);
+                                               it.d.val, rowsCap, ctx.bundleLocal, grp->tpr, bs);
                         if (!tabs[q].ok) bad = true;
                     }
                 };
@@ -533,7 +759,7 @@ class BundleBatchFn : public DeviceFn {
                 if ((e = upload(grp->bn[dir], hn.data(), hn.size()))) return e;
                 if ((e = grp->dev[dir].load(std::move(fused)))) return e;
                 for (int p = 0; p < grp->dev[dir].h.numPhases(); ++p)
-                    if (grp->dev[dir].lds(p) > kMaxLds) return ERR_INCOMPATIBILITY;
+                    if (grp->dev[dir].lds(p, bs) > kMaxLds) return ERR_INCOMPATIBILITY;
                 maxPhases[dir] = std::max(maxPhases[dir], grp->dev[dir].h.numPhases());
             }
             groups.push_back(std::move(grp));

@@ -407,8 +407,7 @@ int race_set_option(race_handle h, const char* key, int value) {
         if (value != 128 && value != 256 && value != 512 && value != 1024) return ERR_INVALID_ARG;
         h->optBundleThreads = value;
     } else if (k == "bundle_tpr") {
-        if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8 && value != 16 && value != 64)
-            return ERR_INVALID_ARG;
+        if (value != 0 && value != 1 && value != 2 && value != 4) return ERR_INVALID_ARG;
         h->optBundleTpr = value;
     } else if (k == "bundle_local") {
         if (value < 1 || value > 65535) return ERR_INVALID_ARG;
@@ -735,8 +734,7 @@ int race_batch_set_option(race_batch b, const char* key, int value) {
         if (value < 1 || value > 8192) return ERR_INVALID_ARG;
         b->ctx.bundleRows = value;
     } else if (k == "bundle_tpr") {
-        if (value != 0 && value != 1 && value != 2 && value != 4 && value != 8 && value != 16 && value != 64)
-            return ERR_INVALID_ARG;
+        if (value != 0 && value != 1 && value != 2 && value != 4) return ERR_INVALID_ARG;
         b->ctx.bundleTpr = value;
     } else if (k == "bundle_local") {
         if (value < 1 || value > 65535) return ERR_INVALID_ARG;

@@ -165,6 +165,34 @@ def test_kacz_matches_race_order(oracle, T, layout, rows, local):
         assert relerr(dx.download(), xo) <= TOL
 
 
+@pytest.mark.parametrize("kind", ["gs", "kacz"])
+@pytest.mark.parametrize("T,rows", [(4, 0), (16, 32)])
+def test_long_rows_variable_lanes(oracle, kind, T, rows):
+    """A Chung-Lu graph with hub rows of up to ~300 entries: rows longer
+    than 64 entries select variable lanes per row (LDS row accumulators);
+    large leaves run as chains.  FP64 against the oracle in RACE order."""
+    rp, col, val = M.chung_lu_laplacian(2500, 12.0, seed=4, max_deg=300)
+    assert np.diff(rp).max() > 65
+    n = len(rp) - 1
+    dist = R.ONE if kind == "gs" else R.TWO
+    ce, (prp, pcol, pval), zt = _prepared(rp, col, val, T, dist, oracle)
+    b = M.uniform_vec(n, 31)
+    x0 = M.uniform_vec(n, 32)
+    dx = R.DeviceArray.from_host(x0)
+    db = R.DeviceArray.from_host(b)
+    ce.setOption("layout", 2)
+    if rows:
+        ce.setOption("bundle_rows", rows)
+    k = R.K_GS if kind == "gs" else R.K_KACZ
+    okind = oracle.K_GS if kind == "gs" else oracle.K_KACZ
+    fid = ce.registerDevice(k, prp, pcol, pval, dx, db)
+    xo = x0.copy()
+    for rev in (False, True):
+        ce.executeFunction(fid, rev)
+        oracle.run_calls(okind, prp, pcol, pval, xo, b, oracle.leaf_calls(zt, rev))
+        assert relerr(dx.download(), xo) <= TOL
+
+
 # symm_mode: 0 default, 1 colour phases, 2 SELL atomics, 3 expanded full matrix
 @pytest.mark.parametrize("mode", [0, 1, 2, 3])
 @pytest.mark.parametrize("T", [1, 8, 64])
