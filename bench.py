@@ -152,7 +152,7 @@ def bench_batched(args, R, M):
     gen = ((kd, n, mean, rp, col, val) for kd, n, mean, rp, col, val in M.batch_matrices(count, args.batch_seed)
            if args.batch_kinds in ("both", kd))
     for k, (kind, n, mean, rp, col, val) in enumerate(gen):
-        T = max(4, min(1024, n // 256))
+        T = max(4, min(1024, n // args.batch_rows_per_thread))
         ce = R.Interface(n, T, R.TWO, rp, col, symm_hint=True)
         ce.RACEColor()
         perm, inv = ce.getPerm(), ce.getInvPerm()
@@ -236,7 +236,8 @@ def bench_batched(args, R, M):
         "value": round(gbs, 3), "unit": "GB/s", "n_gpus": 1, "steps": 1, "warmup": 1,
         "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"{count} matrices (seed {args.batch_seed}; {kinds}), n log-uniform in "
+        "config": {"workload": f"{count} matrices (seed {args.batch_seed}; {kinds}; RACE threads n/"
+                               f"{args.batch_rows_per_thread}), n log-uniform in "
                                f"[2000, 200000], total n={tot_n}, nnz={tot_nnz}; SymmSpMV + {sweeps} KACZ sweeps, "
                                f"distance-2 colouring, FP32 values/vectors, FP64 row norms, {nrhs} RHS",
                    "matrices": count, "rows": tot_n, "nnz": tot_nnz},
@@ -272,6 +273,8 @@ def main():
     ap.add_argument("--batch-count", type=int, default=512)
     ap.add_argument("--batch-sweeps", type=int, default=20)
     ap.add_argument("--batch-seed", type=int, default=2026)
+    ap.add_argument("--batch-rows-per-thread", type=int, default=256,
+                    help="RACE threads requested per matrix: n / this (4 .. 1024)")
     ap.add_argument("--batch-kinds", choices=["both", "banded", "chung-lu"], default="both",
                     help="keep only one matrix family of the batch")
     ap.add_argument("--no-extra", action="store_true",

@@ -99,6 +99,20 @@ struct race_handle_s {
 
 static int checkHandle(race_handle h) { return h ? OK : ERR_INVALID_ARG; }
 
+// A square CRS the device kernels can index: rowPtr[0] = 0, non-decreasing,
+// every column in [0, nrow), values present.  Checked on the host before
+// anything is laid out or uploaded.
+static int checkCrs(const race_kernel_desc* d) {
+    if (!d || d->nrow <= 0 || !d->rowPtr || !d->col || !d->val) return ERR_INVALID_ARG;
+    if (d->rowPtr[0] != 0) return ERR_INVALID_ARG;
+    for (int r = 0; r < d->nrow; ++r)
+        if (d->rowPtr[r + 1] < d->rowPtr[r]) return ERR_INVALID_ARG;
+    const int nnz = d->rowPtr[d->nrow];
+    for (int j = 0; j < nnz; ++j)
+        if (d->col[j] < 0 || d->col[j] >= d->nrow) return ERR_INVALID_ARG;
+    return OK;
+}
+
 // A batch of functions on many matrices (race_batch_*): host copies of each
 // matrix, its handle (colour schedules) and vectors, the layout options and
 // the built fused device function.
@@ -371,7 +385,7 @@ int race_register_power_function(race_handle h, race_power_fn f, void* args, int
 
 int race_register_device(race_handle h, const race_kernel_desc* d, int* id) {
     if (checkHandle(h) || !d || !id) return ERR_INVALID_ARG;
-    if (d->nrow != h->nrow) return ERR_INVALID_ARG;
+    if (d->nrow != h->nrow || checkCrs(d)) return ERR_INVALID_ARG;
     const bool power = d->kind == RACE_K_MPK;
     if (power != (h->dist == DIST_POWER) && d->kind != RACE_K_SPMV) return ERR_INVALID_ARG;
     if (d->kind != RACE_K_SPMV && !h->colored) return ERR_INVALID_ARG;
@@ -432,7 +446,7 @@ int race_register_refinement(race_handle h, const race_kernel_desc* a64, int inn
                              int* id) {
     if (checkHandle(h) || !a64 || !id || inner_id < 0 || inner_id >= (int)h->funcs.size())
         return ERR_INVALID_ARG;
-    if (!h->funcs[inner_id].dev || a64->nrow != h->nrow) return ERR_INVALID_ARG;
+    if (!h->funcs[inner_id].dev || a64->nrow != h->nrow || checkCrs(a64)) return ERR_INVALID_ARG;
     Function fn;
     int e = makeRefineFn(*a64, h->funcs[inner_id].dev.get(), inner_sweeps, &fn.dev);
     if (e) return e;
@@ -707,6 +721,7 @@ int race_batch_create(race_batch* out, int kind, int val_precision, int vec_prec
 
 int race_batch_add(race_batch b, race_handle h, const race_kernel_desc* d) {
     if (!b || checkHandle(h) || !d || !d->rowPtr || !d->col || !d->val || !d->x) return ERR_INVALID_ARG;
+    if (checkCrs(d)) return ERR_INVALID_ARG;
     if (d->kind != b->kind || d->val_precision != b->valPrec || d->vec_precision != b->vecPrec ||
         (d->nrhs > 0 ? d->nrhs : 1) != b->nrhs || d->nrow != h->nrow)
         return ERR_INVALID_ARG;

@@ -60,3 +60,31 @@ def test_bench_rejects_multi_gpu():
                        text=True, timeout=120)
     assert p.returncode != 0
     assert "one GPU" in p.stderr
+
+
+def test_entry_points_reject_bad_arguments_without_a_device():
+    """Argument checks that run before any device call (host only)."""
+    import ctypes
+
+    from race_amd import _lib as L
+    lib = L.load()
+    rp = np.array([0, 1, 2], np.int32)
+    col = np.array([0, 1], np.int32)
+    h = ctypes.c_void_p()
+    ip = lambda a: a.ctypes.data_as(L.i32p)  # noqa: E731
+    INVALID = 1
+    # race_create: empty matrix, no thread, unknown distance, half a permutation
+    assert lib.race_create(ctypes.byref(h), 0, 1, 1, ip(rp), ip(col), 1, 1, 1, None, None, 0, 1) == INVALID
+    assert lib.race_create(ctypes.byref(h), 2, 0, 1, ip(rp), ip(col), 1, 1, 1, None, None, 0, 1) == INVALID
+    assert lib.race_create(ctypes.byref(h), 2, 1, 7, ip(rp), ip(col), 1, 1, 1, None, None, 0, 1) == INVALID
+    assert lib.race_create(ctypes.byref(h), 2, 1, 1, ip(rp), ip(col), 1, 1, 1, ip(rp), None, 0, 1) == INVALID
+    # batches and the tuner's model
+    assert lib.race_batch_create(ctypes.byref(h), 99, 0, 0, 1) == INVALID
+    assert lib.race_batch_create(ctypes.byref(h), 2, 0, 0, 0) == INVALID
+    assert lib.race_mlp_create(None, 0) == INVALID
+    assert lib.race_mlp_forward(None, None, 1, None, 1) == INVALID
+    # SELL helpers: missing arrays, unknown value type
+    v = np.zeros(4)
+    assert lib.race_simdify_resolve(4, 4, 4, None, ip(rp), ip(rp), v.ctypes.data, 0, 0) == INVALID
+    assert lib.race_simdify_d1(4, 4, 4, ip(col), ip(rp), ip(rp), v.ctypes.data, 7, 0) == INVALID
+    assert lib.race_set_option(None, b"layout", 0) == INVALID
