@@ -12,8 +12,10 @@ n*(4 + 3*8) bytes for FP64), the roofline of the Gauss-Seidel kernel against
 the 8 TB/s HBM peak, the CPU oracle (the same sweeps in RACE order, single
 thread) timed on this host beside it, the time to a relative residual of
 1e-10 of FP64 iterative refinement around FP64, FP32 and BF16 inner sweeps,
-and ("other_kernels") SymmSpMV on the 7-pt 64^3 grid (hot and cold caches)
-and the level-blocked matrix powers, k = 8, on the 7-pt 256^3 grid.
+and ("other_kernels") SymmSpMV on the 7-pt 64^3 grid (hot and cold caches),
+the level-blocked matrix powers, k = 8, on the 7-pt 256^3 grid (all powers
+kept, and the only-highest variant with two vector slots), and SpMTV on the
+27-pt 128^3 grid.
 """
 import argparse
 import json
@@ -111,6 +113,18 @@ def bench_mpk(R, M, g=256, k=8, cache_mb=512.0, reps=10):
     ms = ce.timerEnd() / reps
     eff = k * (nnz * 12 + n * (4 + 2 * 8))
     floor = nnz * 12 + n * 4 + (k + 1) * n * 8
+    # only-highest variant: two vector slots instead of k + 1
+    x2 = np.zeros(2 * n)
+    x2[:n] = dx.download()[:n]
+    dx2 = R.DeviceArray.from_host(x2)
+    del x2
+    hid = ce.registerDevice(R.K_MPK_HIGHEST, prp, pcol, pval, dx2, power=k)
+    ce.executeAsync(hid)
+    ce.timerBegin()
+    for _ in range(reps):
+        ce.executeAsync(hid)
+    msh = ce.timerEnd() / reps
+    del dx2
     # the same k products as k separate SpMVs, for comparison
     dy = R.DeviceArray(n)
     sid = ce.registerDevice(R.K_SPMV, prp, pcol, pval, dx, dy)
@@ -123,10 +137,39 @@ def bench_mpk(R, M, g=256, k=8, cache_mb=512.0, reps=10):
                        f"{groups} level groups, FP64",
            "ms_per_call": round(ms, 4), "effective_gbs": round(eff / ms / 1e6, 1),
            "effective_frac_hbm": round(eff / ms / 1e6 / HBM_PEAK_GBS, 4),
-           "compulsory_gbs": round(floor / ms / 1e6, 1), "spmv_ms": round(ms1, 4),
+           "compulsory_gbs": round(floor / ms / 1e6, 1),
+           "only_highest_ms": round(msh, 4), "only_highest_effective_gbs": round(eff / msh / 1e6, 1),
+           "spmv_ms": round(ms1, 4),
            "spmv_gbs": round((nnz * 12 + n * 20) / ms1 / 1e6, 1), "speedup_vs_k_spmv": round(k * ms1 / ms, 3)}
     log(f"MPK {g}^3 k={k}: {ms:.3f} ms/call, effective {out['effective_gbs']} GB/s; "
-        f"SpMV {ms1:.3f} ms ({out['spmv_gbs']} GB/s)")
+        f"only-highest {msh:.3f} ms; SpMV {ms1:.3f} ms ({out['spmv_gbs']} GB/s)")
+    return out
+
+
+def bench_spmtv(R, M, g=128, reps=20):
+    """SpMTV b += A^T x, 27-pt g^3 (GS values, unsymmetric after a row
+    scaling), no colouring: one gather launch over the transpose."""
+    rp, col = M.stencil27(g)
+    val = M.gs_values(rp, col, seed=7)
+    n = len(rp) - 1
+    nnz = int(rp[-1])
+    val = val * np.repeat(np.linspace(0.5, 1.5, n), np.diff(rp))
+    ce = R.Interface(n, 1, R.ONE, rp, col, symm_hint=True)
+    dx = R.DeviceArray.from_host(M.uniform_vec(n, 5))
+    db = R.DeviceArray.from_host(np.zeros(n))
+    fid = ce.registerDevice(R.K_SPMTV, rp, col, val, dx, db)
+    del rp, col, val
+    for _ in range(3):
+        ce.executeAsync(fid)
+    ce.timerBegin()
+    for _ in range(reps):
+        ce.executeAsync(fid)
+    ms = ce.timerEnd() / reps
+    byts = nnz * 12 + n * (4 + 3 * 8)
+    out = {"workload": f"SpMTV 27-pt {g}^3 (n={n}, nnz={nnz}), FP64", "bytes_per_call": byts,
+           "ms_per_call": round(ms, 4), "gbs": round(byts / ms / 1e6, 1),
+           "frac_hbm": round(byts / ms / 1e6 / HBM_PEAK_GBS, 4)}
+    log(f"SpMTV 27-pt {g}^3: {ms:.3f} ms/call, {out['gbs']} GB/s")
     return out
 
 
@@ -378,7 +421,7 @@ def main():
 
     extra = None
     if not args.no_extra:
-        extra = {"symmspmv": bench_symmspmv(R, M), "mpk": bench_mpk(R, M)}
+        extra = {"symmspmv": bench_symmspmv(R, M), "mpk": bench_mpk(R, M), "spmtv": bench_spmtv(R, M)}
 
     # CPU oracle beside it: the same sweeps in RACE order, single thread
     cpu = None

@@ -93,7 +93,12 @@ enum race_kernel_kind {
     RACE_K_GS = 1,       /* Gauss-Seidel sweep (GS_KERNEL kernels.cpp:174-199) */
     RACE_K_KACZ = 2,     /* Kaczmarz sweep (KACZ_KERNEL kernels.cpp:285-316) */
     RACE_K_SPMV = 3,     /* b = A x (SPMV_KERNEL kernels.cpp:5-25) */
-    RACE_K_MPK = 4       /* x_p = A x_{p-1}, p=1..power (MAT_SPMV_KERNEL kernels.cpp:749-777) */
+    RACE_K_MPK = 4,      /* x_p = A x_{p-1}, p=1..power (MAT_SPMV_KERNEL kernels.cpp:749-777) */
+    RACE_K_SPMTV = 5,    /* b += A^T x, full matrix stored (SPMTV_KERNEL kernels.cpp:106-124);
+                            needs no colouring: the transpose is formed at registration */
+    RACE_K_MPK_HIGHEST = 6 /* MPK keeping two vector slots: power p in x + (p%2)*nrow, x holds
+                              2*nrow entries, x_0 in slot 0 (MAT_SPMV_KERNEL_only_highest
+                              kernels.cpp:706-734) */
 };
 enum race_precision { RACE_FP64 = 0, RACE_FP32 = 1, RACE_BF16 = 2 };
 
@@ -108,7 +113,8 @@ typedef struct {
     int nrhs;            /* right-hand sides, vectors are column-major v[r + k*nrow] */
     void *x;             /* device pointer */
     void *b;             /* device pointer (unused for MPK) */
-    int power;           /* MPK: number of powers; x holds (power+1)*nrow entries */
+    int power;           /* MPK: number of powers; x holds (power+1)*nrow entries
+                            (RACE_K_MPK_HIGHEST: 2*nrow) */
 } race_kernel_desc;
 
 /* Typed device registration: the GPU counterpart of registerFunction; the
@@ -159,6 +165,8 @@ int race_execute(race_handle h, int id, int rev);
  * the handle's own stream). */
 int race_execute_async(race_handle h, int id, int rev, void *stream);
 /* Interface::setPower/getPower (src/interface.cpp:574-592) */
+/* Interface::setPower (src/interface.cpp:574-577).  A device MPK runs powers
+ * 1..power; power may not exceed the power it was registered with. */
 int race_set_power(race_handle h, int id, int power);
 int race_get_power(race_handle h, int id, int *power);
 /* Interface::tuneFunction (src/interface.cpp:605-647): best power by timing */

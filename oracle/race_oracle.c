@@ -320,6 +320,38 @@ void or_spmv_range(int n, const int *rp, const int *col, const double *val,
     }
 }
 
+/* SPMTV_KERNEL_BODY examples/src/helpers/kernels.cpp:106-115: b += A^T x,
+ * scattering row by row in call order */
+void or_spmtv_range(int n, const int *rp, const int *col, const double *val,
+                    const double *x, double *b, int start, int end, int nrhs)
+{
+    for (int k = 0; k < nrhs; ++k) {
+        const double *xk = x + (size_t)k * n;
+        double *bk = b + (size_t)k * n;
+        for (int row = start; row < end; ++row) {
+            double xr = xk[row];
+            for (int idx = rp[row]; idx < rp[row + 1]; ++idx) bk[col[idx]] += val[idx] * xr;
+        }
+    }
+}
+
+/* MAT_SPMV_KERNEL_only_highest examples/src/helpers/kernels.cpp:706-734, one
+ * power at a time over all rows: x has 2*n entries, power p goes to slot
+ * p%2 and reads slot (p-1)%2 */
+void or_mpk_highest_serial(int n, const int *rp, const int *col, const double *val,
+                           double *x, int k)
+{
+    for (int p = 1; p <= k; ++p) {
+        const double *src = x + (size_t)((p - 1) % 2) * n;
+        double *dst = x + (size_t)(p % 2) * n;
+        for (int row = 0; row < n; ++row) {
+            double t = 0;
+            for (int idx = rp[row]; idx < rp[row + 1]; ++idx) t += val[idx] * src[col[idx]];
+            dst[row] = t;
+        }
+    }
+}
+
 /* x has (k+1)*n entries: x[p*n + row] = (A^p x0)[row]  (1-based powers) */
 void or_mpk_serial(int n, const int *rp, const int *col, const double *val,
                    double *x, int k)
@@ -336,7 +368,7 @@ void or_mpk_serial(int n, const int *rp, const int *col, const double *val,
 }
 
 /* Run `ncalls` leaf calls (start,end pairs) of a kernel in the given order. */
-enum { OR_K_SYMMSPMV = 0, OR_K_GS = 1, OR_K_KACZ = 2, OR_K_SPMV = 3 };
+enum { OR_K_SYMMSPMV = 0, OR_K_GS = 1, OR_K_KACZ = 2, OR_K_SPMV = 3, OR_K_SPMTV = 5 };
 
 void or_run_calls(int kind, int n, const int *rp, const int *col,
                   const double *val, double *x, double *b, int ncalls,
@@ -349,6 +381,7 @@ void or_run_calls(int kind, int n, const int *rp, const int *col,
         case OR_K_GS: or_gs_range(n, rp, col, val, x, b, s, e, nrhs); break;
         case OR_K_KACZ: or_kacz_range(n, rp, col, val, x, b, s, e, nrhs); break;
         case OR_K_SPMV: or_spmv_range(n, rp, col, val, x, b, s, e, nrhs); break;
+        case OR_K_SPMTV: or_spmtv_range(n, rp, col, val, x, b, s, e, nrhs); break;
         default: break;
         }
     }

@@ -44,6 +44,11 @@ config.h):
     scheduled leaves is checked independent.
   * mpk_partition (stage 0): every group respects the cache bound, groups
     tile the level range, and MPK through the groups equals repeated SpMV.
+  * run_calls kernels (race_oracle.c, examples/src/helpers/kernels.cpp):
+    SpMV and SpMTV against dense products A x and A^T x; SymmSpMV against
+    the full SpMV; GS/KACZ with one thread against textbook serial sweeps;
+    mpk_serial against repeated SpMV; mpk_highest_serial bit-equal to the
+    last two powers of mpk_serial.
   * sell_c_sigmize / simdify_resolve / simdify_d1: the 4x4 all-equal block
     of the reference's testSimdify (src/simdify.cpp:41-77) traced by hand
     to its XOR Latin square; every row keeps its (column, value) multiset;
@@ -62,7 +67,7 @@ import numpy as np
 ONE, TWO, POWER = 1, 2, 3
 TWO_BLOCK, THREE_BLOCK = 0, 1
 LB_ROW, LB_NNZ = 0, 1
-K_SYMMSPMV, K_GS, K_KACZ, K_SPMV = 0, 1, 2, 3
+K_SYMMSPMV, K_GS, K_KACZ, K_SPMV, K_SPMTV = 0, 1, 2, 3, 5
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.path.join(_HERE, "librace_oracle.so")
@@ -95,6 +100,7 @@ def lib():
         L.or_run_calls.argtypes = [ctypes.c_int, ctypes.c_int, _i32p, _i32p, _f64p, _f64p, _f64p,
                                    ctypes.c_int, _i32p, ctypes.c_int]
         L.or_mpk_serial.argtypes = [ctypes.c_int, _i32p, _i32p, _f64p, _f64p, ctypes.c_int]
+        L.or_mpk_highest_serial.argtypes = [ctypes.c_int, _i32p, _i32p, _f64p, _f64p, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -673,6 +679,18 @@ def mpk_serial(rowPtr, col, val, x0, k):
                         _ip(np.ascontiguousarray(col, np.int32)),
                         _dp(np.ascontiguousarray(val, np.float64)), _dp(x), k)
     return x.reshape(k + 1, n)
+
+
+def mpk_highest_serial(rowPtr, col, val, x0, k):
+    """Two-slot matrix powers (MAT_SPMV_KERNEL_only_highest); returns the
+    (2, n) slots: power p in slot p % 2."""
+    n = len(rowPtr) - 1
+    x = np.zeros(2 * n, np.float64)
+    x[:n] = x0
+    lib().or_mpk_highest_serial(n, _ip(np.ascontiguousarray(rowPtr, np.int32)),
+                                _ip(np.ascontiguousarray(col, np.int32)),
+                                _dp(np.ascontiguousarray(val, np.float64)), _dp(x), k)
+    return x.reshape(2, n)
 
 
 # --------------------------------------------------------------------------

@@ -20,7 +20,7 @@ ONE, TWO, POWER = 1, 2, 3
 TWO_BLOCK, THREE_BLOCK = 0, 1
 LB_ROW, LB_NNZ = 0, 1
 FILL, SCATTER = 0, 1
-K_SYMMSPMV, K_GS, K_KACZ, K_SPMV, K_MPK = 0, 1, 2, 3, 4
+K_SYMMSPMV, K_GS, K_KACZ, K_SPMV, K_MPK, K_SPMTV, K_MPK_HIGHEST = 0, 1, 2, 3, 4, 5, 6
 FP64, FP32, BF16 = 0, 1, 2
 
 i32p = ctypes.POINTER(ctypes.c_int)

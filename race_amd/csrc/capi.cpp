@@ -386,9 +386,15 @@ int race_register_power_function(race_handle h, race_power_fn f, void* args, int
 int race_register_device(race_handle h, const race_kernel_desc* d, int* id) {
     if (checkHandle(h) || !d || !id) return ERR_INVALID_ARG;
     if (d->nrow != h->nrow || checkCrs(d)) return ERR_INVALID_ARG;
-    const bool power = d->kind == RACE_K_MPK;
-    if (power != (h->dist == DIST_POWER) && d->kind != RACE_K_SPMV) return ERR_INVALID_ARG;
-    if (d->kind != RACE_K_SPMV && !h->colored) return ERR_INVALID_ARG;
+    const bool power = d->kind == RACE_K_MPK || d->kind == RACE_K_MPK_HIGHEST;
+    // SpMV and SpMTV need no schedule: they run on any handle
+    const bool free = d->kind == RACE_K_SPMV || d->kind == RACE_K_SPMTV;
+    if (power != (h->dist == DIST_POWER) && !free) return ERR_INVALID_ARG;
+    if (!free && !h->colored) return ERR_INVALID_ARG;
+    // power > highestPower is allowed, as in the reference (which only warns,
+    // src/interface.cpp:449-452): the wavefront is correct for any power,
+    // only the cache blocking was sized for highestPower
+    if (power && d->power < 1) return ERR_INVALID_ARG;
     ExecContext ctx;
     ctx.fwd = &h->fwd;
     ctx.rev = &h->rev;
@@ -532,10 +538,13 @@ int race_execute(race_handle h, int id, int rev) {
 int race_set_power(race_handle h, int id, int power) {
     if (checkHandle(h) || id < 0 || id >= (int)h->funcs.size() || power < 1) return ERR_INVALID_ARG;
     Function& fn = h->funcs[id];
-    if (fn.host)
+    if (fn.host) {
         fn.host->power = power;
-    else
+    } else {
+        // a device MPK holds vector slots for the registered power only
+        if (power > fn.dev->powerMax) return ERR_INVALID_ARG;
         fn.dev->power = power;
+    }
     return OK;
 }
 
@@ -552,8 +561,11 @@ int race_tune_function(race_handle h, int id, int rev, int* best_power) {
     if (checkHandle(h) || id < 0 || id >= (int)h->funcs.size() || !best_power) return ERR_INVALID_ARG;
     double bestT = 1e300;
     int best = 1;
-    for (int p = 1; p <= h->highestPower; ++p) {
-        race_set_power(h, id, p);
+    const Function& fn = h->funcs[id];
+    const int pmax = fn.dev ? std::min(h->highestPower, fn.dev->powerMax) : h->highestPower;
+    for (int p = 1; p <= pmax; ++p) {
+        int e0 = race_set_power(h, id, p);
+        if (e0) return e0;
         auto t0 = std::chrono::steady_clock::now();
         double t = 0;
         int it = 0;

@@ -429,7 +429,11 @@ int makeDeviceFn(const race_kernel_desc& d, const ExecContext& ctx, std::unique_
             *out = std::move(f);
             break;
         }
+        case RACE_K_SPMTV:
+            e = makeSpmtvFn(d, out);
+            break;
         case RACE_K_MPK:
+        case RACE_K_MPK_HIGHEST:
             e = makeMpkFn(d, ctx, out);
             break;
         default:

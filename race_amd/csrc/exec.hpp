@@ -33,6 +33,7 @@ class DeviceFn {
     void* x = nullptr;
     void* b = nullptr;
     int power = 1;
+    int powerMax = 1 << 30;  // largest power setPower accepts (MPK: the registered one)
     int kind = 0;
     int vecPrecision = 0;  // RACE_FP64 or RACE_FP32 vectors
 };
@@ -77,6 +78,8 @@ int makeWaveFn(const race_kernel_desc& d, const ExecContext& ctx, std::unique_pt
 // SELL-C SpMV (one right-hand side) and level-blocked matrix powers (sell.hip)
 int makeSellSpmvFn(const race_kernel_desc& d, std::unique_ptr<DeviceFn>* out);
 int makeMpkFn(const race_kernel_desc& d, const ExecContext& ctx, std::unique_ptr<DeviceFn>* out);
+// SpMTV b += A^T x over the transpose formed at registration (sell.hip)
+int makeSpmtvFn(const race_kernel_desc& d, std::unique_ptr<DeviceFn>* out);
 // single-launch SymmSpMV: mode 2 SELL atomics, mode 3 expanded full matrix (sell.hip)
 int makeSymmSellFn(const race_kernel_desc& d, int mode, std::unique_ptr<DeviceFn>* out);
 // streaming device copy, bytes a multiple of 16

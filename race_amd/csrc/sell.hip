@@ -172,10 +172,13 @@ struct MpkSeg {
     int c0, c1, q, blockStart;
 };
 
+// Power q is stored in vector slot q % nslot: nslot = k + 1 keeps every
+// power, nslot = 2 keeps the last two (the only-highest variant).
 __global__ void __launch_bounds__(kBlock) k_sell_mpk_step(const SellChunk* __restrict__ chunks,
                                                           const int* __restrict__ col,
                                                           const double* __restrict__ val, double* __restrict__ x,
-                                                          long n, const MpkSeg* __restrict__ segs, int nseg) {
+                                                          long n, int nslot, const MpkSeg* __restrict__ segs,
+                                                          int nseg) {
     int s = 0;
     while (s + 1 < nseg && (int)blockIdx.x >= segs[s + 1].blockStart) ++s;
     const MpkSeg sg = segs[s];
@@ -184,8 +187,8 @@ __global__ void __launch_bounds__(kBlock) k_sell_mpk_step(const SellChunk* __res
     if (c >= sg.c1) return;
     const SellChunk k = chunks[c];
     if (lane >= k.rows) return;
-    const double* __restrict__ xin = x + (long)(sg.q - 1) * n;
-    x[(long)sg.q * n + k.row0 + lane] = sellRow(k, lane, col, val, xin);
+    const double* __restrict__ xin = x + (long)((sg.q - 1) % nslot) * n;
+    x[(long)(sg.q % nslot) * n + k.row0 + lane] = sellRow(k, lane, col, val, xin);
 }
 
 // b = A x, one right-hand side
@@ -276,46 +279,110 @@ class SymmSellFn : public DeviceFn {
 // launch.  A step touches at most 2k consecutive level groups, which the
 // cache model (src/matrixPower.cpp:88-140, 685-940) sized to stay in the
 // Infinity Cache, so A is read from HBM about once per call.
+//
+// Only-highest variant (MAT_SPMV_KERNEL_only_highest
+// examples/src/helpers/kernels.cpp:706-734): two vector slots, power q in
+// slot q % 2.  Writing (g, q) overwrites (g, q - 2), whose readers (g - 1 .. g
+// + 1 at power q - 1) ran at steps up to L - 1, and (g, q) is overwritten by
+// (g, q + 2) at step L + 4 after its readers (steps up to L + 3).  Inside a
+// step, a segment reads slot (q' - 1) % 2 of groups g' - 1 .. g' + 1 while
+// another writes slot q % 2 of group g = g' + 2 (q' - q); the slots differ
+// unless q' = q, so no segment reads what another writes.
+//
+// The step tables of every power 1..k are built at registration, so
+// setPower(p <= k) runs the first p powers.
 class MpkFn : public DeviceFn {
   public:
     SellMatrix A;
-    std::vector<int> stepSeg, stepBlocks;  // segments stepSeg[L] .. stepSeg[L+1] of step L
+    int nslot = 2;
+    // for power p: segments stepSeg[p][L] .. stepSeg[p][L+1] of step L
+    std::vector<std::vector<int>> stepSeg, stepBlocks;
     DevBuf<MpkSeg> segs;
     int init(const race_kernel_desc& d, const ExecContext& ctx) {
         kind = d.kind;
         x = d.x;
         b = d.b;
         if (!ctx.mpk) return ERR_INVALID_ARG;
-        power = d.power > 0 ? d.power : 1;
+        power = powerMax = d.power > 0 ? d.power : 1;
+        nslot = d.kind == RACE_K_MPK_HIGHEST ? 2 : powerMax + 1;
         const std::vector<int>& gp = ctx.mpk->groupPtr;
         if (gp.size() < 2 || gp.front() != 0 || gp.back() != d.nrow) return ERR_INVALID_ARG;
         int e;
         if ((e = A.build(d, gp))) return e;
         const int G = (int)gp.size() - 1;
         std::vector<MpkSeg> all;
-        stepSeg.assign(1, 0);
-        stepBlocks.clear();
-        for (int L = 0; L < G + 2 * (power - 1); ++L) {
-            int blocks = 0;
-            for (int q = 1; q <= power; ++q) {
-                const int g = L - 2 * (q - 1);
-                if (g < 0 || g >= G) continue;
-                const int c0 = A.cutChunk[g], c1 = A.cutChunk[g + 1];
-                if (c1 <= c0) continue;
-                all.push_back(MpkSeg{c0, c1, q, blocks});
-                blocks += (c1 - c0 + kWavesPerBlock - 1) / kWavesPerBlock;
+        stepSeg.assign(powerMax + 1, {});
+        stepBlocks.assign(powerMax + 1, {});
+        for (int p = 1; p <= powerMax; ++p) {
+            stepSeg[p].assign(1, (int)all.size());
+            for (int L = 0; L < G + 2 * (p - 1); ++L) {
+                int blocks = 0;
+                for (int q = 1; q <= p; ++q) {
+                    const int g = L - 2 * (q - 1);
+                    if (g < 0 || g >= G) continue;
+                    const int c0 = A.cutChunk[g], c1 = A.cutChunk[g + 1];
+                    if (c1 <= c0) continue;
+                    all.push_back(MpkSeg{c0, c1, q, blocks});
+                    blocks += (c1 - c0 + kWavesPerBlock - 1) / kWavesPerBlock;
+                }
+                stepSeg[p].push_back((int)all.size());
+                stepBlocks[p].push_back(blocks);
             }
-            stepSeg.push_back((int)all.size());
-            stepBlocks.push_back(blocks);
         }
         return upload(segs, all.data(), all.size());
     }
     int run(bool, hipStream_t st) override {
-        for (int L = 0; L + 1 < (int)stepSeg.size(); ++L) {
-            if (stepBlocks[L] == 0) continue;
-            hipLaunchKernelGGL(k_sell_mpk_step, dim3(stepBlocks[L]), dim3(kBlock), 0, st, A.chunks.p, A.col.p,
-                               A.val.p, (double*)x, (long)A.n, segs.p + stepSeg[L], stepSeg[L + 1] - stepSeg[L]);
+        if (power < 1 || power > powerMax) return ERR_INVALID_ARG;
+        const std::vector<int>& sg = stepSeg[power];
+        const std::vector<int>& nb = stepBlocks[power];
+        for (int L = 0; L + 1 < (int)sg.size(); ++L) {
+            if (nb[L] == 0) continue;
+            hipLaunchKernelGGL(k_sell_mpk_step, dim3(nb[L]), dim3(kBlock), 0, st, A.chunks.p, A.col.p, A.val.p,
+                               (double*)x, (long)A.n, nslot, segs.p + sg[L], sg[L + 1] - sg[L]);
         }
+        RACE_HIP(hipGetLastError());
+        return OK;
+    }
+};
+
+// SpMTV b += A^T x for nrhs right-hand sides (SPMTV_KERNEL
+// examples/src/helpers/kernels.cpp:106-124, which scatters
+// b[c_j] += a_j x[r] row by row under a distance-2 colouring).  The
+// transpose is formed once at registration, so a call is one gather SpMV
+// over the SELL-C rows of A^T accumulating into b: no atomics, no colour
+// phases, every b entry written once.  Row c of A^T holds its entries in
+// increasing r, the order a sequential sweep adds them in.
+class SpmtvFn : public DeviceFn {
+  public:
+    SellMatrix A;
+    int nrhs = 1;
+    int init(const race_kernel_desc& d) {
+        kind = d.kind;
+        x = d.x;
+        b = d.b;
+        nrhs = d.nrhs > 0 ? d.nrhs : 1;
+        const int n = d.nrow;
+        std::vector<int> rp(n + 1, 0);
+        for (int j = 0; j < d.rowPtr[n]; ++j) ++rp[d.col[j] + 1];
+        for (int r = 0; r < n; ++r) rp[r + 1] += rp[r];
+        std::vector<int> tc(rp[n]), pos(rp.begin(), rp.end() - 1);
+        std::vector<double> tv(rp[n]);
+        for (int r = 0; r < n; ++r)
+            for (int j = d.rowPtr[r]; j < d.rowPtr[r + 1]; ++j) {
+                const int c = d.col[j];
+                tc[pos[c]] = r;
+                tv[pos[c]++] = d.val[j];
+            }
+        race_kernel_desc t = d;
+        t.rowPtr = rp.data();
+        t.col = tc.data();
+        t.val = tv.data();
+        return A.build(t, {0, n});
+    }
+    int run(bool, hipStream_t st) override {
+        const unsigned g = (unsigned)std::max(1, (A.nChunk + kWavesPerBlock - 1) / kWavesPerBlock);
+        hipLaunchKernelGGL(k_sell_spmv_acc, dim3(g), dim3(kBlock), 0, st, A.chunks.p, A.nChunk, A.col.p, A.val.p,
+                           (const double*)x, (double*)b, (long)A.n, nrhs);
         RACE_HIP(hipGetLastError());
         return OK;
     }
@@ -544,6 +611,13 @@ int makeSellSpmvFn(const race_kernel_desc& d, std::unique_ptr<DeviceFn>* out) {
 int makeSymmSellFn(const race_kernel_desc& d, int mode, std::unique_ptr<DeviceFn>* out) {
     auto f = std::make_unique<SymmSellFn>();
     int e = f->init(d, mode);
+    *out = std::move(f);
+    return e;
+}
+
+int makeSpmtvFn(const race_kernel_desc& d, std::unique_ptr<DeviceFn>* out) {
+    auto f = std::make_unique<SpmtvFn>();
+    int e = f->init(d);
     *out = std::move(f);
     return e;
 }

@@ -14,11 +14,11 @@ import ctypes
 import numpy as np
 
 from . import _lib as L
-from ._lib import (BF16, FP32, FP64, K_GS, K_KACZ, K_MPK, K_SPMV, K_SYMMSPMV, LB_NNZ, LB_ROW, ONE,
+from ._lib import (BF16, FP32, FP64, K_GS, K_KACZ, K_MPK, K_MPK_HIGHEST, K_SPMTV, K_SPMV, K_SYMMSPMV, LB_NNZ, LB_ROW, ONE,
                    POWER, SCATTER, TWO, TWO_BLOCK, RaceError)
 
 __all__ = ["Interface", "DeviceArray", "lb_split", "RaceError", "ONE", "TWO", "POWER", "K_SYMMSPMV",
-           "K_GS", "K_KACZ", "K_SPMV", "K_MPK", "FP64", "FP32", "BF16", "LB_NNZ", "LB_ROW",
+           "K_GS", "K_KACZ", "K_SPMV", "K_MPK", "K_SPMTV", "K_MPK_HIGHEST", "FP64", "FP32", "BF16", "LB_NNZ", "LB_ROW",
            "TWO_BLOCK", "device_count"]
 
 

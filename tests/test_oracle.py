@@ -272,6 +272,35 @@ def test_mpk_equals_repeated_spmv(oracle):
         assert relerr(X[p], y) < 1e-14
 
 
+def test_spmtv_equals_transpose_product(oracle):
+    # unsymmetric values and pattern, so A x and A^T x differ
+    rng = np.random.default_rng(8)
+    n = 300
+    A = (sp.random(n, n, density=0.03, random_state=rng, format="csr") + sp.eye(n)).tocsr()
+    A.sort_indices()
+    rp, col, val = A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data.copy()
+    x = rng.uniform(-1, 1, 2 * n)
+    b0 = rng.uniform(-1, 1, 2 * n)
+    assert relerr(A @ x[:n], A.T @ x[:n]) > 1e-2
+    b = b0.copy()
+    # leaves in any order: every b entry sums the same terms
+    oracle.run_calls(oracle.K_SPMTV, rp, col, val, x, b, [(150, n), (0, 150)], nrhs=2)
+    for k in range(2):
+        ref = b0[k * n:(k + 1) * n] + A.T @ x[k * n:(k + 1) * n]
+        assert relerr(b[k * n:(k + 1) * n], ref) < 1e-14
+
+
+@pytest.mark.parametrize("k", [1, 2, 5])
+def test_mpk_highest_is_last_two_powers(oracle, k):
+    rp, col = M.stencil7(5)
+    val = M.jittered_values(rp, col, 0.1, seed=4)
+    x0 = M.uniform_vec(len(rp) - 1, 9)
+    X = oracle.mpk_serial(rp, col, val, x0, k)
+    H = oracle.mpk_highest_serial(rp, col, val, x0, k)
+    assert np.array_equal(H[k % 2], X[k])
+    assert np.array_equal(H[(k - 1) % 2], X[k - 1])
+
+
 def test_multi_rhs_columns_are_independent(oracle):
     rp, col, val = _gs_matrix(4, 1)
     n = len(rp) - 1
