@@ -183,6 +183,24 @@ int race_get_schedule(race_handle h, int rev, int *nphases, int *nleaves, int *p
 int race_get_tree(race_handle h, int *nleaves, int *data6, int cap);
 /* MPK level groups: group_ptr[ngroups+1] (row offsets) */
 int race_get_mpk_groups(race_handle h, int *ngroups, int *group_ptr, int cap);
+
+/* ---- schedule save / load ----
+ * No counterpart in the reference, which rebuilds its zone tree and MPK tree
+ * on every RACEColor call and can only print them (ZoneTree::printTree
+ * src/zone_tree.cpp:307-339, mtxPowerRecursive::printTree
+ * src/matrixPowerRecursive.cpp:447-490).  The byte string holds what
+ * RACEColor computed: permutation, zone tree (colouring) or level groups
+ * (POWER), thread count, and a hash of the matrix pattern it was built on.
+ * race_schedule_save: buf NULL returns the size only; needs a coloured handle.
+ * race_schedule_load: instead of RACEColor, on a handle created with the
+ * same matrix pattern, nrow, nthreads, dist, d2 method and LB target, and
+ * before any function is registered; anything else, a damaged byte string
+ * (checksum) or one whose contents are out of range, gives
+ * RACE_ERR_INVALID_ARG.
+ * race_schedule_info: checks a byte string without a handle. */
+int race_schedule_save(race_handle h, void *buf, size_t cap, size_t *size);
+int race_schedule_load(race_handle h, const void *buf, size_t size);
+int race_schedule_info(const void *buf, size_t size, int *nrow, int *dist, int *nthreads);
 /* dependency waves of a registered GS/KACZ function (critical path length) */
 int race_get_wave_count(race_handle h, int id, int rev, int *waves);
 /* HIP-event timer on the handle's stream (brackets work launched there) */

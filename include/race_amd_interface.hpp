@@ -54,6 +54,20 @@ class Interface {
     }
     RACE_error RACEColor() { return h_ ? (RACE_error)race_color(h_) : RACE_ERR_INVALID_ARG; }
 
+    // schedule save / load (no counterpart in the reference): what RACEColor
+    // computed, as bytes; loadSchedule replaces RACEColor on a handle built
+    // with the same matrix pattern and constructor arguments
+    std::vector<unsigned char> saveSchedule() {
+        size_t n = 0;
+        if (race_schedule_save(h_, nullptr, 0, &n) != RACE_SUCCESS) return {};
+        std::vector<unsigned char> b(n);
+        if (race_schedule_save(h_, b.data(), b.size(), &n) != RACE_SUCCESS) return {};
+        return b;
+    }
+    RACE_error loadSchedule(const std::vector<unsigned char>& b) {
+        return h_ ? (RACE_error)race_schedule_load(h_, b.data(), b.size()) : RACE_ERR_INVALID_ARG;
+    }
+
     double getEfficiency() {
         double e = 0;
         race_get_efficiency(h_, &e);

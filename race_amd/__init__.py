@@ -5,10 +5,10 @@ gfx950 under race_amd/csrc).  This package holds the Python mirror of
 RACE::Interface over that ABI, the synthetic matrix generators and the build
 script.
 """
-from .interface import (BF16, FP32, FP64, K_GS, K_KACZ, K_MPK, K_MPK_HIGHEST, K_SPMTV, K_SPMV, K_SYMMSPMV, LB_NNZ, LB_ROW, ONE,
-                        POWER, TWO, TWO_BLOCK, Batch, DeviceArray, Interface, RaceError, device_count, lb_split,
-                        simdify_resolve, simdifyD1)
+from .interface import (BF16, FP32, FP64, K_GS, K_KACZ, K_MPK, K_MPK_HIGHEST, K_SPMTV, K_SPMV, K_SYMMSPMV,
+                        LB_NNZ, LB_ROW, ONE, POWER, TWO, TWO_BLOCK, Batch, DeviceArray, Interface, RaceError,
+                        device_count, lb_split, schedule_info, simdify_resolve, simdifyD1)
 
-__all__ = ["Interface", "Batch", "DeviceArray", "RaceError", "lb_split", "device_count", "simdify_resolve", "simdifyD1", "ONE", "TWO", "POWER",
-           "K_SYMMSPMV", "K_GS", "K_KACZ", "K_SPMV", "K_MPK", "K_SPMTV", "K_MPK_HIGHEST", "FP64", "FP32", "BF16", "LB_NNZ", "LB_ROW",
-           "TWO_BLOCK"]
+__all__ = ["Interface", "Batch", "DeviceArray", "RaceError", "lb_split", "device_count", "simdify_resolve",
+           "simdifyD1", "schedule_info", "ONE", "TWO", "POWER", "K_SYMMSPMV", "K_GS", "K_KACZ", "K_SPMV", "K_MPK",
+           "K_SPMTV", "K_MPK_HIGHEST", "FP64", "FP32", "BF16", "LB_NNZ", "LB_ROW", "TWO_BLOCK"]

@@ -82,6 +82,9 @@ PROTOTYPES = {
     "race_get_schedule": (ctypes.c_int, [vp, ctypes.c_int, i32p, i32p, i32p, i32p, i32p, ctypes.c_int]),
     "race_get_tree": (ctypes.c_int, [vp, i32p, i32p, ctypes.c_int]),
     "race_get_mpk_groups": (ctypes.c_int, [vp, i32p, i32p, ctypes.c_int]),
+    "race_schedule_save": (ctypes.c_int, [vp, vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
+    "race_schedule_load": (ctypes.c_int, [vp, vp, ctypes.c_size_t]),
+    "race_schedule_info": (ctypes.c_int, [vp, ctypes.c_size_t, i32p, i32p, i32p]),
     "race_get_wave_count": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, i32p]),
     "race_get_preprocess_stats": (ctypes.c_int, [vp, f64p, i32p]),
     "race_timer_begin": (ctypes.c_int, [vp]),

@@ -18,6 +18,7 @@
 #include "simdify.hpp"
 #include "engine.hpp"
 #include "exec.hpp"
+#include "schedule_io.hpp"
 
 using namespace race;
 
@@ -633,6 +634,75 @@ int race_get_tree(race_handle h, int* nleaves, int* data6, int cap) {
             data6[6 * i + 4] = T[i].stage;
             data6[6 * i + 5] = T[i].effRowZ;
         }
+    return OK;
+}
+
+int race_schedule_save(race_handle h, void* buf, size_t cap, size_t* size) {
+    if (checkHandle(h) || !size || !h->colored) return ERR_INVALID_ARG;
+    SavedSchedule s;
+    s.nrow = h->nrow;
+    s.dist = h->dist;
+    s.d2 = h->d2;
+    s.lbTarget = h->lbTarget;
+    s.requested = h->requested;
+    s.available = h->availableThreads;
+    s.highestPower = h->highestPower;
+    s.highestSubPower = h->highestSubPower;
+    s.pattern = patternHash(h->nrow, h->rowPtr.data(), h->col.data());
+    s.perm = h->perm;
+    s.inv = h->inv;
+    if (h->dist == DIST_POWER) {
+        s.hasMpk = true;
+        s.mpk = h->mpk;
+    } else {
+        if (!h->tree) return ERR_INTERNAL;
+        s.hasTree = true;
+        s.tree = h->tree->tree;
+    }
+    const std::vector<unsigned char> b = encodeSchedule(s);
+    *size = b.size();
+    if (!buf) return OK;
+    if (cap < b.size()) return ERR_INVALID_ARG;
+    std::memcpy(buf, b.data(), b.size());
+    return OK;
+}
+
+int race_schedule_load(race_handle h, const void* buf, size_t size) {
+    if (checkHandle(h) || !buf) return ERR_INVALID_ARG;
+    // registered functions were laid out for the schedule they saw
+    if (!h->funcs.empty()) return ERR_INVALID_ARG;
+    SavedSchedule s;
+    int e = decodeSchedule(buf, size, &s);
+    if (e) return e;
+    if (s.nrow != h->nrow || s.dist != h->dist || s.d2 != h->d2 || s.lbTarget != h->lbTarget ||
+        s.requested != h->requested)
+        return ERR_INVALID_ARG;
+    if (s.pattern != patternHash(h->nrow, h->rowPtr.data(), h->col.data())) return ERR_INVALID_ARG;
+    if (s.hasTree) {
+        h->tree = std::make_unique<ZoneTree>(h->dist, h->d2, h->lbTarget);
+        h->tree->tree = std::move(s.tree);
+        h->fwd = flattenSchedule(*h->tree, false, h->nrow);
+        h->rev = flattenSchedule(*h->tree, true, h->nrow);
+    } else {
+        h->tree.reset();
+        h->mpk = std::move(s.mpk);
+        h->highestPower = s.highestPower;
+        h->highestSubPower = s.highestSubPower;
+    }
+    h->perm = std::move(s.perm);
+    h->inv = std::move(s.inv);
+    h->availableThreads = s.available;
+    h->colored = true;
+    return OK;
+}
+
+int race_schedule_info(const void* buf, size_t size, int* nrow, int* dist, int* nthreads) {
+    SavedSchedule s;
+    int e = decodeSchedule(buf, size, &s);
+    if (e) return e;
+    if (nrow) *nrow = s.nrow;
+    if (dist) *dist = s.dist;
+    if (nthreads) *nthreads = s.available;
     return OK;
 }
 

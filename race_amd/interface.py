@@ -14,12 +14,12 @@ import ctypes
 import numpy as np
 
 from . import _lib as L
-from ._lib import (BF16, FP32, FP64, K_GS, K_KACZ, K_MPK, K_MPK_HIGHEST, K_SPMTV, K_SPMV, K_SYMMSPMV, LB_NNZ, LB_ROW, ONE,
-                   POWER, SCATTER, TWO, TWO_BLOCK, RaceError)
+from ._lib import (BF16, FP32, FP64, K_GS, K_KACZ, K_MPK, K_MPK_HIGHEST, K_SPMTV, K_SPMV, K_SYMMSPMV, LB_NNZ,
+                   LB_ROW, ONE, POWER, SCATTER, TWO, TWO_BLOCK, RaceError)
 
 __all__ = ["Interface", "DeviceArray", "lb_split", "RaceError", "ONE", "TWO", "POWER", "K_SYMMSPMV",
            "K_GS", "K_KACZ", "K_SPMV", "K_MPK", "K_SPMTV", "K_MPK_HIGHEST", "FP64", "FP32", "BF16", "LB_NNZ", "LB_ROW",
-           "TWO_BLOCK", "device_count"]
+           "TWO_BLOCK", "device_count", "schedule_info"]
 
 
 def _i32(a):
@@ -264,6 +264,20 @@ class Interface:
         L.check(self._L.race_get_mpk_groups(self.h, ctypes.byref(n), _ip(g), n.value + 1), "mpkGroups")
         return g
 
+    def saveSchedule(self):
+        """What RACEColor computed, as bytes (race_schedule_save)."""
+        n = ctypes.c_size_t()
+        L.check(self._L.race_schedule_save(self.h, None, 0, ctypes.byref(n)), "saveSchedule")
+        buf = ctypes.create_string_buffer(n.value)
+        L.check(self._L.race_schedule_save(self.h, buf, n.value, ctypes.byref(n)), "saveSchedule")
+        return buf.raw[:n.value]
+
+    def loadSchedule(self, data):
+        """Instead of RACEColor: a schedule saved from a handle with the same
+        matrix pattern and constructor arguments (race_schedule_load)."""
+        data = bytes(data)
+        L.check(self._L.race_schedule_load(self.h, data, len(data)), "loadSchedule")
+
     def waveCount(self, fid, rev=False):
         v = ctypes.c_int()
         L.check(self._L.race_get_wave_count(self.h, fid, int(rev), ctypes.byref(v)), "waveCount")
@@ -387,6 +401,15 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def schedule_info(data):
+    """Check a saved schedule without a handle: dict(nrow, dist, nthreads)."""
+    data = bytes(data)
+    n, d, t = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    L.check(L.load().race_schedule_info(data, len(data), ctypes.byref(n), ctypes.byref(d), ctypes.byref(t)),
+            "schedule_info")
+    return {"nrow": n.value, "dist": d.value, "nthreads": t.value}
 
 
 def _sell_args(col, chunkStart, clp, val):

@@ -88,3 +88,68 @@ def test_entry_points_reject_bad_arguments_without_a_device():
     assert lib.race_simdify_resolve(4, 4, 4, None, ip(rp), ip(rp), v.ctypes.data, 0, 0) == INVALID
     assert lib.race_simdify_d1(4, 4, 4, ip(col), ip(rp), ip(rp), v.ctypes.data, 7, 0) == INVALID
     assert lib.race_set_option(None, b"layout", 0) == INVALID
+
+
+def _fnv(data, h=1469598103934665603):
+    for c in data:
+        h = ((h ^ c) * 1099511628211) & 0xFFFFFFFFFFFFFFFF
+    return h
+
+
+def _schedule_bytes(n=6, perm=None, leaves=None, dist=1, checksum=True, mpk=None):
+    """A schedule byte string written field by field: header, permutation,
+    zone tree (root with two threads of two colours each), checksum."""
+    import struct
+    perm = list(range(n))[::-1] if perm is None else perm
+    inv = [0] * n
+    for i, p in enumerate(perm):
+        if 0 <= p < n:
+            inv[p] = i
+    if leaves is None:
+        # root [0, n) split into 4 blocks: thread 0 = leaves 1, 2; thread 1 = 3, 4
+        q = [0, 2, 3, 5, n]
+        leaves = [(2, 2, 1, -1, n, 0, [0, n], [1, 5])]
+        leaves += [(1, 1, 1, 0, q[i + 1] - q[i], 1, [q[i], q[i + 1]], []) for i in range(4)]
+    vec = lambda v: struct.pack("<i", len(v)) + struct.pack(f"<{len(v)}i", *v)  # noqa: E731
+    b = b"RACESCHD" + struct.pack("<I", 1) + struct.pack("<8i", n, dist, 0, 1, 2, 2, 1, 1)
+    b += struct.pack("<Q", 12345) + vec(perm) + vec(inv)
+    if mpk is None:
+        b += struct.pack("<ii", 1, len(leaves))
+        for lf in leaves:
+            b += struct.pack("<7i", *lf[:6], 0) + vec(lf[6]) + vec(lf[7])
+        b += struct.pack("<i", 0)
+    else:
+        # level groups: levelPtr, groupPtr, groupLevel, levelNnz, hopeless
+        b += struct.pack("<ii", 0, 1) + b"".join(vec(v) for v in mpk) + struct.pack("<ddi", 1e6, 20.0, 2)
+    return b + struct.pack("<Q", _fnv(b) if checksum else 0)
+
+
+def test_schedule_bytes_are_checked_without_a_device():
+    """race_schedule_info parses and range-checks a saved schedule on the host."""
+    ok = _schedule_bytes()
+    assert R.schedule_info(ok) == {"nrow": 6, "dist": 1, "nthreads": 2}
+    bad = [
+        _schedule_bytes(checksum=False),                       # damaged
+        ok[:-9] + ok[-8:],                                     # truncated
+        b"RACESCHX" + ok[8:],                                  # magic (checksum fails too)
+        _schedule_bytes(perm=[0, 0, 2, 3, 4, 5]),              # not a permutation
+        _schedule_bytes(leaves=[(2, 2, 1, -1, 6, 0, [0, 6], [0, 4])]),  # child before parent
+        _schedule_bytes(leaves=[(2, 2, 1, -1, 6, 0, [0, 6], [1, 9])]),  # child out of range
+        _schedule_bytes(leaves=[(1, 1, 1, -1, 6, 0, [0, 7], [])]),      # rows beyond nrow
+        _schedule_bytes(leaves=[(2, 2, 1, -1, 6, 0, [0, 6], [1, 5]),    # rows 5 not covered
+                                (1, 1, 1, 0, 2, 1, [0, 2], []), (1, 1, 1, 0, 1, 1, [2, 3], []),
+                                (1, 1, 1, 0, 2, 1, [3, 5], []), (1, 1, 1, 0, 0, 1, [5, 5], [])]),
+        _schedule_bytes(leaves=[(2, 2, 1, -1, 6, 0, [0, 6], [1, 5]),    # row 1 twice
+                                (1, 1, 1, 0, 2, 1, [0, 2], []), (1, 1, 1, 0, 1, 1, [1, 3], []),
+                                (1, 1, 1, 0, 2, 1, [3, 5], []), (1, 1, 1, 0, 1, 1, [5, 6], [])]),
+        _schedule_bytes(dist=3),                               # POWER without level groups
+        _schedule_bytes(dist=3, mpk=[[0, 1, 3, 6], [0, 3, 6], [0, 1, 3], [4, 8, 12], []]),  # group 1 at row 3
+        _schedule_bytes(dist=3, mpk=[[0, 1, 3, 6], [0, 1, 6], [0, 1, 3], [4, 8, 12], [2]]),  # hopeless level
+        _schedule_bytes(dist=1, mpk=[[0, 1, 3, 6], [0, 1, 6], [0, 1, 3], [4, 8, 12], []]),   # groups, not POWER
+    ]
+    power = _schedule_bytes(dist=3, mpk=[[0, 1, 3, 6], [0, 1, 6], [0, 1, 3], [4, 8, 12], []])
+    assert R.schedule_info(power) == {"nrow": 6, "dist": 3, "nthreads": 2}
+    for i, b in enumerate(bad):
+        with pytest.raises(R.RaceError):
+            R.schedule_info(b)
+            pytest.fail(f"case {i} accepted")
