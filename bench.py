@@ -305,7 +305,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--grid", type=int, default=128)
-    ap.add_argument("--threads", type=int, default=int(os.environ.get("RACE_BENCH_THREADS", "16384")))
+    ap.add_argument("--threads", type=int, default=int(os.environ.get("RACE_BENCH_THREADS", "65536")))
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
@@ -322,6 +322,8 @@ def main():
                     help="keep only one matrix family of the batch")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the SymmSpMV 64^3 and MPK 256^3 measurements")
+    ap.add_argument("--graph", type=int, choices=[0, 1], default=0,
+                    help="replay each sweep's phase launches as a HIP graph")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="run a plain copy kernel of this many MiB first (byte-counter reference)")
     args = ap.parse_args()
@@ -350,6 +352,7 @@ def main():
     ce = R.Interface(n, args.threads, R.ONE, rp, col, symm_hint=True)
     ce.RACEColor()
     t_color = time.time() - t0
+    ce.setOption("graph", args.graph)
     bfs_ms, ntrav = ce.preprocessStats()
     perm, inv = ce.getPerm(), ce.getInvPerm()
     log(f"RACEColor T={args.threads}: {t_color:.2f}s (GPU BFS {bfs_ms:.0f} ms in {ntrav} traversals), "
@@ -387,7 +390,7 @@ def main():
     ms_step = ms / args.steps
     bytes_step = 2 * gs_bytes(n, nnz)
     gbs = bytes_step / (ms_step * 1e-3) / 1e9
-    launches_per_step = 2 * (len(pp) - 1)
+    launches_per_step = ce.launchCount(fid) + ce.launchCount(fid, True)
     log(f"GPU: {ms_step:.3f} ms/step, {gbs:.1f} GB/s, {launches_per_step} launches/step")
 
     # mixed precision vs FP64 to the same residual: FP64 iterative refinement

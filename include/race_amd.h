@@ -135,6 +135,10 @@ int race_register_device(race_handle h, const race_kernel_desc *desc, int *id);
  *   "symm_mode"    SymmSpMV: 1 = colour phases (CRS, one thread per row),
  *                  2 = one launch over a SELL-C upper triangle, FP64 atomics,
  *                  3 = one launch over the expanded full matrix (default, 0)
+ *   "graph"        1 = executeFunction replays the function's launch sequence
+ *                  (one kernel per colour phase or wavefront step) as a HIP
+ *                  graph, captured once per sweep direction, vectors and
+ *                  power (default 0)
  *   "bfs_block_max" ranges of at most this many rows are traversed one
  *                  workgroup each, batched (default 131072; 0 = every range
  *                  level by level over the whole device); set before colouring
@@ -203,6 +207,9 @@ int race_schedule_load(race_handle h, const void *buf, size_t size);
 int race_schedule_info(const void *buf, size_t size, int *nrow, int *dist, int *nthreads);
 /* dependency waves of a registered GS/KACZ function (critical path length) */
 int race_get_wave_count(race_handle h, int id, int rev, int *waves);
+/* kernel launches one execution of a registered device function issues
+ * (colour phases, MPK wavefront steps) */
+int race_get_launch_count(race_handle h, int id, int rev, int *launches);
 /* HIP-event timer on the handle's stream (brackets work launched there) */
 int race_timer_begin(race_handle h);
 int race_timer_end(race_handle h, float *ms);

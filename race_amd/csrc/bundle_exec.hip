@@ -77,24 +77,29 @@ struct BundleVec {
     }
 };
 
+// The prologue has three dependent memory levels: the bundle's offsets;
+// then, all issued together, its entry block, row ids, inverse diagonals,
+// wave numbers and outside-column ids; then the gathers of x and b.  A
+// phase whose chains are single bundles (bundle0 >= 0) skips the chain
+// table.
 template <int BS, int TPR, int E, typename VT, typename XT, bool ONE_RHS, bool BATCH>
 __global__ void __launch_bounds__(BS) k_gs_bundle(
     const int* __restrict__ bSlot, const int* __restrict__ bWave, const long* __restrict__ bOff,
-    const int* __restrict__ bLen, const int* __restrict__ bExt, const int* __restrict__ waveSlot,
-    const int* __restrict__ rowId, const XT* __restrict__ diag, const int* __restrict__ ext,
+    const int* __restrict__ bLen, const int* __restrict__ bExt, const uint16_t* __restrict__ slotWave,
+    const int* __restrict__ rowId, const XT* __restrict__ rdiag, const int* __restrict__ ext,
     const uint16_t* __restrict__ sidx, const VT* __restrict__ sval, XT* __restrict__ x0,
-    const XT* __restrict__ b0, int n0, int nrhs, const int* __restrict__ chainPtr, int chain0, int nchain, int xcd,
-    int ldsLocal,
-    XT* const* __restrict__ bX, const XT* const* __restrict__ bB, const int* __restrict__ bN) {
+    const XT* __restrict__ b0, int n0, int nrhs, const int* __restrict__ chainPtr, int chain0, int nchain,
+    int bundle0, int xcd, XT* const* __restrict__ bX, const XT* const* __restrict__ bB,
+    const int* __restrict__ bN) {
     extern __shared__ __align__(16) unsigned char smem[];
     XT* xl = reinterpret_cast<XT*>(smem);
-    int* ws = reinterpret_cast<int*>(xl + ldsLocal);
 
     const int q = bundleOfBlock(nchain, xcd);
     if (q < 0) return;
     // a chain: one bundle, or the segments of a leaf too large for one bundle,
     // run in order (each segment reads what the previous one wrote)
-    const int cLo = chainPtr[chain0 + q], cHi = chainPtr[chain0 + q + 1];
+    const int cLo = bundle0 >= 0 ? bundle0 + q : chainPtr[chain0 + q];
+    const int cHi = bundle0 >= 0 ? cLo + 1 : chainPtr[chain0 + q + 1];
     auto runBundle = [&](const int bnd) {
         const BundleVec<BATCH, XT> vec(x0, b0, n0, bX, bB, bN, bnd);
         XT* __restrict__ x = vec.x;
@@ -103,7 +108,7 @@ __global__ void __launch_bounds__(BS) k_gs_bundle(
         const int s0 = bSlot[bnd], nb = bSlot[bnd + 1] - s0;
         const long off = bOff[bnd];
         const int wb = bLen[bnd];
-        const int w0 = bWave[bnd], nw = bWave[bnd + 1] - w0;
+        const int nw = bWave[bnd + 1] - bWave[bnd];
         const int e0 = bExt[bnd], ne = bExt[bnd + 1] - e0;
         const int t = threadIdx.x, i = t / TPR, l = t % TPR;
         const bool active = i < nb;
@@ -119,30 +124,34 @@ __global__ void __launch_bounds__(BS) k_gs_bundle(
             lv[u] = in ? loadVal<XT>(sval + e) : XT(0);
         }
         const int row = active ? rowId[s0 + i] : 0;
-        const XT dg = active ? diag[s0 + i] : XT(1);
-        for (int w = t; w <= nw; w += BS) ws[w] = waveSlot[w0 + w] - s0;
-        __syncthreads();
-        int myWave = -1;
-        if (active)
-            for (int w = 0; w < nw; ++w)
-                if (i >= ws[w] && i < ws[w + 1]) myWave = w;
+        const XT rd = active ? rdiag[s0 + i] : XT(1);
+        const int myWave = active ? (int)slotWave[s0 + i] : -1;
+        const int ex0 = t < ne ? ext[e0 + t] : 0;
+        const int ex1 = t + BS < ne ? ext[e0 + t + BS] : 0;
 
         // right-hand sides are independent: grid.y splits them over workgroups
         const int k0 = ONE_RHS ? 0 : (int)blockIdx.y;
         const int k1 = ONE_RHS ? 1 : (gridDim.y > 1 ? k0 + 1 : nrhs);
         for (int k = k0; k < k1; ++k) {
             const size_t vo = (size_t)k * n;
-            for (int c = t; c < ne; c += BS) xl[nb + c] = x[vo + ext[e0 + c]];
+            if (t < ne) xl[nb + t] = x[vo + ex0];
+            if (t + BS < ne) xl[nb + t + BS] = x[vo + ex1];
+            for (int c = t + 2 * BS; c < ne; c += BS) xl[nb + c] = x[vo + ext[e0 + c]];
             if (active && l == 0) xl[i] = x[vo + row];
             const XT br = (active && l == 0) ? b[vo + row] : XT(0);
             __syncthreads();
             for (int w = 0; w < nw; ++w) {
                 if (myWave == w) {
-                    XT s = XT(0);
+                    // two partial sums: half the dependent FMA chain
+                    XT sa = XT(0), sb = XT(0);
 #pragma unroll
-                    for (int u = 0; u < E; ++u) s += lv[u] * xl[li[u]];
-                    s = laneSum<TPR>(s);
-                    if (l == 0) xl[i] = (br - s) / dg;
+                    for (int u = 0; u + 1 < E; u += 2) {
+                        sa += lv[u] * xl[li[u]];
+                        sb += lv[u + 1] * xl[li[u + 1]];
+                    }
+                    if (E % 2) sa += lv[E - 1] * xl[li[E - 1]];
+                    const XT sm = laneSum<TPR>(sa + sb);
+                    if (l == 0) xl[i] = (br - sm) * rd;
                 }
                 __syncthreads();
             }
@@ -300,7 +309,7 @@ __global__ void __launch_bounds__(BS) k_gs_bundle_vl(
             lv[u] = in ? loadVal<XT>(sval + e) : XT(0);
         }
         const int row = active ? rowId[s0 + i] : 0;
-        const XT dg = active ? diag[s0 + i] : XT(1);
+        const XT dg = active ? diag[s0 + i] : XT(1);  // 1 / a_rr
         for (int w = t; w <= nw; w += BS) ws[w] = waveSlot[w0 + w] - s0;
         if (first) rl0[i] = t;
         if (t == 0) rl0[nb] = nl;
@@ -330,7 +339,7 @@ __global__ void __launch_bounds__(BS) k_gs_bundle_vl(
                 if (myWave == w && first) {
                     XT s = XT(0);
                     for (int l = rowL0; l < rowL1; ++l) s += part[l];
-                    xl[i] = (br - s) / dg;
+                    xl[i] = (br - s) * dg;
                 }
                 __syncthreads();
             }
@@ -467,7 +476,7 @@ template <typename VT, typename XT>
 struct BundleDev {
     BundleTable h;
     DevBuf<int> slot, wave, len, ext, extPtr, waveSlot, rowId, chainPtr, lanePtr;
-    DevBuf<uint16_t> laneSlot;
+    DevBuf<uint16_t> laneSlot, slotWave;
     DevBuf<long> off;
     DevBuf<uint16_t> sidx;
     DevBuf<VT> sval;
@@ -496,8 +505,17 @@ struct BundleDev {
         if (h.kind == RACE_K_KACZ) {
             if ((e = upload(rowNorm, h.rowNorm.data(), h.rowNorm.size()))) return e;
         } else {
-            std::vector<XT> dg(h.diag.begin(), h.diag.end());
+            // Gauss-Seidel multiplies by the inverse diagonal
+            std::vector<XT> dg(h.diag.size());
+            for (size_t q = 0; q < dg.size(); ++q) dg[q] = (XT)(1.0 / h.diag[q]);
             if ((e = upload(diag, dg.data(), dg.size()))) return e;
+            // slot -> its wave inside the bundle (a bundle holds at most 8192 rows)
+            std::vector<uint16_t> sw(h.rowId.size(), 0);
+            for (size_t bd = 0; bd + 1 < h.wave.size(); ++bd)
+                for (int w = h.wave[bd]; w < h.wave[bd + 1]; ++w)
+                    for (int sl = h.waveSlot[w]; sl < h.waveSlot[w + 1]; ++sl)
+                        sw[sl] = (uint16_t)(w - h.wave[bd]);
+            if ((e = upload(slotWave, sw.data(), sw.size()))) return e;
         }
         // the host copies of the big arrays are no longer needed
         std::vector<uint16_t>().swap(h.sidx);
@@ -524,6 +542,9 @@ template <int BS, int TPR, int EE, typename VT, typename XT, bool BATCH>
 void launchBundles(int kind, int nrhs, const BundleDev<VT, XT>& d, int c0, int nchain, int p, int xcd, XT* x,
                    const XT* b, int n, XT* const* bX, const XT* const* bB, const int* bN, hipStream_t st) {
     const dim3 grid(xcd ? 8u * (unsigned)((nchain + 7) / 8) : (unsigned)nchain, (unsigned)std::max(1, nrhs));
+    // chains of one bundle each: the kernel indexes bundles directly
+    const std::vector<int>& cp = d.h.chainPtr;
+    const int bundle0 = cp[c0 + nchain] - cp[c0] == nchain ? cp[c0] : -1;
     if (kind == RACE_K_KACZ)
         hipLaunchKernelGGL((k_kacz_bundle<BS, TPR, EE, VT, XT, BATCH>), grid, dim3(BS), d.lds(p, BS), st,
                            d.slot.p, d.wave.p, d.off.p, d.len.p, d.extPtr.p, d.waveSlot.p, d.rowId.p,
@@ -531,14 +552,14 @@ void launchBundles(int kind, int nrhs, const BundleDev<VT, XT>& d, int c0, int n
                            d.h.phLocal[p], bX, bB, bN);
     else if (nrhs == 1)
         hipLaunchKernelGGL((k_gs_bundle<BS, TPR, EE, VT, XT, true, BATCH>), dim3(grid.x), dim3(BS), d.lds(p, BS), st,
-                           d.slot.p, d.wave.p, d.off.p, d.len.p, d.extPtr.p, d.waveSlot.p, d.rowId.p, d.diag.p,
-                           d.ext.p, d.sidx.p, d.sval.p, x, b, n, nrhs, d.chainPtr.p, c0, nchain, xcd, d.h.phLocal[p], bX, bB,
-                           bN);
+                           d.slot.p, d.wave.p, d.off.p, d.len.p, d.extPtr.p, d.slotWave.p, d.rowId.p, d.diag.p,
+                           d.ext.p, d.sidx.p, d.sval.p, x, b, n, nrhs, d.chainPtr.p, c0, nchain, bundle0, xcd, bX,
+                           bB, bN);
     else
         hipLaunchKernelGGL((k_gs_bundle<BS, TPR, EE, VT, XT, false, BATCH>), grid, dim3(BS), d.lds(p, BS),
-                           st, d.slot.p, d.wave.p, d.off.p, d.len.p, d.extPtr.p, d.waveSlot.p, d.rowId.p,
-                           d.diag.p, d.ext.p, d.sidx.p, d.sval.p, x, b, n, nrhs, d.chainPtr.p, c0, nchain, xcd, d.h.phLocal[p],
-                           bX, bB, bN);
+                           st, d.slot.p, d.wave.p, d.off.p, d.len.p, d.extPtr.p, d.slotWave.p, d.rowId.p,
+                           d.diag.p, d.ext.p, d.sidx.p, d.sval.p, x, b, n, nrhs, d.chainPtr.p, c0, nchain, bundle0,
+                           xcd, bX, bB, bN);
 }
 
 // the variable-lanes kernels (tpr == 0)
@@ -648,6 +669,14 @@ class BundleFn : public DeviceFn {
     int waves(bool rev) const override {
         if (fallback) return fallback->waves(rev);
         return (int)(rev ? dr.h.critical : df.h.critical);
+    }
+    bool graphable() const override { return fallback ? fallback->graphable() : true; }
+    int launches(bool rev) const override {
+        if (fallback) return fallback->launches(rev);
+        const std::vector<int>& pc = (rev ? dr : df).h.phaseChain;
+        int k = 0;
+        for (size_t p = 0; p + 1 < pc.size(); ++p) k += pc[p + 1] > pc[p];
+        return k;
     }
     int run(bool rev, hipStream_t st) override {
         if (fallback) {

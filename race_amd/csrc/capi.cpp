@@ -18,6 +18,7 @@
 #include "simdify.hpp"
 #include "engine.hpp"
 #include "exec.hpp"
+#include "graph_cache.hpp"
 #include "schedule_io.hpp"
 
 using namespace race;
@@ -34,6 +35,7 @@ struct HostFn {
 struct Function {
     std::unique_ptr<HostFn> host;
     std::unique_ptr<DeviceFn> dev;
+    std::unique_ptr<GraphCache> graph;  // with the "graph" option: captured launch sequences
 };
 
 // Pattern symmetrisation: add (c,r) for every (r,c) without its pair
@@ -91,7 +93,7 @@ struct race_handle_s {
     std::vector<Function> funcs;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int optLayout = 0, optBundleRows = 256, optBundleLocal = 6144, optBundleThreads = 512, optBundleTpr = 0, optXcd = 0;
-    int optBfsBlockMax = 1 << 17, optSymmMode = 0;
+    int optBfsBlockMax = 1 << 17, optSymmMode = 0, optGraph = 0;
     ~race_handle_s() {
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
@@ -439,6 +441,9 @@ int race_set_option(race_handle h, const char* key, int value) {
     } else if (k == "symm_mode") {
         if (value < 0 || value > 3) return ERR_INVALID_ARG;
         h->optSymmMode = value;
+    } else if (k == "graph") {
+        if (value != 0 && value != 1) return ERR_INVALID_ARG;
+        h->optGraph = value;
     } else if (k == "bfs_block_max") {
         if (value < 0) return ERR_INVALID_ARG;
         h->optBfsBlockMax = value;
@@ -526,6 +531,10 @@ int race_execute_async(race_handle h, int id, int rev, void* stream) {
         return OK;
     }
     hipStream_t st = stream ? (hipStream_t)stream : h->stream;
+    if (h->optGraph && fn.dev->graphable()) {
+        if (!fn.graph) fn.graph = std::make_unique<GraphCache>();
+        return fn.graph->run(fn.dev.get(), rev != 0, st);
+    }
     return fn.dev->run(rev != 0, st);
 }
 
@@ -711,6 +720,13 @@ int race_get_mpk_groups(race_handle h, int* ngroups, int* group_ptr, int cap) {
     *ngroups = (int)h->mpk.groupPtr.size() - 1;
     if (group_ptr && (int)h->mpk.groupPtr.size() <= cap)
         std::memcpy(group_ptr, h->mpk.groupPtr.data(), sizeof(int) * h->mpk.groupPtr.size());
+    return OK;
+}
+
+int race_get_launch_count(race_handle h, int id, int rev, int* launches) {
+    if (checkHandle(h) || id < 0 || id >= (int)h->funcs.size() || !launches || !h->funcs[id].dev)
+        return ERR_INVALID_ARG;
+    *launches = h->funcs[id].dev->launches(rev != 0);
     return OK;
 }
 

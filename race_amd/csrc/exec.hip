@@ -264,6 +264,7 @@ class WaveFn : public DeviceFn {
                                w.wavePtr.p, w.rows.p, l0);
     }
 
+    bool graphable() const override { return true; }
     int run(bool rev, hipStream_t st) override {
         const DevWaves& w = rev ? wr : wf;
         const std::vector<int>& pp = w.h.phaseLeafPtr;
@@ -310,6 +311,7 @@ class SymmSpmvFn : public DeviceFn {
         if ((e = upload(cHi, hHi.data(), hHi.size()))) return e;
         return OK;
     }
+    bool graphable() const override { return true; }
     int run(bool, hipStream_t st) override {
         for (size_t p = 0; p + 1 < phaseChunk.size(); ++p) {
             const int c0 = phaseChunk[p], nc = phaseChunk[p + 1] - phaseChunk[p];

@@ -30,6 +30,11 @@ class DeviceFn {
     virtual ~DeviceFn() {}
     virtual int run(bool rev, hipStream_t st) = 0;
     virtual int waves(bool /*rev*/) const { return 0; }
+    // run() only enqueues kernel launches on the stream (no host waits, no
+    // memsets): it can be captured into a HIP graph
+    virtual bool graphable() const { return false; }
+    // kernel launches one execution issues
+    virtual int launches(bool /*rev*/) const { return 1; }
     void* x = nullptr;
     void* b = nullptr;
     int power = 1;

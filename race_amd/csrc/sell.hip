@@ -331,6 +331,13 @@ class MpkFn : public DeviceFn {
         }
         return upload(segs, all.data(), all.size());
     }
+    bool graphable() const override { return true; }
+    int launches(bool) const override {
+        if (power < 1 || power > powerMax) return 0;
+        int k = 0;
+        for (int nb : stepBlocks[power]) k += nb > 0;
+        return k;
+    }
     int run(bool, hipStream_t st) override {
         if (power < 1 || power > powerMax) return ERR_INVALID_ARG;
         const std::vector<int>& sg = stepSeg[power];

@@ -278,6 +278,12 @@ class Interface:
         data = bytes(data)
         L.check(self._L.race_schedule_load(self.h, data, len(data)), "loadSchedule")
 
+    def launchCount(self, fid, rev=False):
+        """kernel launches one execution of a device function issues"""
+        v = ctypes.c_int()
+        L.check(self._L.race_get_launch_count(self.h, fid, int(rev), ctypes.byref(v)), "launchCount")
+        return v.value
+
     def waveCount(self, fid, rev=False):
         v = ctypes.c_int()
         L.check(self._L.race_get_wave_count(self.h, fid, int(rev), ctypes.byref(v)), "waveCount")
