@@ -125,10 +125,11 @@ int race_register_device(race_handle h, const race_kernel_desc *desc, int *id);
  *                  1 = one workgroup per leaf (CRS),
  *                  2 = leaf bundles only (registration fails with
  *                      RACE_ERR_INCOMPATIBILITY if a leaf does not fit)
- *   "bundle_rows"  rows per bundle (default 256)
+ *   "bundle_rows"  rows per bundle (default 128)
  *   "bundle_local" LDS columns (rows + outside columns) per bundle (default 6144)
  *   "xcd_order"    1 = consecutive bundles of a phase run on the same XCD (default 0)
- *   "bundle_threads" workgroup size of the bundle kernels: 128, 256, 512 (default) or 1024
+ *   "bundle_threads" workgroup size of the bundle kernels: 64 (one wavefront), 128,
+ *                  256 (default), 512 or 1024
  *   "bundle_tpr"   lanes per matrix row in the bundle kernels: 1, 2, 4 (0 = auto:
  *                  from the longest row; rows longer than 64 entries select
  *                  variable lanes per row, 16 entries per lane)
@@ -235,7 +236,7 @@ typedef struct race_batch_s *race_batch;
 int race_batch_create(race_batch *out, int kind, int val_precision, int vec_precision, int nrhs);
 /* d: like race_register_device (host arrays are copied) */
 int race_batch_add(race_batch b, race_handle h, const race_kernel_desc *d);
-/* "bundle_threads", "bundle_rows", "bundle_tpr" (0, 1, 2, 4, 8, 16, 64),
+/* "bundle_threads", "bundle_rows", "bundle_tpr" (0, 1, 2, 4),
  * "bundle_local" (GS / KACZ); "sell_c" (8, 16, 32, 64), "sell_sigma"
  * (row-sorting window, 1 = none) (SpMV / SymmSpMV) */
 int race_batch_set_option(race_batch b, const char *key, int value);

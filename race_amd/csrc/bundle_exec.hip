@@ -596,6 +596,7 @@ void launchPhase(int bs, int tpr, int E, int kind, int nrhs, const BundleDev<VT,
         default: launchBundles<BS_, 4, 8, VT, XT, BATCH>(kind, nrhs, d, c0, nchain, p, xcd, x, b, n, bX, bB, bN, st); break;   \
     }
     switch (bs) {
+        case 64: RACE_BUNDLE_CASE(64) break;
         case 128: RACE_BUNDLE_CASE(128) break;
         case 512: RACE_BUNDLE_CASE(512) break;
         case 1024: RACE_BUNDLE_CASE(1024) break;
@@ -824,7 +825,7 @@ class BundleBatchFn : public DeviceFn {
 
 int makeBundleBatchFn(const std::vector<BatchItem>& items, const ExecContext& ctx, std::unique_ptr<DeviceFn>* out) {
     if (items.empty()) return ERR_INVALID_ARG;
-    if (ctx.bundleThreads != 128 && ctx.bundleThreads != 256 && ctx.bundleThreads != 512 &&
+    if (ctx.bundleThreads != 64 && ctx.bundleThreads != 128 && ctx.bundleThreads != 256 && ctx.bundleThreads != 512 &&
         ctx.bundleThreads != 1024)
         return ERR_INVALID_ARG;
     const race_kernel_desc& d0 = items[0].d;
@@ -852,7 +853,7 @@ int makeBundleBatchFn(const std::vector<BatchItem>& items, const ExecContext& ct
 }
 
 int makeBundleFn(const race_kernel_desc& d, const ExecContext& ctx, std::unique_ptr<DeviceFn>* out) {
-    if (ctx.bundleThreads != 128 && ctx.bundleThreads != 256 && ctx.bundleThreads != 512 &&
+    if (ctx.bundleThreads != 64 && ctx.bundleThreads != 128 && ctx.bundleThreads != 256 && ctx.bundleThreads != 512 &&
         ctx.bundleThreads != 1024)
         return ERR_INVALID_ARG;
     int e;

@@ -92,7 +92,7 @@ struct race_handle_s {
     std::vector<int> perm, inv;
     std::vector<Function> funcs;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    int optLayout = 0, optBundleRows = 256, optBundleLocal = 6144, optBundleThreads = 512, optBundleTpr = 0, optXcd = 0;
+    int optLayout = 0, optBundleRows = 128, optBundleLocal = 6144, optBundleThreads = 256, optBundleTpr = 0, optXcd = 0;
     int optBfsBlockMax = 1 << 17, optSymmMode = 0, optGraph = 0;
     ~race_handle_s() {
         if (ev0) (void)hipEventDestroy(ev0);
@@ -427,7 +427,7 @@ int race_set_option(race_handle h, const char* key, int value) {
         if (value < 1 || value > 8192) return ERR_INVALID_ARG;
         h->optBundleRows = value;
     } else if (k == "bundle_threads") {
-        if (value != 128 && value != 256 && value != 512 && value != 1024) return ERR_INVALID_ARG;
+        if (value != 64 && value != 128 && value != 256 && value != 512 && value != 1024) return ERR_INVALID_ARG;
         h->optBundleThreads = value;
     } else if (k == "bundle_tpr") {
         if (value != 0 && value != 1 && value != 2 && value != 4) return ERR_INVALID_ARG;
@@ -841,7 +841,7 @@ int race_batch_set_option(race_batch b, const char* key, int value) {
     if (!b || !key) return ERR_INVALID_ARG;
     const std::string k(key);
     if (k == "bundle_threads") {
-        if (value != 128 && value != 256 && value != 512 && value != 1024) return ERR_INVALID_ARG;
+        if (value != 64 && value != 128 && value != 256 && value != 512 && value != 1024) return ERR_INVALID_ARG;
         b->ctx.bundleThreads = value;
     } else if (k == "bundle_rows") {
         if (value < 1 || value > 8192) return ERR_INVALID_ARG;

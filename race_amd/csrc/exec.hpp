@@ -48,10 +48,10 @@ struct ExecContext {
     const Schedule* rev = nullptr;
     const MPKPartition* mpk = nullptr;
     int layout = 0;           // 0: leaf bundles (SELL, LDS waves), 1: one workgroup per leaf
-    int bundleRows = 256;     // rows per bundle
+    int bundleRows = 128;     // rows per bundle
     int bundleLocal = 6144;   // LDS columns (local column space) per bundle
     int xcdOrder = 0;         // map consecutive bundles to the same XCD
-    int bundleThreads = 512;  // workgroup size of the bundle kernels (128 ... 1024)
+    int bundleThreads = 256;  // workgroup size of the bundle kernels (64 ... 1024)
     int bundleTpr = 0;        // lanes per row in the bundle kernels (0: from the row length)
     int symmMode = 0;         // SymmSpMV: 0 auto, 1 colour phases, 2 SELL atomics, 3 expanded
     int sellC = 64;           // batched SELL kernels: rows per chunk (8, 16, 32, 64)

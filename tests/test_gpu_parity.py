@@ -166,6 +166,32 @@ def test_kacz_matches_race_order(oracle, T, layout, rows, local):
 
 
 @pytest.mark.parametrize("kind", ["gs", "kacz"])
+@pytest.mark.parametrize("threads", [64, 128, 256, 512, 1024])
+def test_bundle_workgroup_sizes(oracle, kind, threads):
+    """Every bundle workgroup size (64 = one wavefront, rows per bundle
+    capped by the lanes) against the oracle in RACE order."""
+    rp, col = M.stencil27(12) if kind == "gs" else M.stencil7(14)
+    val = M.gs_values(rp, col, seed=9)
+    n = len(rp) - 1
+    dist = R.ONE if kind == "gs" else R.TWO
+    ce, (prp, pcol, pval), zt = _prepared(rp, col, val, 256, dist, oracle)
+    b = M.uniform_vec(n, 13)
+    dx = R.DeviceArray.from_host(np.zeros(n))
+    db = R.DeviceArray.from_host(b)
+    ce.setOption("layout", 2)
+    ce.setOption("bundle_threads", threads)
+    ce.setOption("bundle_rows", 1024)
+    k = R.K_GS if kind == "gs" else R.K_KACZ
+    okind = oracle.K_GS if kind == "gs" else oracle.K_KACZ
+    fid = ce.registerDevice(k, prp, pcol, pval, dx, db)
+    xo = np.zeros(n)
+    for rev in (False, True):
+        ce.executeFunction(fid, rev)
+        oracle.run_calls(okind, prp, pcol, pval, xo, b, oracle.leaf_calls(zt, rev))
+        assert relerr(dx.download(), xo) <= TOL
+
+
+@pytest.mark.parametrize("kind", ["gs", "kacz"])
 @pytest.mark.parametrize("T,rows", [(4, 0), (16, 32)])
 def test_long_rows_variable_lanes(oracle, kind, T, rows):
     """A Chung-Lu graph with hub rows of up to ~300 entries: rows longer
