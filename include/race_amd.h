@@ -85,6 +85,18 @@ int race_register_function(race_handle h, race_color_fn f, void *args, int *id);
  * subPower, numaSplit) (include/interface.h:163, src/interface.cpp:435-458) */
 int race_register_power_function(race_handle h, race_power_fn f, void *args, int power,
                                  int sub_power, int numa_split, int *id);
+/* Interface::attachCommunicationToFunction (include/interface.h:161-165,
+ * src/interface.cpp:487-533): a halo-exchange callback comm(pow, subPow,
+ * args) for a POWER function (RACE_ERR_NOT_IMPLEMENTED for colouring
+ * handles).  The reference calls it only in its MPI build
+ * (RACE_ENABLE_MPI_MPK, src/functionManager.cpp:963-1067); on one device
+ * there are no remote rows, so it is stored and never called, as in the
+ * reference's non-MPI build.  The _simple form comm(pow, args) needs
+ * highestSubPower == 1 (else RACE_ERR_INCOMPATIBILITY). */
+typedef void (*race_comm_fn)(int pow, int sub_pow, void *args);
+typedef void (*race_comm_simple_fn)(int pow, void *args);
+int race_attach_communication(race_handle h, int id, race_comm_fn f, void *args);
+int race_attach_communication_simple(race_handle h, int id, race_comm_simple_fn f, void *args);
 
 /* ---- device kernels (the GPU path) ---- */
 enum race_kernel_kind {
@@ -188,6 +200,16 @@ int race_get_schedule(race_handle h, int rev, int *nphases, int *nleaves, int *p
 int race_get_tree(race_handle h, int *nleaves, int *data6, int cap);
 /* MPK level groups: group_ptr[ngroups+1] (row offsets) */
 int race_get_mpk_groups(race_handle h, int *ngroups, int *group_ptr, int cap);
+
+/* ---- initial permutation ----
+ * Reverse Cuthill-McKee order of a symmetric pattern, perm[new] = old, the
+ * initial permutation the reference's harness passes to the Interface
+ * constructor (sparsemat::doRCM examples/src/helpers/sparsemat.cpp:715-760,
+ * via SpMP's CSR::getRCMPermutation, which is not in the reference tree):
+ * per component a George-Liu pseudo-peripheral root, neighbours visited in
+ * increasing degree.  RACE_ERR_MATRIX_SYMM for an unsymmetric pattern.
+ * Host only. */
+int race_rcm(int n, const int *rowPtr, const int *col, int *perm, int *inv_perm);
 
 /* ---- schedule save / load ----
  * No counterpart in the reference, which rebuilds its zone tree and MPK tree

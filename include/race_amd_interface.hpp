@@ -144,6 +144,15 @@ class Interface {
             [f](int s, int e, int p, int, int numa, void* a) { f(s, e, p, numa, a); };
         return registerFunction(g, args, power, 1, numaSplit);
     }
+    // Interface::attachCommunicationToFunction (include/interface.h:161-165):
+    // stored, never called on one device (no remote rows), as in the
+    // reference's non-MPI build
+    int attachCommunicationToFunction(int funcId, void (*f)(int, int, void*), void* args) {
+        return race_attach_communication(h_, funcId, f, args);
+    }
+    int attachCommunicationToFunction(int funcId, void (*f)(int, void*), void* args) {
+        return race_attach_communication_simple(h_, funcId, f, args);
+    }
     // GPU kernels: the typed device registration (no reference counterpart)
     int registerDevice(const race_kernel_desc& d) {
         int id = -1;

@@ -46,6 +46,8 @@ class KernelDesc(ctypes.Structure):
 COLOR_FN = ctypes.CFUNCTYPE(None, ctypes.c_int, ctypes.c_int, ctypes.c_void_p)
 POWER_FN = ctypes.CFUNCTYPE(None, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                             ctypes.c_int, ctypes.c_void_p)
+COMM_FN = ctypes.CFUNCTYPE(None, ctypes.c_int, ctypes.c_int, ctypes.c_void_p)
+COMM_SIMPLE_FN = ctypes.CFUNCTYPE(None, ctypes.c_int, ctypes.c_void_p)
 
 # name -> (restype, argtypes)
 PROTOTYPES = {
@@ -63,6 +65,8 @@ PROTOTYPES = {
     "race_get_max_stage_depth": (ctypes.c_int, [vp, i32p]),
     "race_get_highest_power": (ctypes.c_int, [vp, i32p]),
     "race_register_function": (ctypes.c_int, [vp, COLOR_FN, vp, i32p]),
+    "race_attach_communication": (ctypes.c_int, [vp, ctypes.c_int, COMM_FN, vp]),
+    "race_attach_communication_simple": (ctypes.c_int, [vp, ctypes.c_int, COMM_SIMPLE_FN, vp]),
     "race_register_power_function": (ctypes.c_int, [vp, POWER_FN, vp, ctypes.c_int, ctypes.c_int,
                                                     ctypes.c_int, i32p]),
     "race_register_device": (ctypes.c_int, [vp, ctypes.POINTER(KernelDesc), i32p]),
@@ -83,6 +87,7 @@ PROTOTYPES = {
     "race_get_tree": (ctypes.c_int, [vp, i32p, i32p, ctypes.c_int]),
     "race_get_mpk_groups": (ctypes.c_int, [vp, i32p, i32p, ctypes.c_int]),
     "race_get_launch_count": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, i32p]),
+    "race_rcm": (ctypes.c_int, [ctypes.c_int, i32p, i32p, i32p, i32p]),
     "race_schedule_save": (ctypes.c_int, [vp, vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
     "race_schedule_load": (ctypes.c_int, [vp, vp, ctypes.c_size_t]),
     "race_schedule_info": (ctypes.c_int, [vp, ctypes.c_size_t, i32p, i32p, i32p]),

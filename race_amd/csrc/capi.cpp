@@ -36,6 +36,11 @@ struct Function {
     std::unique_ptr<HostFn> host;
     std::unique_ptr<DeviceFn> dev;
     std::unique_ptr<GraphCache> graph;  // with the "graph" option: captured launch sequences
+    // attached halo-exchange callback (one of the two forms); one device has
+    // no remote rows, so it is never called
+    race_comm_fn comm = nullptr;
+    race_comm_simple_fn commSimple = nullptr;
+    void* commArgs = nullptr;
 };
 
 // Pattern symmetrisation: add (c,r) for every (r,c) without its pair
@@ -383,6 +388,27 @@ int race_register_power_function(race_handle h, race_power_fn f, void* args, int
     fn.host->subPower = sub_power;
     h->funcs.push_back(std::move(fn));
     *id = (int)h->funcs.size() - 1;
+    return OK;
+}
+
+int race_attach_communication(race_handle h, int id, race_comm_fn f, void* args) {
+    if (checkHandle(h) || id < 0 || id >= (int)h->funcs.size() || !f) return ERR_INVALID_ARG;
+    if (h->dist != DIST_POWER) return ERR_NOT_IMPLEMENTED;  // src/interface.cpp:489-494
+    Function& fn = h->funcs[id];
+    fn.comm = f;
+    fn.commSimple = nullptr;
+    fn.commArgs = args;
+    return OK;
+}
+
+int race_attach_communication_simple(race_handle h, int id, race_comm_simple_fn f, void* args) {
+    if (checkHandle(h) || id < 0 || id >= (int)h->funcs.size() || !f) return ERR_INVALID_ARG;
+    if (h->dist != DIST_POWER) return ERR_NOT_IMPLEMENTED;
+    if (h->highestSubPower != 1) return ERR_INCOMPATIBILITY;  // src/interface.cpp:521-531
+    Function& fn = h->funcs[id];
+    fn.comm = nullptr;
+    fn.commSimple = f;
+    fn.commArgs = args;
     return OK;
 }
 

@@ -19,7 +19,7 @@ from ._lib import (BF16, FP32, FP64, K_GS, K_KACZ, K_MPK, K_MPK_HIGHEST, K_SPMTV
 
 __all__ = ["Interface", "DeviceArray", "lb_split", "RaceError", "ONE", "TWO", "POWER", "K_SYMMSPMV",
            "K_GS", "K_KACZ", "K_SPMV", "K_MPK", "K_SPMTV", "K_MPK_HIGHEST", "FP64", "FP32", "BF16", "LB_NNZ", "LB_ROW",
-           "TWO_BLOCK", "device_count", "schedule_info"]
+           "TWO_BLOCK", "device_count", "schedule_info", "rcm"]
 
 
 def _i32(a):
@@ -164,6 +164,19 @@ class Interface:
                     "registerFunction(power)")
         self._keep.append(cb)
         return fid.value
+
+    def attachCommunicationToFunction(self, fid, f, args=None, simple=False):
+        """Interface::attachCommunicationToFunction: f(pow, subPow, args), or
+        f(pow, args) with simple=True.  Stored, never called on one device
+        (no remote rows), as in the reference's non-MPI build."""
+        if simple:
+            cb = L.COMM_SIMPLE_FN(lambda p, a: f(p, args))
+            st = self._L.race_attach_communication_simple(self.h, int(fid), cb, None)
+        else:
+            cb = L.COMM_FN(lambda p, sp, a: f(p, sp, args))
+            st = self._L.race_attach_communication(self.h, int(fid), cb, None)
+        L.check(st, "attachCommunicationToFunction")
+        self._keep.append(cb)
 
     def registerDevice(self, kind, rowPtr, col, val, x, b=None, nrhs=1, power=1,
                        val_precision=FP64, vec_precision=FP64):
@@ -407,6 +420,16 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def rcm(rowPtr, col):
+    """Reverse Cuthill-McKee order of a symmetric pattern: (perm, invPerm),
+    perm[new] = old, for Interface(..., initPerm=perm, initInvPerm=invPerm)."""
+    rp, c = _i32(rowPtr), _i32(col)
+    n = len(rp) - 1
+    perm, inv = np.empty(n, np.int32), np.empty(n, np.int32)
+    L.check(L.load().race_rcm(n, _ip(rp), _ip(c), _ip(perm), _ip(inv)), "rcm")
+    return perm, inv
 
 
 def schedule_info(data):

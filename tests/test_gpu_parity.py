@@ -285,6 +285,79 @@ def test_mpk_groups_and_powers(oracle):
         assert relerr(got[p], ref[p]) <= TOL, p
 
 
+@pytest.mark.parametrize("dist,T", [(R.ONE, 64), (R.TWO, 16)])
+def test_rcm_initial_permutation(oracle, dist, T):
+    """The harness path: a relabelled matrix, its reverse Cuthill-McKee order
+    as initPerm (examples/src/helpers/sparsemat.cpp:715-771), then
+    RACEColor: the permutation equals the oracle's bit for bit and GS / KACZ
+    follow the RACE order of the oracle's tree."""
+    import scipy.sparse as sp
+    rp, col = M.stencil27(10)
+    val = M.gs_values(rp, col, seed=3)
+    n = len(rp) - 1
+    q = np.random.default_rng(9).permutation(n)
+    A = sp.csr_matrix((val, col, rp), shape=(n, n))[q][:, q].tocsr()
+    A.sort_indices()
+    rp, col, val = A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data
+    perm0, inv0 = R.rcm(rp, col)
+    ce = R.Interface(n, T, dist, rp, col, symm_hint=True, initPerm=perm0, initInvPerm=inv0)
+    ce.RACEColor()
+    g = oracle.Graph(rp, col, symm_hint=True, init_perm=perm0, init_inv_perm=inv0)
+    zt = oracle.level_recursion(g, T, dist)
+    perm, inv = ce.getPerm(), ce.getInvPerm()
+    assert np.array_equal(perm, g.perm)
+    prp, pcol, pval = M.make_diag_first(*M.permute(rp, col, val, perm, inv))
+    b = M.uniform_vec(n, 2)
+    dx, db = R.DeviceArray.from_host(np.zeros(n)), R.DeviceArray.from_host(b)
+    kind, okind = (R.K_GS, oracle.K_GS) if dist == R.ONE else (R.K_KACZ, oracle.K_KACZ)
+    fid = ce.registerDevice(kind, prp, pcol, pval, dx, db)
+    xo = np.zeros(n)
+    for rev in (False, True):
+        ce.executeFunction(fid, rev)
+        oracle.run_calls(okind, prp, pcol, pval, xo, b, oracle.leaf_calls(zt, rev))
+        assert relerr(dx.download(), xo) <= TOL
+
+
+def test_attach_communication(oracle):
+    """attachCommunicationToFunction: accepted for POWER functions, stored and
+    never called (one device has no remote rows, as the reference's non-MPI
+    build); NOT_IMPLEMENTED for colouring handles; the two-argument form
+    needs highestSubPower 1."""
+    rp, col = M.stencil7(10)
+    val = M.laplace7_values(rp, col)
+    n, k = len(rp) - 1, 3
+    calls = []
+    ce = R.Interface(n, 1, R.POWER, rp, col, symm_hint=True)
+    ce.RACEColor(k, 64 * 1024.0)
+    prp, pcol, pval = M.permute(rp, col, val, ce.getPerm(), ce.getInvPerm())
+    xs = np.zeros((k + 1) * n)
+    xs[:n] = M.uniform_vec(n, 1)
+    dx = R.DeviceArray.from_host(xs)
+    fid = ce.registerDevice(R.K_MPK, prp, pcol, pval, dx, power=k)
+    ce.attachCommunicationToFunction(fid, lambda p, sp, a: calls.append((p, sp)))
+    ce.attachCommunicationToFunction(fid, lambda p, a: calls.append(p), simple=True)
+    hid = ce.registerFunction(lambda s, e, p, sp, nu, a: None, power=k)
+    ce.attachCommunicationToFunction(hid, lambda p, sp, a: calls.append((p, sp)))
+    ce.executeFunction(fid)
+    ce.executeFunction(hid)
+    assert calls == []
+    ref = oracle.mpk_serial(prp, pcol, pval, xs[:n], k)
+    assert relerr(dx.download().reshape(k + 1, n)[k], ref[k]) <= TOL
+    with pytest.raises(R.RaceError):
+        ce.attachCommunicationToFunction(99, lambda p, sp, a: None)
+    sub = R.Interface(n, 1, R.POWER, rp, col, symm_hint=True)
+    sub.RACEColor(2, 64 * 1024.0, highestSubPower=2)
+    sid = sub.registerFunction(lambda s, e, p, sp, nu, a: None, power=2, subPower=2)
+    sub.attachCommunicationToFunction(sid, lambda p, sp, a: None)
+    with pytest.raises(R.RaceError, match="INCOMPATIBILITY"):
+        sub.attachCommunicationToFunction(sid, lambda p, a: None, simple=True)
+    col1 = _iface(rp, col, 4, R.ONE)
+    col1.RACEColor()
+    cid = col1.registerFunction(lambda s, e, a: None)
+    with pytest.raises(R.RaceError, match="NOT_IMPLEMENTED"):
+        col1.attachCommunicationToFunction(cid, lambda p, sp, a: None)
+
+
 def test_registered_vectors_stay_alive(oracle):
     """A device array passed to registerDevice without another reference
     must live as long as the registration (the kernel writes through it)."""

@@ -42,6 +42,27 @@ def test_bfs_closed_form_7pt(oracle, g):
     assert np.array_equal(G.inv[G.perm], np.arange(G.n))
 
 
+@pytest.mark.parametrize("dist", [1, 2])
+def test_init_perm_composes(oracle, dist):
+    """initPerm (src/graph_SoA.cpp:64-79 with RACE_PERMUTE_ON_FLY): the
+    traversal runs on the initially permuted graph and composes into it.  A
+    grid relabelled by q and handed back its natural order as initPerm
+    must give perm = qinv[perm of the natural grid]."""
+    rp, col = M.stencil7(6)
+    n = len(rp) - 1
+    q = np.random.default_rng(4).permutation(n)
+    qinv = np.empty(n, np.int64)
+    qinv[q] = np.arange(n)
+    B = M.to_scipy(rp, col, np.ones(len(col)))[q][:, q].tocsr()
+    B.sort_indices()
+    ga = oracle.Graph(rp, col, symm_hint=True)
+    ga.traverse(dist, 0, n)
+    gb = oracle.Graph(B.indptr, B.indices, symm_hint=True, init_perm=qinv, init_inv_perm=q)
+    gb.traverse(dist, 0, n)
+    assert np.array_equal(gb.perm, qinv[ga.perm])
+    assert np.array_equal(gb.inv[gb.perm], np.arange(n))
+
+
 @pytest.mark.parametrize("g", [3, 6])
 def test_bfs_closed_form_27pt(oracle, g):
     rp, col = M.stencil27(g)
