@@ -175,7 +175,7 @@ __global__ void __launch_bounds__(BS) k_gs_bundle(
 template <int BS, int TPR, int E, typename VT, typename XT, bool BATCH>
 __global__ void __launch_bounds__(BS) k_kacz_bundle(
     const int* __restrict__ bSlot, const int* __restrict__ bWave, const long* __restrict__ bOff,
-    const int* __restrict__ bLen, const int* __restrict__ bExt, const int* __restrict__ waveSlot,
+    const int* __restrict__ bLen, const int* __restrict__ bExt, const uint16_t* __restrict__ slotWave,
     const int* __restrict__ rowId, const double* __restrict__ rowNorm, const int* __restrict__ ext,
     const uint16_t* __restrict__ sidx, const VT* __restrict__ sval, XT* __restrict__ x0,
     const XT* __restrict__ b0, int n0, int nrhs, const int* __restrict__ chainPtr, int chain0, int nchain, int xcd,
@@ -183,7 +183,7 @@ __global__ void __launch_bounds__(BS) k_kacz_bundle(
     XT* const* __restrict__ bX, const XT* const* __restrict__ bB, const int* __restrict__ bN) {
     extern __shared__ __align__(16) unsigned char smem[];
     XT* xl = reinterpret_cast<XT*>(smem);
-    int* ws = reinterpret_cast<int*>(xl + ldsLocal);
+    (void)ldsLocal;
 
     const int q = bundleOfBlock(nchain, xcd);
     if (q < 0) return;
@@ -198,7 +198,7 @@ __global__ void __launch_bounds__(BS) k_kacz_bundle(
         const int s0 = bSlot[bnd], nb = bSlot[bnd + 1] - s0;
         const long off = bOff[bnd];
         const int wb = bLen[bnd];
-        const int w0 = bWave[bnd], nw = bWave[bnd + 1] - w0;
+        const int nw = bWave[bnd + 1] - bWave[bnd];
         const int e0 = bExt[bnd], ne = bExt[bnd + 1] - e0;
         const int t = threadIdx.x, i = t / TPR, l = t % TPR;
         const bool active = i < nb;
@@ -213,21 +213,21 @@ __global__ void __launch_bounds__(BS) k_kacz_bundle(
             li[u] = in ? (int)sidx[e] : 0;
             lv[u] = in ? loadVal<XT>(sval + e) : XT(0);
         }
+        // issued with the entry block: row ids, norms, waves, column ids
         const int row = active ? rowId[s0 + i] : 0;
         const double rn = active ? rowNorm[s0 + i] : 1.0;
-        for (int w = t; w <= nw; w += BS) ws[w] = waveSlot[w0 + w] - s0;
-        __syncthreads();
-        int myWave = -1;
-        if (active)
-            for (int w = 0; w < nw; ++w)
-                if (i >= ws[w] && i < ws[w + 1]) myWave = w;
+        const int myWave = active ? (int)slotWave[s0 + i] : -1;
+        const int ex0 = t < ne ? ext[e0 + t] : 0;
+        const int ex1 = t + BS < ne ? ext[e0 + t + BS] : 0;
 
         // right-hand sides are independent: grid.y splits them over workgroups
         const int k0 = (int)blockIdx.y;
         const int k1 = gridDim.y > 1 ? k0 + 1 : nrhs;
         for (int k = k0; k < k1; ++k) {
             const size_t vo = (size_t)k * n;
-            for (int c = t; c < ne; c += BS) xl[c] = x[vo + ext[e0 + c]];
+            if (t < ne) xl[t] = x[vo + ex0];
+            if (t + BS < ne) xl[t + BS] = x[vo + ex1];
+            for (int c = t + 2 * BS; c < ne; c += BS) xl[c] = x[vo + ext[e0 + c]];
             const XT br = active ? b[vo + row] : XT(0);
             __syncthreads();
             for (int w = 0; w < nw; ++w) {
@@ -243,7 +243,9 @@ __global__ void __launch_bounds__(BS) k_kacz_bundle(
                 }
                 __syncthreads();
             }
-            for (int c = t; c < ne; c += BS) x[vo + ext[e0 + c]] = xl[c];
+            if (t < ne) x[vo + ex0] = xl[t];
+            if (t + BS < ne) x[vo + ex1] = xl[t + BS];
+            for (int c = t + 2 * BS; c < ne; c += BS) x[vo + ext[e0 + c]] = xl[c];
             __syncthreads();
         }
     };
@@ -267,7 +269,7 @@ __global__ void __launch_bounds__(BS) k_kacz_bundle(
 template <int BS, typename VT, typename XT, bool BATCH>
 __global__ void __launch_bounds__(BS) k_gs_bundle_vl(
     const int* __restrict__ bSlot, const int* __restrict__ bWave, const long* __restrict__ bOff,
-    const int* __restrict__ bLen, const int* __restrict__ bExt, const int* __restrict__ waveSlot,
+    const int* __restrict__ bLen, const int* __restrict__ bExt, const uint16_t* __restrict__ slotWave,
     const int* __restrict__ rowId, const XT* __restrict__ diag, const int* __restrict__ ext,
     const uint16_t* __restrict__ sidx, const VT* __restrict__ sval, XT* __restrict__ x0,
     const XT* __restrict__ b0, int n0, int nrhs, const int* __restrict__ chainPtr, int chain0, int nchain, int xcd,
@@ -278,7 +280,7 @@ __global__ void __launch_bounds__(BS) k_gs_bundle_vl(
     XT* xl = reinterpret_cast<XT*>(smem);
     XT* part = xl + ldsLocal;                               // BS partial sums
     int* rl0 = reinterpret_cast<int*>(part + BS);           // row -> first lane
-    int* ws = rl0 + ldsRows + 1;
+    (void)ldsRows;
 
     const int q = bundleOfBlock(nchain, xcd);
     if (q < 0) return;
@@ -291,7 +293,7 @@ __global__ void __launch_bounds__(BS) k_gs_bundle_vl(
         const int s0 = bSlot[bnd], nb = bSlot[bnd + 1] - s0;
         const long off = bOff[bnd];
         const int wb = bLen[bnd];
-        const int w0 = bWave[bnd], nw = bWave[bnd + 1] - w0;
+        const int nw = bWave[bnd + 1] - bWave[bnd];
         const int e0 = bExt[bnd], ne = bExt[bnd + 1] - e0;
         const int lp = lanePtr[bnd], nl = lanePtr[bnd + 1] - lp;
         const int t = threadIdx.x;
@@ -308,26 +310,32 @@ __global__ void __launch_bounds__(BS) k_gs_bundle_vl(
             li[u] = in ? (int)sidx[e] : 0;
             lv[u] = in ? loadVal<XT>(sval + e) : XT(0);
         }
+        // issued with the entry block: row ids, inverse diagonals, waves,
+        // outside-column ids; the first lane of every row goes to LDS and is
+        // read after the barrier that follows the gathers
         const int row = active ? rowId[s0 + i] : 0;
         const XT dg = active ? diag[s0 + i] : XT(1);  // 1 / a_rr
-        for (int w = t; w <= nw; w += BS) ws[w] = waveSlot[w0 + w] - s0;
+        const int myWave = active ? (int)slotWave[s0 + i] : -1;
+        const int ex0 = t < ne ? ext[e0 + t] : 0;
+        const int ex1 = t + BS < ne ? ext[e0 + t + BS] : 0;
         if (first) rl0[i] = t;
         if (t == 0) rl0[nb] = nl;
-        __syncthreads();
-        const int rowL0 = active ? rl0[i] : 0, rowL1 = active ? rl0[i + 1] : 0;
-        int myWave = -1;
-        if (active)
-            for (int w = 0; w < nw; ++w)
-                if (i >= ws[w] && i < ws[w + 1]) myWave = w;
+        int rowL0 = 0, rowL1 = 0;
 
         const int k0 = (int)blockIdx.y;
         const int k1 = gridDim.y > 1 ? k0 + 1 : nrhs;
         for (int k = k0; k < k1; ++k) {
             const size_t vo = (size_t)k * n;
-            for (int c = t; c < ne; c += BS) xl[nb + c] = x[vo + ext[e0 + c]];
+            if (t < ne) xl[nb + t] = x[vo + ex0];
+            if (t + BS < ne) xl[nb + t + BS] = x[vo + ex1];
+            for (int c = t + 2 * BS; c < ne; c += BS) xl[nb + c] = x[vo + ext[e0 + c]];
             if (first) xl[i] = x[vo + row];
             const XT br = first ? b[vo + row] : XT(0);
             __syncthreads();
+            if (k == k0 && active) {
+                rowL0 = rl0[i];
+                rowL1 = rl0[i + 1];
+            }
             for (int w = 0; w < nw; ++w) {
                 if (myWave == w) {
                     XT sum = XT(0);
@@ -360,7 +368,7 @@ __global__ void __launch_bounds__(BS) k_gs_bundle_vl(
 template <int BS, typename VT, typename XT, bool BATCH>
 __global__ void __launch_bounds__(BS) k_kacz_bundle_vl(
     const int* __restrict__ bSlot, const int* __restrict__ bWave, const long* __restrict__ bOff,
-    const int* __restrict__ bLen, const int* __restrict__ bExt, const int* __restrict__ waveSlot,
+    const int* __restrict__ bLen, const int* __restrict__ bExt, const uint16_t* __restrict__ slotWave,
     const int* __restrict__ rowId, const double* __restrict__ rowNorm, const int* __restrict__ ext,
     const uint16_t* __restrict__ sidx, const VT* __restrict__ sval, XT* __restrict__ x0,
     const XT* __restrict__ b0, int n0, int nrhs, const int* __restrict__ chainPtr, int chain0, int nchain, int xcd,
@@ -371,7 +379,7 @@ __global__ void __launch_bounds__(BS) k_kacz_bundle_vl(
     XT* xl = reinterpret_cast<XT*>(smem);
     XT* part = xl + ldsLocal;                               // BS partial sums
     int* rl0 = reinterpret_cast<int*>(part + BS);           // row -> first lane
-    int* ws = rl0 + ldsRows + 1;
+    (void)ldsRows;
 
     const int q = bundleOfBlock(nchain, xcd);
     if (q < 0) return;
@@ -384,7 +392,7 @@ __global__ void __launch_bounds__(BS) k_kacz_bundle_vl(
         const int s0 = bSlot[bnd], nb = bSlot[bnd + 1] - s0;
         const long off = bOff[bnd];
         const int wb = bLen[bnd];
-        const int w0 = bWave[bnd], nw = bWave[bnd + 1] - w0;
+        const int nw = bWave[bnd + 1] - bWave[bnd];
         const int e0 = bExt[bnd], ne = bExt[bnd + 1] - e0;
         const int lp = lanePtr[bnd], nl = lanePtr[bnd + 1] - lp;
         const int t = threadIdx.x;
@@ -400,25 +408,29 @@ __global__ void __launch_bounds__(BS) k_kacz_bundle_vl(
             li[u] = in ? (int)sidx[e] : 0;
             lv[u] = in ? loadVal<XT>(sval + e) : XT(0);
         }
+        // issued with the entry block (see k_gs_bundle_vl)
         const int row = active ? rowId[s0 + i] : 0;
         const double rn = active ? rowNorm[s0 + i] : 1.0;
-        for (int w = t; w <= nw; w += BS) ws[w] = waveSlot[w0 + w] - s0;
+        const int myWave = active ? (int)slotWave[s0 + i] : -1;
+        const int ex0 = t < ne ? ext[e0 + t] : 0;
+        const int ex1 = t + BS < ne ? ext[e0 + t + BS] : 0;
         if (active && (t == 0 || (int)laneSlot[lp + t - 1] != i)) rl0[i] = t;
         if (t == 0) rl0[nb] = nl;
-        __syncthreads();
-        const int rowL0 = active ? rl0[i] : 0, rowL1 = active ? rl0[i + 1] : 0;
-        int myWave = -1;
-        if (active)
-            for (int w = 0; w < nw; ++w)
-                if (i >= ws[w] && i < ws[w + 1]) myWave = w;
+        int rowL0 = 0, rowL1 = 0;
 
         const int k0 = (int)blockIdx.y;
         const int k1 = gridDim.y > 1 ? k0 + 1 : nrhs;
         for (int k = k0; k < k1; ++k) {
             const size_t vo = (size_t)k * n;
-            for (int c = t; c < ne; c += BS) xl[c] = x[vo + ext[e0 + c]];
+            if (t < ne) xl[t] = x[vo + ex0];
+            if (t + BS < ne) xl[t + BS] = x[vo + ex1];
+            for (int c = t + 2 * BS; c < ne; c += BS) xl[c] = x[vo + ext[e0 + c]];
             const XT br = active ? b[vo + row] : XT(0);
             __syncthreads();
+            if (k == k0 && active) {
+                rowL0 = rl0[i];
+                rowL1 = rl0[i + 1];
+            }
             for (int w = 0; w < nw; ++w) {
                 if (myWave == w) {
                     XT dot = XT(0);
@@ -437,7 +449,9 @@ __global__ void __launch_bounds__(BS) k_kacz_bundle_vl(
                 }
                 __syncthreads();
             }
-            for (int c = t; c < ne; c += BS) x[vo + ext[e0 + c]] = xl[c];
+            if (t < ne) x[vo + ex0] = xl[t];
+            if (t + BS < ne) x[vo + ex1] = xl[t + BS];
+            for (int c = t + 2 * BS; c < ne; c += BS) x[vo + ext[e0 + c]] = xl[c];
             __syncthreads();
         }
     };
@@ -475,7 +489,7 @@ int pickE(int tpr, int need) {
 template <typename VT, typename XT>
 struct BundleDev {
     BundleTable h;
-    DevBuf<int> slot, wave, len, ext, extPtr, waveSlot, rowId, chainPtr, lanePtr;
+    DevBuf<int> slot, wave, len, ext, extPtr, rowId, chainPtr, lanePtr;
     DevBuf<uint16_t> laneSlot, slotWave;
     DevBuf<long> off;
     DevBuf<uint16_t> sidx;
@@ -491,7 +505,6 @@ struct BundleDev {
         if ((e = upload(off, h.off.data(), h.off.size()))) return e;
         if ((e = upload(ext, h.ext.data(), h.ext.size()))) return e;
         if ((e = upload(extPtr, h.extPtr.data(), h.extPtr.size()))) return e;
-        if ((e = upload(waveSlot, h.waveSlot.data(), h.waveSlot.size()))) return e;
         if ((e = upload(chainPtr, h.chainPtr.data(), h.chainPtr.size()))) return e;
         if (h.tpr == 0) {
             if ((e = upload(lanePtr, h.lanePtr.data(), h.lanePtr.size()))) return e;
@@ -509,14 +522,14 @@ struct BundleDev {
             std::vector<XT> dg(h.diag.size());
             for (size_t q = 0; q < dg.size(); ++q) dg[q] = (XT)(1.0 / h.diag[q]);
             if ((e = upload(diag, dg.data(), dg.size()))) return e;
-            // slot -> its wave inside the bundle (a bundle holds at most 8192 rows)
-            std::vector<uint16_t> sw(h.rowId.size(), 0);
-            for (size_t bd = 0; bd + 1 < h.wave.size(); ++bd)
-                for (int w = h.wave[bd]; w < h.wave[bd + 1]; ++w)
-                    for (int sl = h.waveSlot[w]; sl < h.waveSlot[w + 1]; ++sl)
-                        sw[sl] = (uint16_t)(w - h.wave[bd]);
-            if ((e = upload(slotWave, sw.data(), sw.size()))) return e;
         }
+        // slot -> its wave inside the bundle (a bundle holds at most 8192 rows)
+        std::vector<uint16_t> sw(h.rowId.size(), 0);
+        for (size_t bd = 0; bd + 1 < h.wave.size(); ++bd)
+            for (int w = h.wave[bd]; w < h.wave[bd + 1]; ++w)
+                for (int sl = h.waveSlot[w]; sl < h.waveSlot[w + 1]; ++sl)
+                    sw[sl] = (uint16_t)(w - h.wave[bd]);
+        if ((e = upload(slotWave, sw.data(), sw.size()))) return e;
         // the host copies of the big arrays are no longer needed
         std::vector<uint16_t>().swap(h.sidx);
         std::vector<double>().swap(h.sval);
@@ -529,11 +542,11 @@ struct BundleDev {
             return (VT)v;
     }
     // LDS bytes of phase p for workgroups of bs threads: the local column
-    // space, the wave table and, with variable lanes, a partial sum per lane
-    // and the first lane of every row
+    // space and, with variable lanes, a partial sum per lane and the first
+    // lane of every row
     size_t lds(int p, int bs) const {
         const size_t var = h.tpr == 0 ? (size_t)bs * sizeof(XT) + (size_t)(h.phRows[p] + 1) * sizeof(int) : 0;
-        return (size_t)h.phLocal[p] * sizeof(XT) + var + (size_t)(h.phWaves[p] + 1) * sizeof(int) + 32;
+        return (size_t)h.phLocal[p] * sizeof(XT) + var + 32;
     }
 };
 
@@ -547,7 +560,7 @@ void launchBundles(int kind, int nrhs, const BundleDev<VT, XT>& d, int c0, int n
     const int bundle0 = cp[c0 + nchain] - cp[c0] == nchain ? cp[c0] : -1;
     if (kind == RACE_K_KACZ)
         hipLaunchKernelGGL((k_kacz_bundle<BS, TPR, EE, VT, XT, BATCH>), grid, dim3(BS), d.lds(p, BS), st,
-                           d.slot.p, d.wave.p, d.off.p, d.len.p, d.extPtr.p, d.waveSlot.p, d.rowId.p,
+                           d.slot.p, d.wave.p, d.off.p, d.len.p, d.extPtr.p, d.slotWave.p, d.rowId.p,
                            d.rowNorm.p, d.ext.p, d.sidx.p, d.sval.p, x, b, n, nrhs, d.chainPtr.p, c0, nchain, xcd,
                            d.h.phLocal[p], bX, bB, bN);
     else if (nrhs == 1)
@@ -569,12 +582,12 @@ void launchBundlesVL(int kind, int nrhs, const BundleDev<VT, XT>& d, int c0, int
     const dim3 grid(xcd ? 8u * (unsigned)((nchain + 7) / 8) : (unsigned)nchain, (unsigned)std::max(1, nrhs));
     if (kind == RACE_K_KACZ)
         hipLaunchKernelGGL((k_kacz_bundle_vl<BS, VT, XT, BATCH>), grid, dim3(BS), d.lds(p, BS), st, d.slot.p, d.wave.p,
-                           d.off.p, d.len.p, d.extPtr.p, d.waveSlot.p, d.rowId.p, d.rowNorm.p, d.ext.p, d.sidx.p,
+                           d.off.p, d.len.p, d.extPtr.p, d.slotWave.p, d.rowId.p, d.rowNorm.p, d.ext.p, d.sidx.p,
                            d.sval.p, x, b, n, nrhs, d.chainPtr.p, c0, nchain, xcd, d.h.phLocal[p], d.h.phRows[p],
                            d.lanePtr.p, d.laneSlot.p, bX, bB, bN);
     else
         hipLaunchKernelGGL((k_gs_bundle_vl<BS, VT, XT, BATCH>), grid, dim3(BS), d.lds(p, BS), st, d.slot.p, d.wave.p,
-                           d.off.p, d.len.p, d.extPtr.p, d.waveSlot.p, d.rowId.p, d.diag.p, d.ext.p, d.sidx.p,
+                           d.off.p, d.len.p, d.extPtr.p, d.slotWave.p, d.rowId.p, d.diag.p, d.ext.p, d.sidx.p,
                            d.sval.p, x, b, n, nrhs, d.chainPtr.p, c0, nchain, xcd, d.h.phLocal[p], d.h.phRows[p],
                            d.lanePtr.p, d.laneSlot.p, bX, bB, bN);
 }
