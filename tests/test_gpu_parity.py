@@ -73,6 +73,68 @@ def test_spin_symmetrized_perm(oracle, spin):
         assert np.array_equal(ce.getPerm(), g.perm)
 
 
+@pytest.mark.parametrize("T", [4, 32])
+def test_spin_fixture_kernels(oracle, spin, T):
+    """The reference's own matrix (examples/matrices/spinSZ12.mm.gz, 924 rows,
+    400 without a diagonal) through every device kernel: GS at distance 1,
+    KACZ and SymmSpMV at distance 2, MPK under POWER, each against the
+    oracle in the RACE order of the oracle's tree.  A missing diagonal is
+    inserted (as the harness's makeDiagFirst does) as 1 + the row's
+    absolute sum, so that GS and KACZ are defined."""
+    rp, col, val = spin
+    n = len(rp) - 1
+    for dist in (R.ONE, R.TWO):
+        print("case spin", T, dist, flush=True)
+        ce = R.Interface(n, T, dist, rp, col, symm_hint=False)
+        ce.RACEColor()
+        g = oracle.Graph(rp, col, symm_hint=False)
+        zt = oracle.level_recursion(g, T, dist)
+        perm, inv = ce.getPerm(), ce.getInvPerm()
+        assert np.array_equal(perm, g.perm)
+        prp, pcol, pval = M.make_diag_first(*M.permute(rp, col, val, perm, inv))
+        pval = pval.copy()
+        for r in range(n):
+            pval[prp[r]] = 1.0 + np.abs(pval[prp[r] + 1:prp[r + 1]]).sum()
+        b = M.uniform_vec(n, 17)
+        if dist == R.ONE:
+            kinds = [(R.K_GS, oracle.K_GS)]
+        else:
+            kinds = [(R.K_KACZ, oracle.K_KACZ)]
+        for kind, okind in kinds:
+            dx, db = R.DeviceArray.from_host(np.zeros(n)), R.DeviceArray.from_host(b)
+            fid = ce.registerDevice(kind, prp, pcol, pval, dx, db)
+            xo = np.zeros(n)
+            for rev in (False, True):
+                ce.executeFunction(fid, rev)
+                oracle.run_calls(okind, prp, pcol, pval, xo, b, oracle.leaf_calls(zt, rev))
+                assert relerr(dx.download(), xo) <= TOL, (kind, rev)
+        if dist == R.TWO:
+            urp, ucol, uval = M.upper_triangle(prp, pcol, pval)
+            x = M.uniform_vec(n, 18)
+            dx, db = R.DeviceArray.from_host(x), R.DeviceArray.from_host(np.zeros(n))
+            fid = ce.registerDevice(R.K_SYMMSPMV, urp, ucol, uval, dx, db)
+            ce.executeFunction(fid)
+            bo = np.zeros(n)
+            oracle.run_calls(oracle.K_SYMMSPMV, urp, ucol, uval, x.copy(), bo, oracle.leaf_calls(zt))
+            assert relerr(db.download(), bo) <= TOL
+    k = 4
+    ce = R.Interface(n, 1, R.POWER, rp, col, symm_hint=False)
+    ce.RACEColor(k, 16 * 1024.0)
+    g = oracle.Graph(rp, col, symm_hint=False)
+    part = oracle.mpk_partition(g, k, 16 * 1024.0)
+    assert np.array_equal(ce.getPerm(), g.perm)
+    assert np.array_equal(ce.mpkGroups(), part["groupPtr"])
+    prp, pcol, pval = M.permute(rp, col, val, ce.getPerm(), ce.getInvPerm())
+    xs = np.zeros((k + 1) * n)
+    xs[:n] = M.uniform_vec(n, 19)
+    dx = R.DeviceArray.from_host(xs)
+    ce.executeFunction(ce.registerDevice(R.K_MPK, prp, pcol, pval, dx, power=k))
+    ref = oracle.mpk_serial(prp, pcol, pval, xs[:n], k)
+    got = dx.download().reshape(k + 1, n)
+    for p in range(1, k + 1):
+        assert relerr(got[p], ref[p]) <= TOL, p
+
+
 @pytest.mark.parametrize("bmax", BFS_PATHS)
 @pytest.mark.parametrize("T", [2, 3, 8, 20, 64])
 @pytest.mark.parametrize("dist", [R.ONE, R.TWO])
