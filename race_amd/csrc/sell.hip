@@ -30,14 +30,18 @@ struct SellChunk {
 
 struct SellMatrix {
     int n = 0, nChunk = 0;
+    int uniformW = -1;          // >= 0: every chunk has this width, chunk c at c * 64 * width
     long entries = 0;
     std::vector<int> cutChunk;  // chunk index of every cut
     DevBuf<SellChunk> chunks;
     DevBuf<int> col;
     DevBuf<double> val;
 
-    // cuts: sorted row boundaries starting with 0 and ending with n
-    int build(const race_kernel_desc& d, const std::vector<int>& cuts) {
+    // cuts: sorted row boundaries starting with 0 and ending with n.  With
+    // one range and chunk widths close to each other (padding every chunk to
+    // the widest adds at most 1/8), all chunks get the widest width: the
+    // kernels then compute a chunk's place instead of loading it.
+    int build(const race_kernel_desc& d, const std::vector<int>& cuts, bool allowUniform = false) {
         n = d.nrow;
         const int* rp = d.rowPtr;
         std::vector<SellChunk> ch;
@@ -51,6 +55,21 @@ struct SellMatrix {
                 for (int i = 0; i < rows; ++i) w = std::max(w, rp[r0 + i + 1] - rp[r0 + i]);
                 ch.push_back(SellChunk{r0, rows, w, 0, total});
                 total += (long)w * kC;
+            }
+        }
+        uniformW = -1;
+        if (allowUniform && cuts.size() == 2 && !ch.empty()) {
+            int wmax = 0;
+            for (const SellChunk& c : ch) wmax = std::max(wmax, c.width);
+            const long padded = (long)wmax * kC * (long)ch.size();
+            if (padded * 8 <= total * 9) {
+                uniformW = wmax;
+                total = 0;
+                for (SellChunk& c : ch) {
+                    c.width = wmax;
+                    c.off = total;
+                    total += (long)wmax * kC;
+                }
             }
         }
         cutChunk.push_back((int)ch.size());
@@ -111,6 +130,32 @@ __global__ void __launch_bounds__(kBlock) k_sell_spmv(const SellChunk* __restric
     const SellChunk k = chunks[c];
     if (lane >= k.rows) return;
     y[k.row0 + lane] = sellRow(k, lane, col, val, x);
+}
+
+// the same over uniform chunks: no chunk records to load
+__device__ __forceinline__ double sellRowU(long off, int W, int lane, const int* __restrict__ col,
+                                           const double* __restrict__ val, const double* __restrict__ x) {
+    const SellChunk k{0, kC, W, 0, off};
+    return sellRow(k, lane, col, val, x);
+}
+
+template <bool ACC>
+__global__ void __launch_bounds__(kBlock) k_sell_spmv_u(int W, const int* __restrict__ col,
+                                                        const double* __restrict__ val,
+                                                        const double* __restrict__ x, double* __restrict__ y,
+                                                        int n, int nrhs) {
+    const int c = blockIdx.x * kWavesPerBlock + threadIdx.x / kC;
+    const int lane = threadIdx.x % kC;
+    const int r = c * kC + lane;
+    if (r >= n) return;
+    const long off = (long)c * kC * W;
+    for (int q = 0; q < nrhs; ++q) {
+        const double t = sellRowU(off, W, lane, col, val, x + (long)q * n);
+        if (ACC)
+            y[(long)q * n + r] += t;
+        else
+            y[(long)q * n + r] = t;
+    }
 }
 
 // b += A x for nrhs right-hand sides (vectors column-major, v[r + k n])
@@ -199,12 +244,16 @@ class SellSpmvFn : public DeviceFn {
         kind = d.kind;
         x = d.x;
         b = d.b;
-        return A.build(d, {0, d.nrow});
+        return A.build(d, {0, d.nrow}, true);
     }
     int run(bool, hipStream_t st) override {
         const unsigned g = (unsigned)std::max(1, (A.nChunk + kWavesPerBlock - 1) / kWavesPerBlock);
-        hipLaunchKernelGGL(k_sell_spmv, dim3(g), dim3(kBlock), 0, st, A.chunks.p, A.nChunk, A.col.p, A.val.p,
-                           (const double*)x, (double*)b);
+        if (A.uniformW >= 0)
+            hipLaunchKernelGGL(k_sell_spmv_u<false>, dim3(g), dim3(kBlock), 0, st, A.uniformW, A.col.p, A.val.p,
+                               (const double*)x, (double*)b, A.n, 1);
+        else
+            hipLaunchKernelGGL(k_sell_spmv, dim3(g), dim3(kBlock), 0, st, A.chunks.p, A.nChunk, A.col.p, A.val.p,
+                               (const double*)x, (double*)b);
         RACE_HIP(hipGetLastError());
         return OK;
     }
@@ -255,13 +304,16 @@ class SymmSellFn : public DeviceFn {
         f.rowPtr = rp.data();
         f.col = fc.data();
         f.val = fv.data();
-        return A.build(f, {0, n});
+        return A.build(f, {0, n}, true);
     }
     int run(bool, hipStream_t st) override {
         const unsigned g = (unsigned)std::max(1, (A.nChunk + kWavesPerBlock - 1) / kWavesPerBlock);
         if (mode == 2)
             hipLaunchKernelGGL(k_sell_symm_atomic, dim3(g), dim3(kBlock), 0, st, A.chunks.p, A.nChunk, A.col.p,
                                A.val.p, (const double*)x, (double*)b, (long)A.n, nrhs);
+        else if (A.uniformW >= 0)
+            hipLaunchKernelGGL(k_sell_spmv_u<true>, dim3(g), dim3(kBlock), 0, st, A.uniformW, A.col.p, A.val.p,
+                               (const double*)x, (double*)b, A.n, nrhs);
         else
             hipLaunchKernelGGL(k_sell_spmv_acc, dim3(g), dim3(kBlock), 0, st, A.chunks.p, A.nChunk, A.col.p,
                                A.val.p, (const double*)x, (double*)b, (long)A.n, nrhs);
@@ -384,12 +436,16 @@ class SpmtvFn : public DeviceFn {
         t.rowPtr = rp.data();
         t.col = tc.data();
         t.val = tv.data();
-        return A.build(t, {0, n});
+        return A.build(t, {0, n}, true);
     }
     int run(bool, hipStream_t st) override {
         const unsigned g = (unsigned)std::max(1, (A.nChunk + kWavesPerBlock - 1) / kWavesPerBlock);
-        hipLaunchKernelGGL(k_sell_spmv_acc, dim3(g), dim3(kBlock), 0, st, A.chunks.p, A.nChunk, A.col.p, A.val.p,
-                           (const double*)x, (double*)b, (long)A.n, nrhs);
+        if (A.uniformW >= 0)
+            hipLaunchKernelGGL(k_sell_spmv_u<true>, dim3(g), dim3(kBlock), 0, st, A.uniformW, A.col.p, A.val.p,
+                               (const double*)x, (double*)b, A.n, nrhs);
+        else
+            hipLaunchKernelGGL(k_sell_spmv_acc, dim3(g), dim3(kBlock), 0, st, A.chunks.p, A.nChunk, A.col.p,
+                               A.val.p, (const double*)x, (double*)b, (long)A.n, nrhs);
         RACE_HIP(hipGetLastError());
         return OK;
     }
