@@ -465,15 +465,6 @@ __global__ void __launch_bounds__(BS) k_kacz_bundle_vl(
     }
 }
 
-uint16_t toBf16(double v) {
-    float f = (float)v;
-    uint32_t u;
-    std::memcpy(&u, &f, 4);
-    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);  // inf / nan
-    u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
-    return (uint16_t)(u >> 16);
-}
-
 // (lanes per row, register width) pairs with a uniform kernel
 // instantiation; longer rows use variable lanes per row
 int pickE(int tpr, int need) {

@@ -391,7 +391,11 @@ int makeDeviceFn(const race_kernel_desc& d, const ExecContext& ctx, std::unique_
     if (d.nrow <= 0 || !d.rowPtr || !d.col || !d.val) return ERR_INVALID_ARG;
     const bool fp64 = d.val_precision == RACE_FP64 && d.vec_precision == RACE_FP64;
     const bool bundleKind = d.kind == RACE_K_GS || d.kind == RACE_K_KACZ;
-    if (!fp64 && !(bundleKind && ctx.layout != 1)) return ERR_NOT_IMPLEMENTED;
+    // FP32 / BF16 storage: the bundle kernels and the SELL kernels (SpMV,
+    // SpMTV, SymmSpMV over the expanded matrix, MPK); the CRS kernels are FP64
+    const bool sellKind = d.kind == RACE_K_SPMV || d.kind == RACE_K_SPMTV || d.kind == RACE_K_MPK ||
+                          d.kind == RACE_K_MPK_HIGHEST || (d.kind == RACE_K_SYMMSPMV && ctx.symmMode != 1);
+    if (!fp64 && !(bundleKind && ctx.layout != 1) && !sellKind) return ERR_NOT_IMPLEMENTED;
     if (d.vec_precision != RACE_FP64 && d.vec_precision != RACE_FP32) return ERR_INVALID_ARG;
     if (d.val_precision < RACE_FP64 || d.val_precision > RACE_BF16) return ERR_INVALID_ARG;
     int e = OK;
@@ -422,7 +426,7 @@ int makeDeviceFn(const race_kernel_desc& d, const ExecContext& ctx, std::unique_
             break;
         }
         case RACE_K_SPMV: {
-            if (d.nrhs <= 1) {
+            if (d.nrhs <= 1 || !fp64) {
                 e = makeSellSpmvFn(d, out);
                 break;
             }

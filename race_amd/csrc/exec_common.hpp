@@ -2,6 +2,9 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
+#include <cstring>
+
 #include "engine.hpp"
 #include "hip_util.hpp"
 
@@ -15,6 +18,16 @@ __device__ __forceinline__ double groupSum(double v) {
 #pragma unroll
     for (int o = W / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, W);
     return v;
+}
+
+// bfloat16 bits of v (through float, round to nearest even)
+inline uint16_t toBf16(double v) {
+    float f = (float)v;
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);  // inf / nan
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
 }
 
 inline unsigned gridFor(long n, int block = kBlock) {

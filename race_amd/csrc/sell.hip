@@ -12,6 +12,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdint>
+#include <type_traits>
 
 #include "exec.hpp"
 #include "exec_common.hpp"
@@ -28,6 +30,29 @@ struct SellChunk {
     long off;
 };
 
+// matrix value storage (double, float, or bfloat16 bits); the arithmetic
+// runs in the vector type XT: FP64, or FP32 accumulation for FP32 / BF16
+template <typename VT>
+VT toStorage(double v) {
+    if constexpr (sizeof(VT) == 2)
+        return toBf16(v);
+    else
+        return (VT)v;
+}
+template <typename XT>
+__device__ __forceinline__ XT fromStorage(double v) {
+    return (XT)v;
+}
+template <typename XT>
+__device__ __forceinline__ XT fromStorage(float v) {
+    return (XT)v;
+}
+template <typename XT>
+__device__ __forceinline__ XT fromStorage(uint16_t v) {
+    return (XT)__uint_as_float((unsigned)v << 16);
+}
+
+template <typename VT>
 struct SellMatrix {
     int n = 0, nChunk = 0;
     int uniformW = -1;          // >= 0: every chunk has this width, chunk c at c * 64 * width
@@ -35,7 +60,7 @@ struct SellMatrix {
     std::vector<int> cutChunk;  // chunk index of every cut
     DevBuf<SellChunk> chunks;
     DevBuf<int> col;
-    DevBuf<double> val;
+    DevBuf<VT> val;
 
     // cuts: sorted row boundaries starting with 0 and ending with n.  With
     // one range and chunk widths close to each other (padding every chunk to
@@ -76,7 +101,7 @@ struct SellMatrix {
         nChunk = (int)ch.size();
         entries = total;
         std::vector<int> hc((size_t)total);
-        std::vector<double> hv((size_t)total);
+        std::vector<VT> hv((size_t)total);
         for (const SellChunk& c : ch)
             for (int i = 0; i < kC; ++i) {
                 const int r = c.row0 + std::min(i, c.rows - 1);
@@ -85,10 +110,10 @@ struct SellMatrix {
                     const long pos = c.off + (long)j * kC + i;
                     if (j < len) {
                         hc[pos] = d.col[rp[r] + j];
-                        hv[pos] = d.val[rp[r] + j];
+                        hv[pos] = toStorage<VT>(d.val[rp[r] + j]);
                     } else {
                         hc[pos] = r;
-                        hv[pos] = 0.0;
+                        hv[pos] = toStorage<VT>(0.0);
                     }
                 }
             }
@@ -102,74 +127,66 @@ struct SellMatrix {
 
 // y[row] = sum_j a_j x[c_j] for the rows of one chunk; sequential in j, as
 // the CRS row order
-__device__ __forceinline__ double sellRow(const SellChunk& k, int lane, const int* __restrict__ col,
-                                          const double* __restrict__ val, const double* __restrict__ x) {
+template <typename VT, typename XT>
+__device__ __forceinline__ XT sellRow(const SellChunk& k, int lane, const int* __restrict__ col,
+                                      const VT* __restrict__ val, const XT* __restrict__ x) {
     const int* __restrict__ cc = col + k.off + lane;
-    const double* __restrict__ vv = val + k.off + lane;
-    double t = 0.0;
+    const VT* __restrict__ vv = val + k.off + lane;
+    XT t = XT(0);
     int j = 0;
     for (; j + 4 <= k.width; j += 4) {
         const int c0 = cc[(j + 0) * kC], c1 = cc[(j + 1) * kC], c2 = cc[(j + 2) * kC], c3 = cc[(j + 3) * kC];
-        const double v0 = vv[(j + 0) * kC], v1 = vv[(j + 1) * kC], v2 = vv[(j + 2) * kC], v3 = vv[(j + 3) * kC];
-        const double x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
+        const XT v0 = fromStorage<XT>(vv[(j + 0) * kC]), v1 = fromStorage<XT>(vv[(j + 1) * kC]);
+        const XT v2 = fromStorage<XT>(vv[(j + 2) * kC]), v3 = fromStorage<XT>(vv[(j + 3) * kC]);
+        const XT x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
         t += v0 * x0;
         t += v1 * x1;
         t += v2 * x2;
         t += v3 * x3;
     }
-    for (; j < k.width; ++j) t += vv[j * kC] * x[cc[j * kC]];
+    for (; j < k.width; ++j) t += fromStorage<XT>(vv[j * kC]) * x[cc[j * kC]];
     return t;
 }
 
+// y = A x (ACC: y += A x) for nrhs right-hand sides (vectors column-major,
+// v[r + k n])
+template <bool ACC, typename VT, typename XT>
 __global__ void __launch_bounds__(kBlock) k_sell_spmv(const SellChunk* __restrict__ chunks, int nChunk,
-                                                      const int* __restrict__ col, const double* __restrict__ val,
-                                                      const double* __restrict__ x, double* __restrict__ y) {
+                                                      const int* __restrict__ col, const VT* __restrict__ val,
+                                                      const XT* __restrict__ x, XT* __restrict__ y, long n,
+                                                      int nrhs) {
     const int c = blockIdx.x * kWavesPerBlock + threadIdx.x / kC;
     const int lane = threadIdx.x % kC;
     if (c >= nChunk) return;
     const SellChunk k = chunks[c];
     if (lane >= k.rows) return;
-    y[k.row0 + lane] = sellRow(k, lane, col, val, x);
+    for (int q = 0; q < nrhs; ++q) {
+        const XT t = sellRow(k, lane, col, val, x + q * n);
+        if (ACC)
+            y[q * n + k.row0 + lane] += t;
+        else
+            y[q * n + k.row0 + lane] = t;
+    }
 }
 
-// the same over uniform chunks: no chunk records to load
-__device__ __forceinline__ double sellRowU(long off, int W, int lane, const int* __restrict__ col,
-                                           const double* __restrict__ val, const double* __restrict__ x) {
-    const SellChunk k{0, kC, W, 0, off};
-    return sellRow(k, lane, col, val, x);
-}
-
-template <bool ACC>
+// the same over uniform chunks: chunk c holds rows [64 c, 64 c + 64) at
+// c * 64 * W, no chunk records to load
+template <bool ACC, typename VT, typename XT>
 __global__ void __launch_bounds__(kBlock) k_sell_spmv_u(int W, const int* __restrict__ col,
-                                                        const double* __restrict__ val,
-                                                        const double* __restrict__ x, double* __restrict__ y,
-                                                        int n, int nrhs) {
+                                                        const VT* __restrict__ val, const XT* __restrict__ x,
+                                                        XT* __restrict__ y, int n, int nrhs) {
     const int c = blockIdx.x * kWavesPerBlock + threadIdx.x / kC;
     const int lane = threadIdx.x % kC;
     const int r = c * kC + lane;
     if (r >= n) return;
-    const long off = (long)c * kC * W;
+    const SellChunk k{0, kC, W, 0, (long)c * kC * W};
     for (int q = 0; q < nrhs; ++q) {
-        const double t = sellRowU(off, W, lane, col, val, x + (long)q * n);
+        const XT t = sellRow(k, lane, col, val, x + (long)q * n);
         if (ACC)
             y[(long)q * n + r] += t;
         else
             y[(long)q * n + r] = t;
     }
-}
-
-// b += A x for nrhs right-hand sides (vectors column-major, v[r + k n])
-__global__ void __launch_bounds__(kBlock) k_sell_spmv_acc(const SellChunk* __restrict__ chunks, int nChunk,
-                                                          const int* __restrict__ col,
-                                                          const double* __restrict__ val,
-                                                          const double* __restrict__ x, double* __restrict__ y,
-                                                          long n, int nrhs) {
-    const int c = blockIdx.x * kWavesPerBlock + threadIdx.x / kC;
-    const int lane = threadIdx.x % kC;
-    if (c >= nChunk) return;
-    const SellChunk k = chunks[c];
-    if (lane >= k.rows) return;
-    for (int q = 0; q < nrhs; ++q) y[q * n + k.row0 + lane] += sellRow(k, lane, col, val, x + q * n);
 }
 
 // SymmSpMV over the upper triangle (diagonal included) in one pass:
@@ -219,11 +236,11 @@ struct MpkSeg {
 
 // Power q is stored in vector slot q % nslot: nslot = k + 1 keeps every
 // power, nslot = 2 keeps the last two (the only-highest variant).
+template <typename VT, typename XT>
 __global__ void __launch_bounds__(kBlock) k_sell_mpk_step(const SellChunk* __restrict__ chunks,
                                                           const int* __restrict__ col,
-                                                          const double* __restrict__ val, double* __restrict__ x,
-                                                          long n, int nslot, const MpkSeg* __restrict__ segs,
-                                                          int nseg) {
+                                                          const VT* __restrict__ val, XT* __restrict__ x, long n,
+                                                          int nslot, const MpkSeg* __restrict__ segs, int nseg) {
     int s = 0;
     while (s + 1 < nseg && (int)blockIdx.x >= segs[s + 1].blockStart) ++s;
     const MpkSeg sg = segs[s];
@@ -232,41 +249,51 @@ __global__ void __launch_bounds__(kBlock) k_sell_mpk_step(const SellChunk* __res
     if (c >= sg.c1) return;
     const SellChunk k = chunks[c];
     if (lane >= k.rows) return;
-    const double* __restrict__ xin = x + (long)((sg.q - 1) % nslot) * n;
+    const XT* __restrict__ xin = x + (long)((sg.q - 1) % nslot) * n;
     x[(long)(sg.q % nslot) * n + k.row0 + lane] = sellRow(k, lane, col, val, xin);
 }
 
-// b = A x, one right-hand side
+// one launch of y = A x or y += A x over a single-range SELL matrix
+template <bool ACC, typename VT, typename XT>
+void launchSell(const SellMatrix<VT>& A, const void* x, void* y, int nrhs, hipStream_t st) {
+    const unsigned g = (unsigned)std::max(1, (A.nChunk + kWavesPerBlock - 1) / kWavesPerBlock);
+    if (A.uniformW >= 0)
+        hipLaunchKernelGGL((k_sell_spmv_u<ACC, VT, XT>), dim3(g), dim3(kBlock), 0, st, A.uniformW, A.col.p,
+                           A.val.p, (const XT*)x, (XT*)y, A.n, nrhs);
+    else
+        hipLaunchKernelGGL((k_sell_spmv<ACC, VT, XT>), dim3(g), dim3(kBlock), 0, st, A.chunks.p, A.nChunk,
+                           A.col.p, A.val.p, (const XT*)x, (XT*)y, (long)A.n, nrhs);
+}
+
+// b = A x
+template <typename VT, typename XT>
 class SellSpmvFn : public DeviceFn {
   public:
-    SellMatrix A;
+    SellMatrix<VT> A;
+    int nrhs = 1;
     int init(const race_kernel_desc& d) {
         kind = d.kind;
         x = d.x;
         b = d.b;
+        nrhs = d.nrhs > 0 ? d.nrhs : 1;
         return A.build(d, {0, d.nrow}, true);
     }
     int run(bool, hipStream_t st) override {
-        const unsigned g = (unsigned)std::max(1, (A.nChunk + kWavesPerBlock - 1) / kWavesPerBlock);
-        if (A.uniformW >= 0)
-            hipLaunchKernelGGL(k_sell_spmv_u<false>, dim3(g), dim3(kBlock), 0, st, A.uniformW, A.col.p, A.val.p,
-                               (const double*)x, (double*)b, A.n, 1);
-        else
-            hipLaunchKernelGGL(k_sell_spmv, dim3(g), dim3(kBlock), 0, st, A.chunks.p, A.nChunk, A.col.p, A.val.p,
-                               (const double*)x, (double*)b);
+        launchSell<false, VT, XT>(A, x, b, nrhs, st);
         RACE_HIP(hipGetLastError());
         return OK;
     }
 };
 
 // SymmSpMV b += A x from the upper triangle, in one launch:
-//  mode 2: SELL-C upper triangle, FP64 atomics for every write to b;
+//  mode 2: SELL-C upper triangle, FP64 atomics for every write to b (FP64 only);
 //  mode 3: the full matrix (upper triangle and its transpose) built once at
 //          registration, SELL-C SpMV accumulating into b (no atomics, reads
 //          the strict upper part twice).
+template <typename VT, typename XT>
 class SymmSellFn : public DeviceFn {
   public:
-    SellMatrix A;
+    SellMatrix<VT> A;
     int mode = 2, nrhs = 1;
     int init(const race_kernel_desc& d, int mode_) {
         kind = d.kind;
@@ -275,7 +302,10 @@ class SymmSellFn : public DeviceFn {
         mode = mode_;
         nrhs = d.nrhs > 0 ? d.nrhs : 1;
         const int n = d.nrow;
-        if (mode == 2) return A.build(d, {0, n});
+        if (mode == 2) {
+            if (!std::is_same<VT, double>::value || !std::is_same<XT, double>::value) return ERR_NOT_IMPLEMENTED;
+            return A.build(d, {0, n});
+        }
         // full pattern: row r holds its upper entries, then the transposed
         // entries (c, r) of rows c < r, each in increasing c
         std::vector<int> cnt(n + 1, 0);
@@ -307,16 +337,16 @@ class SymmSellFn : public DeviceFn {
         return A.build(f, {0, n}, true);
     }
     int run(bool, hipStream_t st) override {
-        const unsigned g = (unsigned)std::max(1, (A.nChunk + kWavesPerBlock - 1) / kWavesPerBlock);
-        if (mode == 2)
-            hipLaunchKernelGGL(k_sell_symm_atomic, dim3(g), dim3(kBlock), 0, st, A.chunks.p, A.nChunk, A.col.p,
-                               A.val.p, (const double*)x, (double*)b, (long)A.n, nrhs);
-        else if (A.uniformW >= 0)
-            hipLaunchKernelGGL(k_sell_spmv_u<true>, dim3(g), dim3(kBlock), 0, st, A.uniformW, A.col.p, A.val.p,
-                               (const double*)x, (double*)b, A.n, nrhs);
-        else
-            hipLaunchKernelGGL(k_sell_spmv_acc, dim3(g), dim3(kBlock), 0, st, A.chunks.p, A.nChunk, A.col.p,
-                               A.val.p, (const double*)x, (double*)b, (long)A.n, nrhs);
+        if constexpr (std::is_same<VT, double>::value && std::is_same<XT, double>::value) {
+            if (mode == 2) {
+                const unsigned g = (unsigned)std::max(1, (A.nChunk + kWavesPerBlock - 1) / kWavesPerBlock);
+                hipLaunchKernelGGL(k_sell_symm_atomic, dim3(g), dim3(kBlock), 0, st, A.chunks.p, A.nChunk,
+                                   A.col.p, A.val.p, (const double*)x, (double*)b, (long)A.n, nrhs);
+                RACE_HIP(hipGetLastError());
+                return OK;
+            }
+        }
+        launchSell<true, VT, XT>(A, x, b, nrhs, st);
         RACE_HIP(hipGetLastError());
         return OK;
     }
@@ -342,10 +372,12 @@ class SymmSellFn : public DeviceFn {
 // unless q' = q, so no segment reads what another writes.
 //
 // The step tables of every power 1..k are built at registration, so
-// setPower(p <= k) runs the first p powers.
+// setPower(p <= k) runs the first p powers.  With FP32 / BF16 values the
+// powers are FP32 vectors accumulated in FP32.
+template <typename VT, typename XT>
 class MpkFn : public DeviceFn {
   public:
-    SellMatrix A;
+    SellMatrix<VT> A;
     int nslot = 2;
     // for power p: segments stepSeg[p][L] .. stepSeg[p][L+1] of step L
     std::vector<std::vector<int>> stepSeg, stepBlocks;
@@ -396,8 +428,8 @@ class MpkFn : public DeviceFn {
         const std::vector<int>& nb = stepBlocks[power];
         for (int L = 0; L + 1 < (int)sg.size(); ++L) {
             if (nb[L] == 0) continue;
-            hipLaunchKernelGGL(k_sell_mpk_step, dim3(nb[L]), dim3(kBlock), 0, st, A.chunks.p, A.col.p, A.val.p,
-                               (double*)x, (long)A.n, nslot, segs.p + sg[L], sg[L + 1] - sg[L]);
+            hipLaunchKernelGGL((k_sell_mpk_step<VT, XT>), dim3(nb[L]), dim3(kBlock), 0, st, A.chunks.p, A.col.p,
+                               A.val.p, (XT*)x, (long)A.n, nslot, segs.p + sg[L], sg[L + 1] - sg[L]);
         }
         RACE_HIP(hipGetLastError());
         return OK;
@@ -411,9 +443,10 @@ class MpkFn : public DeviceFn {
 // over the SELL-C rows of A^T accumulating into b: no atomics, no colour
 // phases, every b entry written once.  Row c of A^T holds its entries in
 // increasing r, the order a sequential sweep adds them in.
+template <typename VT, typename XT>
 class SpmtvFn : public DeviceFn {
   public:
-    SellMatrix A;
+    SellMatrix<VT> A;
     int nrhs = 1;
     int init(const race_kernel_desc& d) {
         kind = d.kind;
@@ -439,13 +472,7 @@ class SpmtvFn : public DeviceFn {
         return A.build(t, {0, n}, true);
     }
     int run(bool, hipStream_t st) override {
-        const unsigned g = (unsigned)std::max(1, (A.nChunk + kWavesPerBlock - 1) / kWavesPerBlock);
-        if (A.uniformW >= 0)
-            hipLaunchKernelGGL(k_sell_spmv_u<true>, dim3(g), dim3(kBlock), 0, st, A.uniformW, A.col.p, A.val.p,
-                               (const double*)x, (double*)b, A.n, nrhs);
-        else
-            hipLaunchKernelGGL(k_sell_spmv_acc, dim3(g), dim3(kBlock), 0, st, A.chunks.p, A.nChunk, A.col.p,
-                               A.val.p, (const double*)x, (double*)b, (long)A.n, nrhs);
+        launchSell<true, VT, XT>(A, x, b, nrhs, st);
         RACE_HIP(hipGetLastError());
         return OK;
     }
@@ -664,32 +691,43 @@ int makeSellBatchFn(const std::vector<BatchItem>& items, const ExecContext& ctx,
     return e;
 }
 
-int makeSellSpmvFn(const race_kernel_desc& d, std::unique_ptr<DeviceFn>* out) {
-    auto f = std::make_unique<SellSpmvFn>();
-    int e = f->init(d);
-    *out = std::move(f);
+// the storage / arithmetic pairs of the single-matrix SELL functions:
+// FP64 / FP64, FP32 / FP32, BF16 / FP32
+template <template <typename, typename> class F, typename Init>
+int makeTyped(const race_kernel_desc& d, std::unique_ptr<DeviceFn>* out, Init&& init) {
+    int e;
+    if (d.val_precision == RACE_FP64 && d.vec_precision == RACE_FP64) {
+        auto f = std::make_unique<F<double, double>>();
+        e = init(*f);
+        *out = std::move(f);
+    } else if (d.val_precision == RACE_FP32 && d.vec_precision == RACE_FP32) {
+        auto f = std::make_unique<F<float, float>>();
+        e = init(*f);
+        *out = std::move(f);
+    } else if (d.val_precision == RACE_BF16 && d.vec_precision == RACE_FP32) {
+        auto f = std::make_unique<F<uint16_t, float>>();
+        e = init(*f);
+        *out = std::move(f);
+    } else {
+        return ERR_NOT_IMPLEMENTED;
+    }
     return e;
+}
+
+int makeSellSpmvFn(const race_kernel_desc& d, std::unique_ptr<DeviceFn>* out) {
+    return makeTyped<SellSpmvFn>(d, out, [&](auto& f) { return f.init(d); });
 }
 
 int makeSymmSellFn(const race_kernel_desc& d, int mode, std::unique_ptr<DeviceFn>* out) {
-    auto f = std::make_unique<SymmSellFn>();
-    int e = f->init(d, mode);
-    *out = std::move(f);
-    return e;
+    return makeTyped<SymmSellFn>(d, out, [&](auto& f) { return f.init(d, mode); });
 }
 
 int makeSpmtvFn(const race_kernel_desc& d, std::unique_ptr<DeviceFn>* out) {
-    auto f = std::make_unique<SpmtvFn>();
-    int e = f->init(d);
-    *out = std::move(f);
-    return e;
+    return makeTyped<SpmtvFn>(d, out, [&](auto& f) { return f.init(d); });
 }
 
 int makeMpkFn(const race_kernel_desc& d, const ExecContext& ctx, std::unique_ptr<DeviceFn>* out) {
-    auto f = std::make_unique<MpkFn>();
-    int e = f->init(d, ctx);
-    *out = std::move(f);
-    return e;
+    return makeTyped<MpkFn>(d, out, [&](auto& f) { return f.init(d, ctx); });
 }
 
 }  // namespace race
