@@ -76,12 +76,47 @@ def bench_symmspmv(R, M, g=64, threads=64, reps=200):
         ce.executeAsync(fid)
         cold.append(ce.timerEnd())
     cold = float(np.median(cold))
+    db.upload(np.zeros(n))
+    ce.executeFunction(fid)
+    ref = db.download()
+    # FP32 / BF16 values with FP32 vectors and accumulation: same calls,
+    # bytes with the smaller values and vectors, error against the FP64 result
+    low = {}
+    for name, vp, sv in (("fp32", R.FP32, 4), ("bf16", R.BF16, 2)):
+        lx = R.DeviceArray.from_host(dx.download().astype(np.float32))
+        lb = R.DeviceArray.from_host(np.zeros(n, np.float32))
+        lid = ce.registerDevice(R.K_SYMMSPMV, urp, ucol, uval, lx, lb, val_precision=vp, vec_precision=R.FP32)
+        for _ in range(5):
+            ce.executeAsync(lid)
+        lb.upload(np.zeros(n, np.float32))
+        ce.executeFunction(lid)
+        err = float(np.linalg.norm(lb.download() - ref) / np.linalg.norm(ref))
+        ce.timerBegin()
+        for _ in range(reps):
+            ce.executeAsync(lid)
+        lh = ce.timerEnd() / reps
+        lc = []
+        for _ in range(20):
+            R.interface.L.check(R.interface.L.load().race_device_copy_async(fb.ptr, fa.ptr, fa.nbytes, st), "copy")
+            ce.timerBegin()
+            ce.executeAsync(lid)
+            lc.append(ce.timerEnd())
+        lc = float(np.median(lc))
+        lbyts = nnzu * (sv + 4) + n * (4 + 3 * 4) + 4
+        low[name] = {"bytes_per_call": lbyts, "hot_us": round(lh * 1e3, 2), "hot_gbs": round(lbyts / lh / 1e6, 1),
+                     "cold_us": round(lc * 1e3, 2), "cold_gbs": round(lbyts / lc / 1e6, 1),
+                     "speedup_hot_vs_fp64": round(hot / lh, 3), "speedup_cold_vs_fp64": round(cold / lc, 3),
+                     "rel_err_vs_fp64": err}
+        log(f"SymmSpMV {g}^3 {name}: hot {low[name]['hot_us']} us, cold {low[name]['cold_us']} us, "
+            f"err {err:.1e}")
+        del lx, lb
     out = {"workload": f"SymmSpMV 7-pt {g}^3 (n={n}, nnz_upper={nnzu}), distance-2 colouring, "
                        f"{threads} RACE threads, FP64",
            "bytes_per_call": byts, "hot_us": round(hot * 1e3, 2), "hot_gbs": round(byts / hot / 1e6, 1),
            "cold_us": round(cold * 1e3, 2), "cold_gbs": round(byts / cold / 1e6, 1),
            "cold_method": "1 GiB device copy enqueued right before each timed call",
-           "cold_frac_hbm": round(byts / cold / 1e6 / HBM_PEAK_GBS, 4)}
+           "cold_frac_hbm": round(byts / cold / 1e6 / HBM_PEAK_GBS, 4),
+           "low_precision": low}
     log(f"SymmSpMV {g}^3: hot {out['hot_us']} us ({out['hot_gbs']} GB/s), cold {out['cold_us']} us "
         f"({out['cold_gbs']} GB/s)")
     return out
@@ -125,6 +160,30 @@ def bench_mpk(R, M, g=256, k=8, cache_mb=512.0, reps=10):
         ce.executeAsync(hid)
     msh = ce.timerEnd() / reps
     del dx2
+    # FP32 / BF16 values, FP32 powers and accumulation: time and the error
+    # of every power against the FP64 path (itself within 1e-12 of the oracle)
+    ref = dx.download().reshape(k + 1, n)
+    low = {}
+    for name, vp, sv in (("fp32", R.FP32, 4), ("bf16", R.BF16, 2)):
+        xl = np.zeros((k + 1) * n, np.float32)
+        xl[:n] = ref[0]
+        dl = R.DeviceArray.from_host(xl)
+        del xl
+        lid = ce.registerDevice(R.K_MPK, prp, pcol, pval, dl, power=k, val_precision=vp, vec_precision=R.FP32)
+        ce.executeFunction(lid)
+        got = dl.download().reshape(k + 1, n)
+        errs = [float(np.linalg.norm(got[p] - ref[p]) / np.linalg.norm(ref[p])) for p in range(1, k + 1)]
+        del got
+        ce.timerBegin()
+        for _ in range(reps):
+            ce.executeAsync(lid)
+        ml = ce.timerEnd() / reps
+        leff = k * (nnz * (sv + 4) + n * (4 + 2 * 4))
+        low[name] = {"ms_per_call": round(ml, 4), "effective_gbs": round(leff / ml / 1e6, 1),
+                     "speedup_vs_fp64": round(ms / ml, 3), "rel_err_per_power_vs_fp64": errs}
+        log(f"MPK {g}^3 k={k} {name}: {ml:.3f} ms/call ({low[name]['effective_gbs']} GB/s), "
+            f"error power 1 / {k}: {errs[0]:.1e} / {errs[-1]:.1e}")
+        del dl
     # the same k products as k separate SpMVs, for comparison
     dy = R.DeviceArray(n)
     sid = ce.registerDevice(R.K_SPMV, prp, pcol, pval, dx, dy)
@@ -140,7 +199,8 @@ def bench_mpk(R, M, g=256, k=8, cache_mb=512.0, reps=10):
            "compulsory_gbs": round(floor / ms / 1e6, 1),
            "only_highest_ms": round(msh, 4), "only_highest_effective_gbs": round(eff / msh / 1e6, 1),
            "spmv_ms": round(ms1, 4),
-           "spmv_gbs": round((nnz * 12 + n * 20) / ms1 / 1e6, 1), "speedup_vs_k_spmv": round(k * ms1 / ms, 3)}
+           "spmv_gbs": round((nnz * 12 + n * 20) / ms1 / 1e6, 1), "speedup_vs_k_spmv": round(k * ms1 / ms, 3),
+           "low_precision": low}
     log(f"MPK {g}^3 k={k}: {ms:.3f} ms/call, effective {out['effective_gbs']} GB/s; "
         f"only-highest {msh:.3f} ms; SpMV {ms1:.3f} ms ({out['spmv_gbs']} GB/s)")
     return out
