@@ -121,7 +121,8 @@ typedef struct {
     const int *col;
     const double *val;
     int val_precision;   /* storage precision of the matrix values on the device */
-    int vec_precision;   /* RACE_FP64 or RACE_FP32 for x and b */
+    int vec_precision;   /* RACE_FP64 or RACE_FP32 for x and b; pairs: FP64/FP64,
+                            FP32/FP32, BF16/FP32 (FP32 accumulation) */
     int nrhs;            /* right-hand sides, vectors are column-major v[r + k*nrow] */
     void *x;             /* device pointer */
     void *b;             /* device pointer (unused for MPK) */
