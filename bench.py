@@ -239,13 +239,19 @@ def bench_batched(args, R, M):
     coloured by RACE (distance 2); one step = SymmSpMV + args.batch_sweeps
     Kaczmarz sweeps (alternating forward / backward) on all of them, FP32
     values and vectors, FP64 row norms, 8 right-hand sides.  Layouts are
-    tuned on the first 64 matrices and applied to the whole batch."""
+    tuned on the 64 matrices with the most entries (a fused launch lasts as
+    long as its largest matrices' chains) and applied to the whole batch;
+    fused against per-matrix launches is compared on the first 64."""
     nrhs, sweeps, count = 8, args.batch_sweeps, args.batch_count
     tune_n = min(64, count)
+    sizes = [(k, n * mean) for k, (kd, n, mean) in enumerate(M.batch_sizes(count, args.batch_seed))
+             if args.batch_kinds in ("both", kd)]
+    tune_set = {k for k, _ in sorted(sizes, key=lambda t: -t[1])[:tune_n]}
     kacz = R.Batch(R.K_KACZ, R.FP32, R.FP32, nrhs)
     symm = R.Batch(R.K_SYMMSPMV, R.FP32, R.FP32, nrhs)
     tk = R.Batch(R.K_KACZ, R.FP32, R.FP32, nrhs)
     ts = R.Batch(R.K_SYMMSPMV, R.FP32, R.FP32, nrhs)
+    fk = R.Batch(R.K_KACZ, R.FP32, R.FP32, nrhs)   # the first tune_n matrices
     bytes_symm = bytes_kacz = 0
     tot_n = tot_nnz = 0
     kinds = {}
@@ -254,6 +260,7 @@ def bench_batched(args, R, M):
     t0 = time.time()
     gen = ((kd, n, mean, rp, col, val) for kd, n, mean, rp, col, val in M.batch_matrices(count, args.batch_seed)
            if args.batch_kinds in ("both", kd))
+    kept = [k for k, _ in sizes]
     for k, (kind, n, mean, rp, col, val) in enumerate(gen):
         T = max(4, min(1024, n // args.batch_rows_per_thread))
         ce = R.Interface(n, T, R.TWO, rp, col, symm_hint=True)
@@ -269,9 +276,11 @@ def bench_batched(args, R, M):
         sx, sb = R.DeviceArray.from_host(xvec), R.DeviceArray.from_host(np.zeros(nrhs * n, np.float32))
         kacz.add(ce, prp, pcol, pval, dx, db)
         symm.add(ce, urp, ucol, uval, sx, sb)
-        if k < tune_n:
+        if kept[k] in tune_set:
             tk.add(ce, prp, pcol, pval, dx, db)
             ts.add(ce, urp, ucol, uval, sx, sb)
+        if k < tune_n:
+            fk.add(ce, prp, pcol, pval, dx, db)
             singles.append((ce, prp, pcol, pval, n))
         if k < 8:
             cpu_sample.append((ce, prp, pcol, pval, n, bvec.astype(np.float64)))
@@ -290,8 +299,10 @@ def bench_batched(args, R, M):
     bs, times_s = ts.autotune(False, reps=3)
     log(f"tuned on {tune_n} matrices in {time.time() - t0:.1f}s: KACZ layout {bk} "
         f"{R.Batch.candidate(R.K_KACZ, bk)}, SymmSpMV layout {bs} {R.Batch.candidate(R.K_SYMMSPMV, bs)}")
-    # batched against per-matrix launches on the tuning sample (forward sweep)
-    t_batch_sample = tk.time(False, reps=3)
+    # batched against per-matrix launches on the first tune_n matrices
+    # (forward sweep, tuned layout)
+    fk.useCandidate(bk)
+    t_batch_sample = fk.time(False, reps=3)
     t_single = 0.0
     for ce, prp, pcol, pval, n in singles:
         wx = R.DeviceArray(nrhs * n, np.float32)
@@ -303,7 +314,7 @@ def bench_batched(args, R, M):
         for _ in range(3):
             ce.executeAsync(fid)
         t_single += ce.timerEnd() / 3
-    del tk, ts
+    del tk, ts, fk
     t0 = time.time()
     kacz.useCandidate(bk)
     symm.useCandidate(bs)

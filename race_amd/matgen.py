@@ -267,6 +267,19 @@ def _dominant_values(rp, col, seed):
     return np.where(col == r, 1.0 + s[r], off)
 
 
+def batch_sizes(count, seed, n_lo=2000, n_hi=200000):
+    """(kind, n, mean) of batch_matrices(count, seed) without building them
+    (the same random draws)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        n = int(np.exp(rng.uniform(np.log(n_lo), np.log(n_hi))))
+        mean = float(rng.uniform(5, 40))
+        rng.integers(1 << 30)
+        out.append(("banded" if k % 2 == 0 else "chung-lu", n, mean))
+    return out
+
+
 def batch_matrices(count, seed, n_lo=2000, n_hi=200000):
     """The batched workload: count matrices, n log-uniform in [n_lo, n_hi],
     even indices random banded (band 1 % of n), odd indices Chung-Lu
