@@ -8,6 +8,10 @@ matrix is the relative log-time log(t_i / t_best) of every candidate i
 (masked where a layout does not fit); the prediction is the candidate with
 the smallest predicted value.
 
+For the matrix powers (kind "mpk", k = 4) the candidates are the cache
+budgets MPK_CACHE_MB handed to RACEColor(4, budget): the MPK block depth
+(level groups per wavefront step), which sets the number of launches.
+
     python -m race_amd.tuner sweep  --kind symmspmv --count 1200 --out data.npz
     python -m race_amd.tuner train  --data data.npz --out model.npz
     python -m race_amd.tuner score  --data data.npz --model model.npz
@@ -22,10 +26,16 @@ import numpy as np
 
 from . import _lib as L
 from . import matgen as M
-from .interface import BF16, FP32, FP64, K_GS, K_KACZ, K_SYMMSPMV, ONE, TWO, Batch, DeviceArray, Interface
+from .interface import (BF16, FP32, FP64, K_GS, K_KACZ, K_MPK, K_SYMMSPMV, ONE, POWER, TWO, Batch, DeviceArray,
+                        Interface)
 
 N_FEATURES, N_CANDIDATES = 16, 12
-KINDS = {"symmspmv": K_SYMMSPMV, "gs": K_GS, "kacz": K_KACZ}
+KINDS = {"symmspmv": K_SYMMSPMV, "gs": K_GS, "kacz": K_KACZ, "mpk": K_MPK}
+# MPK candidates: cache budgets in MB; the last is the Infinity Cache size,
+# the fixed choice the tuner is compared with
+MPK_CACHE_MB = [0.125, 0.25, 0.5, 1, 2, 4, 8, 16, 32, 64, 128, 256]
+MPK_POWER = 4
+DEFAULT_CANDIDATE = {"symmspmv": 0, "gs": 0, "kacz": 0, "mpk": len(MPK_CACHE_MB) - 1}
 
 
 def _fp(a):
@@ -154,10 +164,44 @@ def tuner_matrices(count, seed, n_lo=2000, n_hi=100000):
             yield name, rp, col, val
 
 
+def sweep_mpk(rp, col, val, reps=3, stats=None):
+    """(features, times[12]) of the matrix powers, k = MPK_POWER, FP64, for
+    every cache budget of MPK_CACHE_MB; -1 where RACEColor refuses one."""
+    n = len(rp) - 1
+    x0 = M.uniform_vec(n, 7)
+    times = np.full(N_CANDIDATES, -1.0)
+    t0 = time.time()
+    for i, mb in enumerate(MPK_CACHE_MB):
+        ce = Interface(n, 1, POWER, rp, col, symm_hint=True)
+        try:
+            ce.RACEColor(MPK_POWER, mb * 1e6)
+        except Exception:
+            continue
+        prp, pcol, pval = M.permute(rp, col, val, ce.getPerm(), ce.getInvPerm())
+        xs = np.zeros((MPK_POWER + 1) * n)
+        xs[:n] = x0
+        dx = DeviceArray.from_host(xs)
+        fid = ce.registerDevice(K_MPK, prp, pcol, pval, dx, power=MPK_POWER)
+        ce.executeAsync(fid)
+        ce.timerBegin()
+        for _ in range(reps):
+            ce.executeAsync(fid)
+        times[i] = ce.timerEnd() / reps
+        ce.close()
+    t1 = time.time()
+    f = features(rp, col)
+    if stats is not None:
+        stats["autotune"] = t1 - t0
+        stats["features"] = time.time() - t1
+    return f, times
+
+
 def sweep_one(kind, rp, col, val, reps=3, stats=None):
     """(features, times[12]) of one matrix; times -1 where a layout does not
     fit.  Matrix coloured by RACE with max(4, n / 256) threads.  stats, if
     given, receives the seconds spent on features and on timing layouts."""
+    if kind == K_MPK:
+        return sweep_mpk(rp, col, val, reps, stats)
     n = len(rp) - 1
     T = max(4, min(1024, n // 256))
     dist = ONE if kind == K_GS else TWO
@@ -188,14 +232,18 @@ def sweep_one(kind, rp, col, val, reps=3, stats=None):
     return f, times
 
 
-def targets(times):
-    """log(t_i / t_best) per matrix and the mask of the layouts that fit."""
+def targets(times, penalty=None):
+    """log(t_i / t_best) per matrix and the mask of the layouts that fit.
+    With a penalty, a candidate that does not fit gets that target instead
+    of being masked, so the model learns to avoid it."""
     t = np.atleast_2d(np.asarray(times, np.float64))
     ok = t > 0
     best = np.where(ok, t, np.inf).min(axis=1, keepdims=True)
     best = np.where(np.isfinite(best), best, 1.0)
     y = np.where(ok, np.log(np.where(ok, t, 1.0) / best), 0.0)
-    return y, ok.astype(np.float64)
+    if penalty is None:
+        return y, ok.astype(np.float64)
+    return np.where(ok, y, penalty), np.ones_like(y)
 
 
 class Tuner:
@@ -209,11 +257,11 @@ class Tuner:
     def _norm(self, F):
         return ((np.asarray(F, np.float64) - self.mu) / self.sd).astype(np.float32)
 
-    def fit(self, F, times, epochs=3000, lr=3e-3):
+    def fit(self, F, times, epochs=3000, lr=3e-3, penalty=None):
         F = np.asarray(F, np.float64)
         self.mu = F.mean(axis=0)
         self.sd = F.std(axis=0) + 1e-9
-        y, m = targets(times)
+        y, m = targets(times, penalty)
         return self.mlp.train(self._norm(F), y, m, epochs, lr)
 
     def predict(self, F, precision=BF16):
@@ -222,8 +270,8 @@ class Tuner:
 
     @classmethod
     def pretrained(cls, kind):
-        """The model shipped in race_amd/tuner_data for "symmspmv" or "gs"
-        (trained on the sweeps stored beside it)."""
+        """The model shipped in race_amd/tuner_data for "symmspmv", "gs" or
+        "mpk" (trained on the sweeps stored beside it)."""
         path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuner_data", f"{kind}_model.npz")
         return cls(0).load(path)
 
@@ -260,6 +308,8 @@ def _main(argv=None):
     t.add_argument("--data", required=True)
     t.add_argument("--holdout", type=int, default=200)
     t.add_argument("--epochs", type=int, default=3000)
+    t.add_argument("--penalty", type=float, default=None,
+                   help="target for candidates that do not fit (default: masked)")
     t.add_argument("--out", required=True)
     c = sub.add_parser("score")
     c.add_argument("--data", required=True)
@@ -288,7 +338,7 @@ def _main(argv=None):
         tr = slice(0, len(F) - a.holdout)
         tu = Tuner(0)
         t0 = time.time()
-        loss = tu.fit(F[tr], T[tr], epochs=a.epochs)
+        loss = tu.fit(F[tr], T[tr], epochs=a.epochs, penalty=a.penalty)
         print(f"trained on {len(F) - a.holdout} matrices, loss {loss:.4f}, {time.time() - t0:.1f}s", flush=True)
         tu.save(a.out)
     else:
@@ -302,7 +352,7 @@ def _main(argv=None):
         res = {"kind": str(z["kind"]), "heldout": int(len(F[ho])),
                "score_mlp_bf16": round(score(T[ho], pick), 4),
                "score_mlp_fp32": round(score(T[ho], tu.predict(F[ho], FP32)), 4),
-               "score_default_layout": round(score(T[ho], np.zeros(len(F[ho]), int)), 4),
+               "score_default_layout": round(score(T[ho], np.full(len(F[ho]), DEFAULT_CANDIDATE[str(z["kind"])])), 4),
                "predict_seconds": round(t_pred, 4),
                "autotune_seconds": round(float(z["sweep_seconds"][ho].sum()), 2)}
         if "feature_seconds" in z:

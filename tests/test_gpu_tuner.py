@@ -66,3 +66,25 @@ def test_sweep_and_predict():
     tu.fit(F, T, epochs=300)
     pick = tu.predict(F)
     assert pick.shape == (8,) and ((0 <= pick) & (pick < 12)).all()
+
+
+def test_pretrained_mpk_tuner_scores_on_held_out():
+    """The shipped MPK model (cache budget of RACEColor(4, budget)) on the
+    held-out tail of its sweep: within 10 % of the autotuned best, and never
+    a budget RACEColor refused."""
+    import os
+    z = np.load(os.path.join(os.path.dirname(TU.__file__), "tuner_data", "mpk_sweep.npz"))
+    F, T = z["features"], z["times"]
+    ho = slice(len(F) - 80, len(F))
+    tu = TU.Tuner.pretrained("mpk")
+    pick = tu.predict(F[ho])
+    assert (T[ho][np.arange(80), pick] > 0).all()
+    assert TU.score(T[ho], pick) >= 0.9
+
+
+def test_mpk_sweep_one_matrix():
+    rp, col = M.stencil7(10)
+    val = M.laplace7_values(rp, col)
+    f, t = TU.sweep_one(R.K_MPK, rp, col, val)
+    assert f.shape == (16,) and t.shape == (12,)
+    assert (t[-4:] > 0).all()
