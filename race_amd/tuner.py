@@ -11,6 +11,10 @@ the smallest predicted value.
 For the matrix powers (kind "mpk", k = 4) the candidates are the cache
 budgets MPK_CACHE_MB handed to RACEColor(4, budget): the MPK block depth
 (level groups per wavefront step), which sets the number of launches.
+For RACE's own preprocessing (kind "race") the candidates are the rows per
+RACE thread RACE_ROWS_PER_THREAD of a distance-1 colouring for a symmetric
+Gauss-Seidel step: the thread count sets the recursion depth (stages,
+colour phases) and the leaf sizes (dependency waves).
 
     python -m race_amd.tuner sweep  --kind symmspmv --count 1200 --out data.npz
     python -m race_amd.tuner train  --data data.npz --out model.npz
@@ -30,12 +34,16 @@ from .interface import (BF16, FP32, FP64, K_GS, K_KACZ, K_MPK, K_SYMMSPMV, ONE, 
                         Interface)
 
 N_FEATURES, N_CANDIDATES = 16, 12
-KINDS = {"symmspmv": K_SYMMSPMV, "gs": K_GS, "kacz": K_KACZ, "mpk": K_MPK}
+KINDS = {"symmspmv": K_SYMMSPMV, "gs": K_GS, "kacz": K_KACZ, "mpk": K_MPK, "race": -1}
 # MPK candidates: cache budgets in MB; the last is the Infinity Cache size,
 # the fixed choice the tuner is compared with
 MPK_CACHE_MB = [0.125, 0.25, 0.5, 1, 2, 4, 8, 16, 32, 64, 128, 256]
 MPK_POWER = 4
-DEFAULT_CANDIDATE = {"symmspmv": 0, "gs": 0, "kacz": 0, "mpk": len(MPK_CACHE_MB) - 1}
+# RACE threads = max(1, n // rows per thread); the fixed choice compared
+# with is 256 rows per thread (the batched workload's rule)
+RACE_ROWS_PER_THREAD = [2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096]
+DEFAULT_CANDIDATE = {"symmspmv": 0, "gs": 0, "kacz": 0, "mpk": len(MPK_CACHE_MB) - 1,
+                     "race": RACE_ROWS_PER_THREAD.index(256)}
 
 
 def _fp(a):
@@ -196,12 +204,46 @@ def sweep_mpk(rp, col, val, reps=3, stats=None):
     return f, times
 
 
+def sweep_race(rp, col, val, reps=3, stats=None):
+    """(features, times[12]) of a symmetric Gauss-Seidel step (forward +
+    backward sweep, FP64, default layout) for every RACE thread count
+    n // RACE_ROWS_PER_THREAD[i], colouring at distance 1."""
+    n = len(rp) - 1
+    b = M.uniform_vec(n, 8)
+    times = np.full(N_CANDIDATES, -1.0)
+    t0 = time.time()
+    for i, rpt in enumerate(RACE_ROWS_PER_THREAD):
+        ce = Interface(n, max(1, n // rpt), ONE, rp, col, symm_hint=True)
+        ce.RACEColor()
+        prp, pcol, pval = M.permute(rp, col, val, ce.getPerm(), ce.getInvPerm())
+        prp, pcol, pval = M.make_diag_first(prp, pcol, pval)
+        x, bd = DeviceArray.from_host(np.zeros(n)), DeviceArray.from_host(b)
+        fid = ce.registerDevice(K_GS, prp, pcol, pval, x, bd)
+        ce.executeAsync(fid)
+        ce.executeAsync(fid, True)
+        ce.timerBegin()
+        for _ in range(reps):
+            ce.executeAsync(fid)
+            ce.executeAsync(fid, True)
+        times[i] = ce.timerEnd() / reps
+        ce.close()
+    t1 = time.time()
+    f = features(rp, col)
+    if stats is not None:
+        stats["autotune"] = t1 - t0
+        stats["features"] = time.time() - t1
+    return f, times
+
+
 def sweep_one(kind, rp, col, val, reps=3, stats=None):
     """(features, times[12]) of one matrix; times -1 where a layout does not
     fit.  Matrix coloured by RACE with max(4, n / 256) threads.  stats, if
-    given, receives the seconds spent on features and on timing layouts."""
+    given, receives the seconds spent on features and on timing layouts.
+    kind K_MPK sweeps cache budgets, -1 RACE thread counts."""
     if kind == K_MPK:
         return sweep_mpk(rp, col, val, reps, stats)
+    if kind == -1:
+        return sweep_race(rp, col, val, reps, stats)
     n = len(rp) - 1
     T = max(4, min(1024, n // 256))
     dist = ONE if kind == K_GS else TWO
@@ -270,8 +312,8 @@ class Tuner:
 
     @classmethod
     def pretrained(cls, kind):
-        """The model shipped in race_amd/tuner_data for "symmspmv", "gs" or
-        "mpk" (trained on the sweeps stored beside it)."""
+        """The model shipped in race_amd/tuner_data for "symmspmv", "gs",
+        "mpk" or "race" (trained on the sweeps stored beside it)."""
         path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuner_data", f"{kind}_model.npz")
         return cls(0).load(path)
 

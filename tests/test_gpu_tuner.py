@@ -88,3 +88,18 @@ def test_mpk_sweep_one_matrix():
     f, t = TU.sweep_one(R.K_MPK, rp, col, val)
     assert f.shape == (16,) and t.shape == (12,)
     assert (t[-4:] > 0).all()
+
+
+def test_pretrained_race_threads_tuner_scores_on_held_out():
+    """The shipped RACE-threads model (rows per RACE thread for a symmetric
+    GS step) on the held-out tail of its sweep: well above the fixed
+    256-rows-per-thread rule."""
+    import os
+    z = np.load(os.path.join(os.path.dirname(TU.__file__), "tuner_data", "race_sweep.npz"))
+    F, T = z["features"], z["times"]
+    ho = slice(len(F) - 48, len(F))
+    tu = TU.Tuner.pretrained("race")
+    pick = tu.predict(F[ho])
+    s_mlp = TU.score(T[ho], pick)
+    s_fixed = TU.score(T[ho], np.full(48, TU.DEFAULT_CANDIDATE["race"]))
+    assert s_mlp >= 0.85 and s_mlp > s_fixed + 0.1
