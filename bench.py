@@ -418,6 +418,18 @@ def main():
     nnz = int(rp[-1])
     b = M.uniform_vec(n, 7)
     log(f"matrix 27-pt {g}^3: n={n} nnz={nnz} ({time.time() - t0:.1f}s)")
+    # what the learned tuner proposes for RACE's thread count on this matrix
+    # (its training matrices have at most ~100k rows: an extrapolation here)
+    race_tuner = None
+    try:
+        from race_amd import tuner as TU
+        t0 = time.time()
+        rpt = TU.RACE_ROWS_PER_THREAD[int(TU.Tuner.pretrained("race").predict(TU.features(rp, col))[0])]
+        race_tuner = {"proposed_rows_per_thread": rpt, "proposed_threads": max(1, n // rpt),
+                      "seconds": round(time.time() - t0, 2), "threads_used": args.threads}
+        log(f"tuner proposes {rpt} rows per RACE thread ({n // rpt} threads); measured best 32 (65536)")
+    except OSError:
+        pass
 
     t0 = time.time()
     ce = R.Interface(n, args.threads, R.ONE, rp, col, symm_hint=True)
@@ -549,6 +561,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(gbs, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 5), "traffic": traffic},
         "cpu_baseline": cpu,
+        "race_tuner": race_tuner,
         "mixed_precision": mixed,
         "other_kernels": extra,
     }
