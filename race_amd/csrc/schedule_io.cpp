@@ -111,8 +111,9 @@ bool treeOk(const std::vector<ZoneLeaf>& t, int n, int bpt) {
             if (z.valueZ[i] < 0 || z.valueZ[i] > n) return false;
             if (i && z.valueZ[i] < z.valueZ[i - 1]) return false;
         }
+        // a leaf without children runs its range once: one sub-block
         if (!z.children.empty() && (int)z.children.size() < 2 * z.totalSubBlocks) return false;
-        if (z.children.empty() && z.totalSubBlocks > 1 << 20) return false;
+        if (z.children.empty() && z.totalSubBlocks != 1) return false;
         for (size_t s = 0; s + 1 < z.children.size(); s += 2) {
             const int c0 = z.children[s], c1 = z.children[s + 1];
             if (c0 <= p || c1 < c0 || c1 > L) return false;
