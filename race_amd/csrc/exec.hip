@@ -265,6 +265,12 @@ class WaveFn : public DeviceFn {
     }
 
     bool graphable() const override { return true; }
+    int launches(bool rev) const override {
+        const std::vector<int>& pp = (rev ? wr : wf).h.phaseLeafPtr;
+        int k = 0;
+        for (size_t p = 0; p + 1 < pp.size(); ++p) k += pp[p + 1] > pp[p];
+        return k;
+    }
     int run(bool rev, hipStream_t st) override {
         const DevWaves& w = rev ? wr : wf;
         const std::vector<int>& pp = w.h.phaseLeafPtr;
@@ -312,6 +318,11 @@ class SymmSpmvFn : public DeviceFn {
         return OK;
     }
     bool graphable() const override { return true; }
+    int launches(bool) const override {
+        int k = 0;
+        for (size_t p = 0; p + 1 < phaseChunk.size(); ++p) k += phaseChunk[p + 1] > phaseChunk[p];
+        return k;
+    }
     int run(bool, hipStream_t st) override {
         for (size_t p = 0; p + 1 < phaseChunk.size(); ++p) {
             const int c0 = phaseChunk[p], nc = phaseChunk[p + 1] - phaseChunk[p];

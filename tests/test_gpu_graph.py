@@ -73,3 +73,33 @@ def test_graph_mpk_follows_set_power(oracle):
         got = dx.download().reshape(k + 1, n)
         for p in range(1, k + 1):
             assert np.linalg.norm(got[p] - ref[p]) <= 1e-12 * np.linalg.norm(ref[p]), p
+
+
+@pytest.mark.parametrize("what", ["symm_phases", "gs_leaf_workgroups"])
+def test_graph_other_phase_launchers(what):
+    """The colour-phase SymmSpMV (symm_mode 1) and the workgroup-per-leaf GS
+    layout (layout 1) replayed as graphs equal their direct launches."""
+    rp, col = M.stencil7(14)
+    val = M.gs_values(rp, col, seed=2)
+    n = len(rp) - 1
+    dist = R.TWO if what == "symm_phases" else R.ONE
+    a, g = _pair(rp, col, 32, dist)
+    for h in (a, g):
+        h.setOption("symm_mode" if what == "symm_phases" else "layout", 1)
+    prp, pcol, pval = M.make_diag_first(*M.permute(rp, col, val, a.getPerm(), a.getInvPerm()))
+    if what == "symm_phases":
+        prp, pcol, pval = M.upper_triangle(prp, pcol, pval)
+        kind, x0, b0 = R.K_SYMMSPMV, M.uniform_vec(n, 1), np.zeros(n)
+    else:
+        kind, x0, b0 = R.K_GS, np.zeros(n), M.uniform_vec(n, 1)
+    xa, ba = R.DeviceArray.from_host(x0), R.DeviceArray.from_host(b0)
+    xg, bg = R.DeviceArray.from_host(x0), R.DeviceArray.from_host(b0)
+    fa = a.registerDevice(kind, prp, pcol, pval, xa, ba)
+    fg = g.registerDevice(kind, prp, pcol, pval, xg, bg)
+    assert g.launchCount(fg) > 1
+    for it in range(3):
+        a.executeFunction(fa)
+        g.executeFunction(fg)
+        out_a = (ba if what == "symm_phases" else xa).download()
+        out_g = (bg if what == "symm_phases" else xg).download()
+        assert np.array_equal(out_g, out_a), it
