@@ -83,6 +83,15 @@ int makeWaveFn(const race_kernel_desc& d, const ExecContext& ctx, std::unique_pt
 // SELL-C SpMV (one right-hand side) and level-blocked matrix powers (sell.hip)
 int makeSellSpmvFn(const race_kernel_desc& d, std::unique_ptr<DeviceFn>* out);
 int makeMpkFn(const race_kernel_desc& d, const ExecContext& ctx, std::unique_ptr<DeviceFn>* out);
+// r = b - A x in FP64 with the sums of r^2 and b^2 added to norms[0..1]
+// (the refinement's residual, over the SELL layout: sell.hip)
+class ResidualOp {
+  public:
+    virtual ~ResidualOp() {}
+    virtual int run(const double* x, const double* b, void* r, bool rFp32, int nrhs, double* norms,
+                    hipStream_t st) = 0;
+};
+int makeSellResidual(const race_kernel_desc& d, std::unique_ptr<ResidualOp>* out);
 // SpMTV b += A^T x over the transpose formed at registration (sell.hip)
 int makeSpmtvFn(const race_kernel_desc& d, std::unique_ptr<DeviceFn>* out);
 // single-launch SymmSpMV: mode 2 SELL atomics, mode 3 expanded full matrix (sell.hip)

@@ -1,6 +1,7 @@
 // refine.hip -- FP64 iterative refinement around a low-precision sweep.
 //
-// One refinement step: r = b - A x in FP64 (and ||r||, ||b|| by reduction),
+// One refinement step: r = b - A x in FP64 over the SELL layout (and ||r||,
+// ||b|| by reduction),
 // d = 0, `sweeps` forward+backward sweeps of the inner Gauss-Seidel or
 // Kaczmarz function on A d = r in the inner function's precision, x += d in
 // FP64.  FP64 is used only for the residual and the update (and the
@@ -20,50 +21,6 @@ namespace race {
 
 namespace {
 
-template <typename T, int TPR>
-__global__ void __launch_bounds__(kBlock) k_residual(const int* __restrict__ rp,
-                                                     const int* __restrict__ col,
-                                                     const double* __restrict__ val,
-                                                     const double* __restrict__ x,
-                                                     const double* __restrict__ b, T* __restrict__ r,
-                                                     int n, int nrhs, double* __restrict__ norms) {
-    __shared__ double sr[kBlock / TPR], sb[kBlock / TPR];
-    const int lane = threadIdx.x % TPR, grp = threadIdx.x / TPR;
-    const int rowsPerPass = gridDim.x * (blockDim.x / TPR);
-    double rr = 0.0, bb = 0.0;
-    // grid-stride over rows: a bounded grid keeps the norm atomics few
-    for (int row = (blockIdx.x * blockDim.x + threadIdx.x) / TPR; row < n; row += rowsPerPass) {
-        const int s = rp[row], e = rp[row + 1];
-        for (int k = 0; k < nrhs; ++k) {
-            const size_t o = (size_t)k * n;
-            double t = 0.0;
-            for (int j = s + lane; j < e; j += TPR) t += val[j] * x[o + col[j]];
-            t = groupSum<TPR>(t);
-            const double bv = b[o + row];
-            const double rv = bv - t;
-            if (lane == 0) {
-                r[o + row] = (T)rv;
-                rr += rv * rv;
-                bb += bv * bv;
-            }
-        }
-    }
-    if (lane == 0) {
-        sr[grp] = rr;
-        sb[grp] = bb;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double a = 0.0, c = 0.0;
-        for (int g = 0; g < kBlock / TPR; ++g) {
-            a += sr[g];
-            c += sb[g];
-        }
-        atomicAdd(&norms[0], a);
-        atomicAdd(&norms[1], c);
-    }
-}
-
 template <typename T>
 __global__ void k_zero(T* __restrict__ v, long n) {
     long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -80,10 +37,9 @@ __global__ void k_update(double* __restrict__ x, const T* __restrict__ d, long n
 
 class RefineFn : public DeviceFn {
   public:
-    int n = 0, nrhs = 1, tpr = 8, sweeps = 1;
-    long nnz = 0;
-    DevBuf<int> rp, col;
-    DevBuf<double> val, norms, r64, d64;
+    int n = 0, nrhs = 1, sweeps = 1;
+    std::unique_ptr<ResidualOp> res;
+    DevBuf<double> norms, r64, d64;
     DevBuf<float> r32, d32;
     DeviceFn* inner = nullptr;
     double* hostNorms = nullptr;
@@ -105,12 +61,8 @@ class RefineFn : public DeviceFn {
         nrhs = d.nrhs > 0 ? d.nrhs : 1;
         x = d.x;
         b = d.b;
-        nnz = d.rowPtr[n];
-        tpr = pickTpr(nnz, n);
         int e;
-        if ((e = upload(rp, d.rowPtr, (size_t)n + 1))) return e;
-        if ((e = upload(col, d.col, (size_t)nnz))) return e;
-        if ((e = upload(val, d.val, (size_t)nnz))) return e;
+        if ((e = makeSellResidual(d, &res))) return e;
         RACE_HIP(norms.reserve(2));
         const size_t len = (size_t)n * nrhs;
         if (inner->vecPrecision == RACE_FP32) {
@@ -128,27 +80,13 @@ class RefineFn : public DeviceFn {
         return OK;
     }
 
-    template <int TPR>
-    void residualKernel(hipStream_t st) {
-        const long threads = std::min((long)n * TPR, 4096l * kBlock);
-        if (inner->vecPrecision == RACE_FP32)
-            hipLaunchKernelGGL((k_residual<float, TPR>), dim3(gridFor(threads)), dim3(kBlock), 0, st, rp.p,
-                               col.p, val.p, (const double*)x, (const double*)b, r32.p, n, nrhs, norms.p);
-        else
-            hipLaunchKernelGGL((k_residual<double, TPR>), dim3(gridFor(threads)), dim3(kBlock), 0, st, rp.p,
-                               col.p, val.p, (const double*)x, (const double*)b, r64.p, n, nrhs, norms.p);
-    }
-
     // r = b - A x; relres = ||r|| / ||b|| when requested (synchronises)
     int residual(hipStream_t st, double* relres) {
         hipLaunchKernelGGL(k_zero<double>, dim3(1), dim3(64), 0, st, norms.p, 2l);
-        switch (tpr) {
-            case 4: residualKernel<4>(st); break;
-            case 8: residualKernel<8>(st); break;
-            case 16: residualKernel<16>(st); break;
-            default: residualKernel<32>(st); break;
-        }
-        RACE_HIP(hipGetLastError());
+        const bool f32 = inner->vecPrecision == RACE_FP32;
+        int e = res->run((const double*)x, (const double*)b, f32 ? (void*)r32.p : (void*)r64.p, f32, nrhs,
+                         norms.p, st);
+        if (e) return e;
         if (relres) {
             RACE_HIP(hipMemcpyAsync(hostNorms, norms.p, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
             RACE_HIP(hipStreamSynchronize(st));

@@ -478,6 +478,70 @@ class SpmtvFn : public DeviceFn {
     }
 };
 
+// r = b - A x (FP64 product, r stored as RT) and the sums of r^2 and b^2
+// for the refinement; grid-stride over chunks so few workgroups add to the
+// two norm accumulators
+template <typename RT>
+__global__ void __launch_bounds__(kBlock) k_sell_residual(const SellChunk* __restrict__ chunks, int nChunk,
+                                                          int W, const int* __restrict__ col,
+                                                          const double* __restrict__ val,
+                                                          const double* __restrict__ x,
+                                                          const double* __restrict__ b, RT* __restrict__ r,
+                                                          long n, int nrhs, double* __restrict__ norms) {
+    __shared__ double sr[kWavesPerBlock], sb[kWavesPerBlock];
+    const int lane = threadIdx.x % kC, wv = threadIdx.x / kC;
+    double rr = 0.0, bb = 0.0;
+    for (int c = blockIdx.x * kWavesPerBlock + wv; c < nChunk; c += gridDim.x * kWavesPerBlock) {
+        const SellChunk k = W >= 0 ? SellChunk{c * kC, (int)std::min<long>(kC, n - (long)c * kC), W, 0,
+                                               (long)c * kC * W}
+                                   : chunks[c];
+        if (lane >= k.rows) continue;
+        const long row = k.row0 + lane;
+        for (int q = 0; q < nrhs; ++q) {
+            const double t = sellRow(k, lane, col, val, x + q * n);
+            const double bv = b[q * n + row];
+            const double rv = bv - t;
+            r[q * n + row] = (RT)rv;
+            rr += rv * rv;
+            bb += bv * bv;
+        }
+    }
+    rr = groupSum<kC>(rr);
+    bb = groupSum<kC>(bb);
+    if (lane == 0) {
+        sr[wv] = rr;
+        sb[wv] = bb;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0.0, e = 0.0;
+        for (int w = 0; w < kWavesPerBlock; ++w) {
+            a += sr[w];
+            e += sb[w];
+        }
+        atomicAdd(&norms[0], a);
+        atomicAdd(&norms[1], e);
+    }
+}
+
+class SellResidual : public ResidualOp {
+  public:
+    SellMatrix<double> A;
+    int init(const race_kernel_desc& d) { return A.build(d, {0, d.nrow}, true); }
+    int run(const double* x, const double* b, void* r, bool rFp32, int nrhs, double* norms,
+            hipStream_t st) override {
+        const int blocks = std::max(1, std::min(4096, (A.nChunk + kWavesPerBlock - 1) / kWavesPerBlock));
+        if (rFp32)
+            hipLaunchKernelGGL(k_sell_residual<float>, dim3(blocks), dim3(kBlock), 0, st, A.chunks.p, A.nChunk,
+                               A.uniformW, A.col.p, A.val.p, x, b, (float*)r, (long)A.n, nrhs, norms);
+        else
+            hipLaunchKernelGGL(k_sell_residual<double>, dim3(blocks), dim3(kBlock), 0, st, A.chunks.p,
+                               A.nChunk, A.uniformW, A.col.p, A.val.p, x, b, (double*)r, (long)A.n, nrhs, norms);
+        RACE_HIP(hipGetLastError());
+        return OK;
+    }
+};
+
 // ---------------------------------------------------------------------------
 // Generalised SELL-C-sigma for batches and the layout tuner: chunk height
 // C in {8, 16, 32, 64} (a wavefront holds 64 / C chunks), rows sorted by
@@ -711,6 +775,13 @@ int makeTyped(const race_kernel_desc& d, std::unique_ptr<DeviceFn>* out, Init&& 
     } else {
         return ERR_NOT_IMPLEMENTED;
     }
+    return e;
+}
+
+int makeSellResidual(const race_kernel_desc& d, std::unique_ptr<ResidualOp>* out) {
+    auto f = std::make_unique<SellResidual>();
+    int e = f->init(d);
+    *out = std::move(f);
     return e;
 }
 
