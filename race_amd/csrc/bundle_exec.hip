@@ -24,18 +24,33 @@ namespace race {
 
 namespace {
 
-// matrix value storage: double, float, or bfloat16 bits (uint16_t)
+// matrix value storage: double, float, or bfloat16 bits (uint16_t).  The
+// prologue keeps a value as loaded (ValBits) and converts it where it is
+// used.  A bfloat16 shift placed right after its conditional load makes the
+// wave wait for that load before it issues the next entry's loads (13
+// round trips in a row); the empty asm statement, which emits no
+// instruction, keeps the compiler from folding the shift back into the
+// conditional load.
+template <typename VT>
+struct ValBits {
+    using T = VT;
+};
+template <>
+struct ValBits<uint16_t> {
+    using T = unsigned;
+};
 template <typename XT>
-__device__ __forceinline__ XT loadVal(const double* p) {
-    return (XT)*p;
+__device__ __forceinline__ XT valOf(double v) {
+    return (XT)v;
 }
 template <typename XT>
-__device__ __forceinline__ XT loadVal(const float* p) {
-    return (XT)*p;
+__device__ __forceinline__ XT valOf(float v) {
+    return (XT)v;
 }
 template <typename XT>
-__device__ __forceinline__ XT loadVal(const uint16_t* p) {
-    return (XT)__uint_as_float((unsigned)*p << 16);
+__device__ __forceinline__ XT valOf(unsigned v) {
+    asm volatile("" : "+v"(v));
+    return (XT)__uint_as_float(v << 16);
 }
 
 template <int TPR, typename T>
@@ -79,15 +94,17 @@ struct BundleVec {
 
 // The prologue has three dependent memory levels: the bundle's offsets;
 // then, all issued together, its entry block, row ids, inverse diagonals,
-// wave numbers and outside-column ids; then the gathers of x and b.  A
-// phase whose chains are single bundles (bundle0 >= 0) skips the chain
-// table.
+// wave numbers and outside-column ids; then the gathers of x and b.  Row
+// and column ids are unsigned: widening them for the 64-bit address is then
+// a zero high half, which does not wait for the load as a sign extension
+// inside the conditional load would.  A phase whose chains are single
+// bundles (bundle0 >= 0) skips the chain table.
 template <int BS, int TPR, int E, typename VT, typename XT, bool ONE_RHS, bool BATCH>
 __global__ void __launch_bounds__(BS) k_gs_bundle(
     const int* __restrict__ bSlot, const int* __restrict__ bWave, const long* __restrict__ bOff,
     const int* __restrict__ bLen, const int* __restrict__ bExt, const uint16_t* __restrict__ slotWave,
     const int* __restrict__ rowId, const XT* __restrict__ rdiag, const int* __restrict__ ext,
-    const uint16_t* __restrict__ sidx, const VT* __restrict__ sval, XT* __restrict__ x0,
+    const uint16_t* __restrict__ sidx, const VT* __restrict__ sval, long zero, XT* __restrict__ x0,
     const XT* __restrict__ b0, int n0, int nrhs, const int* __restrict__ chainPtr, int chain0, int nchain,
     int bundle0, int xcd, XT* const* __restrict__ bX, const XT* const* __restrict__ bB,
     const int* __restrict__ bN) {
@@ -114,48 +131,62 @@ __global__ void __launch_bounds__(BS) k_gs_bundle(
         const bool active = i < nb;
         const long lanes = (long)nb * TPR;
 
+        // Branch-free prologue.  An entry load past the bundle's width or of
+        // a padding row reads the zero entry (local column 0, value 0) that
+        // ends the entry arrays, at index `zero`.  Any other load whose lane
+        // is idle reads element 0 of its array (every device array holds at
+        // least one element); the lane ignores it, and an id read that way
+        // is replaced by 0 before it indexes x.  Without branches the wait
+        // before the gathers counts exactly the entry loads still in flight,
+        // so the gathers overlap the entry block.  The ids the gathers need
+        // go out first.
+        const int sr = active ? s0 + i : 0;
+        const unsigned row = (unsigned)rowId[sr];
+        const unsigned ex0 = (unsigned)ext[t < ne ? e0 + t : 0];
+        const unsigned ex1 = (unsigned)ext[t + BS < ne ? e0 + t + BS : 0];
         int li[E];
-        XT lv[E];
+        typename ValBits<VT>::T lv[E];
 #pragma unroll
         for (int u = 0; u < E; ++u) {
-            const bool in = active && u < wb;
-            const long e = off + (long)u * lanes + t;
-            li[u] = in ? (int)sidx[e] : 0;
-            lv[u] = in ? loadVal<XT>(sval + e) : XT(0);
+            const long e = (active && u < wb) ? off + (long)u * lanes + t : zero;
+            li[u] = (int)sidx[e];
+            lv[u] = sval[e];
         }
-        const int row = active ? rowId[s0 + i] : 0;
-        const XT rd = active ? rdiag[s0 + i] : XT(1);
-        const int myWave = active ? (int)slotWave[s0 + i] : -1;
-        const int ex0 = t < ne ? ext[e0 + t] : 0;
-        const int ex1 = t + BS < ne ? ext[e0 + t + BS] : 0;
+        const XT rd = rdiag[sr];
+        const int myWave = (int)slotWave[sr];
 
         // right-hand sides are independent: grid.y splits them over workgroups
         const int k0 = ONE_RHS ? 0 : (int)blockIdx.y;
         const int k1 = ONE_RHS ? 1 : (gridDim.y > 1 ? k0 + 1 : nrhs);
+        const bool own = active && l == 0;
         for (int k = k0; k < k1; ++k) {
             const size_t vo = (size_t)k * n;
-            if (t < ne) xl[nb + t] = x[vo + ex0];
-            if (t + BS < ne) xl[nb + t + BS] = x[vo + ex1];
+            // every gather in flight before the first LDS write
+            const XT g0 = x[vo + (t < ne ? ex0 : 0u)];
+            const XT g1 = x[vo + (t + BS < ne ? ex1 : 0u)];
+            const XT gr = x[vo + (own ? row : 0u)];
+            const XT br = b[vo + (own ? row : 0u)];
+            if (t < ne) xl[nb + t] = g0;
+            if (t + BS < ne) xl[nb + t + BS] = g1;
+            if (own) xl[i] = gr;
             for (int c = t + 2 * BS; c < ne; c += BS) xl[nb + c] = x[vo + ext[e0 + c]];
-            if (active && l == 0) xl[i] = x[vo + row];
-            const XT br = (active && l == 0) ? b[vo + row] : XT(0);
             __syncthreads();
             for (int w = 0; w < nw; ++w) {
-                if (myWave == w) {
+                if (active && myWave == w) {
                     // two partial sums: half the dependent FMA chain
                     XT sa = XT(0), sb = XT(0);
 #pragma unroll
                     for (int u = 0; u + 1 < E; u += 2) {
-                        sa += lv[u] * xl[li[u]];
-                        sb += lv[u + 1] * xl[li[u + 1]];
+                        sa += valOf<XT>(lv[u]) * xl[li[u]];
+                        sb += valOf<XT>(lv[u + 1]) * xl[li[u + 1]];
                     }
-                    if (E % 2) sa += lv[E - 1] * xl[li[E - 1]];
+                    if (E % 2) sa += valOf<XT>(lv[E - 1]) * xl[li[E - 1]];
                     const XT sm = laneSum<TPR>(sa + sb);
                     if (l == 0) xl[i] = (br - sm) * rd;
                 }
                 __syncthreads();
             }
-            if (active && l == 0) x[vo + row] = xl[i];
+            if (own) x[vo + row] = xl[i];
             __syncthreads();
         }
     };
@@ -177,7 +208,7 @@ __global__ void __launch_bounds__(BS) k_kacz_bundle(
     const int* __restrict__ bSlot, const int* __restrict__ bWave, const long* __restrict__ bOff,
     const int* __restrict__ bLen, const int* __restrict__ bExt, const uint16_t* __restrict__ slotWave,
     const int* __restrict__ rowId, const double* __restrict__ rowNorm, const int* __restrict__ ext,
-    const uint16_t* __restrict__ sidx, const VT* __restrict__ sval, XT* __restrict__ x0,
+    const uint16_t* __restrict__ sidx, const VT* __restrict__ sval, long zero, XT* __restrict__ x0,
     const XT* __restrict__ b0, int n0, int nrhs, const int* __restrict__ chainPtr, int chain0, int nchain, int xcd,
     int ldsLocal,
     XT* const* __restrict__ bX, const XT* const* __restrict__ bB, const int* __restrict__ bN) {
@@ -204,42 +235,44 @@ __global__ void __launch_bounds__(BS) k_kacz_bundle(
         const bool active = i < nb;
         const long lanes = (long)nb * TPR;
 
+        // branch-free prologue, ids first (as in k_gs_bundle)
+        const int sr = active ? s0 + i : 0;
+        const unsigned row = (unsigned)rowId[sr];
+        const unsigned ex0 = (unsigned)ext[t < ne ? e0 + t : 0];
+        const unsigned ex1 = (unsigned)ext[t + BS < ne ? e0 + t + BS : 0];
         int li[E];
-        XT lv[E];
+        typename ValBits<VT>::T lv[E];
 #pragma unroll
         for (int u = 0; u < E; ++u) {
-            const bool in = active && u < wb;
-            const long e = off + (long)u * lanes + t;
-            li[u] = in ? (int)sidx[e] : 0;
-            lv[u] = in ? loadVal<XT>(sval + e) : XT(0);
+            const long e = (active && u < wb) ? off + (long)u * lanes + t : zero;
+            li[u] = (int)sidx[e];
+            lv[u] = sval[e];
         }
-        // issued with the entry block: row ids, norms, waves, column ids
-        const int row = active ? rowId[s0 + i] : 0;
-        const double rn = active ? rowNorm[s0 + i] : 1.0;
-        const int myWave = active ? (int)slotWave[s0 + i] : -1;
-        const int ex0 = t < ne ? ext[e0 + t] : 0;
-        const int ex1 = t + BS < ne ? ext[e0 + t + BS] : 0;
+        const double rn = rowNorm[sr];
+        const int myWave = (int)slotWave[sr];
 
         // right-hand sides are independent: grid.y splits them over workgroups
         const int k0 = (int)blockIdx.y;
         const int k1 = gridDim.y > 1 ? k0 + 1 : nrhs;
         for (int k = k0; k < k1; ++k) {
             const size_t vo = (size_t)k * n;
-            if (t < ne) xl[t] = x[vo + ex0];
-            if (t + BS < ne) xl[t + BS] = x[vo + ex1];
+            const XT g0 = x[vo + (t < ne ? ex0 : 0u)];
+            const XT g1 = x[vo + (t + BS < ne ? ex1 : 0u)];
+            const XT br = b[vo + (active ? row : 0u)];
+            if (t < ne) xl[t] = g0;
+            if (t + BS < ne) xl[t + BS] = g1;
             for (int c = t + 2 * BS; c < ne; c += BS) xl[c] = x[vo + ext[e0 + c]];
-            const XT br = active ? b[vo + row] : XT(0);
             __syncthreads();
             for (int w = 0; w < nw; ++w) {
-                if (myWave == w) {
+                if (active && myWave == w) {
                     XT dot = XT(0);
 #pragma unroll
-                    for (int u = 0; u < E; ++u) dot += lv[u] * xl[li[u]];
+                    for (int u = 0; u < E; ++u) dot += valOf<XT>(lv[u]) * xl[li[u]];
                     dot = laneSum<TPR>(dot);
                     const XT scale = (XT)(((double)dot - (double)br) / rn);
 #pragma unroll
                     for (int u = 0; u < E; ++u)
-                        if (lv[u] != XT(0)) xl[li[u]] -= scale * lv[u];
+                        if (valOf<XT>(lv[u]) != XT(0)) xl[li[u]] -= scale * valOf<XT>(lv[u]);
                 }
                 __syncthreads();
             }
@@ -271,7 +304,7 @@ __global__ void __launch_bounds__(BS) k_gs_bundle_vl(
     const int* __restrict__ bSlot, const int* __restrict__ bWave, const long* __restrict__ bOff,
     const int* __restrict__ bLen, const int* __restrict__ bExt, const uint16_t* __restrict__ slotWave,
     const int* __restrict__ rowId, const XT* __restrict__ diag, const int* __restrict__ ext,
-    const uint16_t* __restrict__ sidx, const VT* __restrict__ sval, XT* __restrict__ x0,
+    const uint16_t* __restrict__ sidx, const VT* __restrict__ sval, long zero, XT* __restrict__ x0,
     const XT* __restrict__ b0, int n0, int nrhs, const int* __restrict__ chainPtr, int chain0, int nchain, int xcd,
     int ldsLocal, int ldsRows, const int* __restrict__ lanePtr, const uint16_t* __restrict__ laneSlot,
     XT* const* __restrict__ bX, const XT* const* __restrict__ bB, const int* __restrict__ bN) {
@@ -298,26 +331,26 @@ __global__ void __launch_bounds__(BS) k_gs_bundle_vl(
         const int lp = lanePtr[bnd], nl = lanePtr[bnd + 1] - lp;
         const int t = threadIdx.x;
         const bool active = t < nl;
-        const int i = active ? (int)laneSlot[lp + t] : 0;
-        const bool first = active && (t == 0 || (int)laneSlot[lp + t - 1] != i);
-
+        // branch-free prologue, ids first (as in k_gs_bundle); the first
+        // lane of every row goes to LDS and is read after the barrier that
+        // follows the gathers
+        const unsigned ex0 = (unsigned)ext[t < ne ? e0 + t : 0];
+        const unsigned ex1 = (unsigned)ext[t + BS < ne ? e0 + t + BS : 0];
+        const int i = (int)laneSlot[active ? lp + t : 0];  // used by active lanes only
+        const int iPrev = (int)laneSlot[active && t > 0 ? lp + t - 1 : 0];
         int li[E];
-        XT lv[E];
+        typename ValBits<VT>::T lv[E];
 #pragma unroll
         for (int u = 0; u < E; ++u) {
-            const bool in = active && u < wb;
-            const long e = off + (long)u * nl + t;
-            li[u] = in ? (int)sidx[e] : 0;
-            lv[u] = in ? loadVal<XT>(sval + e) : XT(0);
+            const long e = (active && u < wb) ? off + (long)u * nl + t : zero;
+            li[u] = (int)sidx[e];
+            lv[u] = sval[e];
         }
-        // issued with the entry block: row ids, inverse diagonals, waves,
-        // outside-column ids; the first lane of every row goes to LDS and is
-        // read after the barrier that follows the gathers
-        const int row = active ? rowId[s0 + i] : 0;
-        const XT dg = active ? diag[s0 + i] : XT(1);  // 1 / a_rr
-        const int myWave = active ? (int)slotWave[s0 + i] : -1;
-        const int ex0 = t < ne ? ext[e0 + t] : 0;
-        const int ex1 = t + BS < ne ? ext[e0 + t + BS] : 0;
+        const bool first = active && (t == 0 || iPrev != i);
+        const int sr = active ? s0 + i : 0;
+        const unsigned row = (unsigned)rowId[sr];
+        const XT dg = diag[sr];  // 1 / a_rr
+        const int myWave = (int)slotWave[sr];
         if (first) rl0[i] = t;
         if (t == 0) rl0[nb] = nl;
         int rowL0 = 0, rowL1 = 0;
@@ -326,25 +359,28 @@ __global__ void __launch_bounds__(BS) k_gs_bundle_vl(
         const int k1 = gridDim.y > 1 ? k0 + 1 : nrhs;
         for (int k = k0; k < k1; ++k) {
             const size_t vo = (size_t)k * n;
-            if (t < ne) xl[nb + t] = x[vo + ex0];
-            if (t + BS < ne) xl[nb + t + BS] = x[vo + ex1];
+            const XT g0 = x[vo + (t < ne ? ex0 : 0u)];
+            const XT g1 = x[vo + (t + BS < ne ? ex1 : 0u)];
+            const XT gr = x[vo + (first ? row : 0u)];
+            const XT br = b[vo + (first ? row : 0u)];
+            if (t < ne) xl[nb + t] = g0;
+            if (t + BS < ne) xl[nb + t + BS] = g1;
+            if (first) xl[i] = gr;
             for (int c = t + 2 * BS; c < ne; c += BS) xl[nb + c] = x[vo + ext[e0 + c]];
-            if (first) xl[i] = x[vo + row];
-            const XT br = first ? b[vo + row] : XT(0);
             __syncthreads();
             if (k == k0 && active) {
                 rowL0 = rl0[i];
                 rowL1 = rl0[i + 1];
             }
             for (int w = 0; w < nw; ++w) {
-                if (myWave == w) {
+                if (active && myWave == w) {
                     XT sum = XT(0);
 #pragma unroll
-                    for (int u = 0; u < E; ++u) sum += lv[u] * xl[li[u]];
+                    for (int u = 0; u < E; ++u) sum += valOf<XT>(lv[u]) * xl[li[u]];
                     part[t] = sum;
                 }
                 __syncthreads();
-                if (myWave == w && first) {
+                if (first && myWave == w) {
                     XT s = XT(0);
                     for (int l = rowL0; l < rowL1; ++l) s += part[l];
                     xl[i] = (br - s) * dg;
@@ -370,7 +406,7 @@ __global__ void __launch_bounds__(BS) k_kacz_bundle_vl(
     const int* __restrict__ bSlot, const int* __restrict__ bWave, const long* __restrict__ bOff,
     const int* __restrict__ bLen, const int* __restrict__ bExt, const uint16_t* __restrict__ slotWave,
     const int* __restrict__ rowId, const double* __restrict__ rowNorm, const int* __restrict__ ext,
-    const uint16_t* __restrict__ sidx, const VT* __restrict__ sval, XT* __restrict__ x0,
+    const uint16_t* __restrict__ sidx, const VT* __restrict__ sval, long zero, XT* __restrict__ x0,
     const XT* __restrict__ b0, int n0, int nrhs, const int* __restrict__ chainPtr, int chain0, int nchain, int xcd,
     int ldsLocal, int ldsRows, const int* __restrict__ lanePtr, const uint16_t* __restrict__ laneSlot,
     XT* const* __restrict__ bX, const XT* const* __restrict__ bB, const int* __restrict__ bN) {
@@ -397,24 +433,24 @@ __global__ void __launch_bounds__(BS) k_kacz_bundle_vl(
         const int lp = lanePtr[bnd], nl = lanePtr[bnd + 1] - lp;
         const int t = threadIdx.x;
         const bool active = t < nl;
-        const int i = active ? (int)laneSlot[lp + t] : 0;
-
+        // branch-free prologue, ids first (as in k_gs_bundle_vl)
+        const unsigned ex0 = (unsigned)ext[t < ne ? e0 + t : 0];
+        const unsigned ex1 = (unsigned)ext[t + BS < ne ? e0 + t + BS : 0];
+        const int i = (int)laneSlot[active ? lp + t : 0];  // used by active lanes only
+        const int iPrev = (int)laneSlot[active && t > 0 ? lp + t - 1 : 0];
         int li[E];
-        XT lv[E];
+        typename ValBits<VT>::T lv[E];
 #pragma unroll
         for (int u = 0; u < E; ++u) {
-            const bool in = active && u < wb;
-            const long e = off + (long)u * nl + t;
-            li[u] = in ? (int)sidx[e] : 0;
-            lv[u] = in ? loadVal<XT>(sval + e) : XT(0);
+            const long e = (active && u < wb) ? off + (long)u * nl + t : zero;
+            li[u] = (int)sidx[e];
+            lv[u] = sval[e];
         }
-        // issued with the entry block (see k_gs_bundle_vl)
-        const int row = active ? rowId[s0 + i] : 0;
-        const double rn = active ? rowNorm[s0 + i] : 1.0;
-        const int myWave = active ? (int)slotWave[s0 + i] : -1;
-        const int ex0 = t < ne ? ext[e0 + t] : 0;
-        const int ex1 = t + BS < ne ? ext[e0 + t + BS] : 0;
-        if (active && (t == 0 || (int)laneSlot[lp + t - 1] != i)) rl0[i] = t;
+        const int sr = active ? s0 + i : 0;
+        const unsigned row = (unsigned)rowId[sr];
+        const double rn = rowNorm[sr];
+        const int myWave = (int)slotWave[sr];
+        if (active && (t == 0 || iPrev != i)) rl0[i] = t;
         if (t == 0) rl0[nb] = nl;
         int rowL0 = 0, rowL1 = 0;
 
@@ -422,30 +458,32 @@ __global__ void __launch_bounds__(BS) k_kacz_bundle_vl(
         const int k1 = gridDim.y > 1 ? k0 + 1 : nrhs;
         for (int k = k0; k < k1; ++k) {
             const size_t vo = (size_t)k * n;
-            if (t < ne) xl[t] = x[vo + ex0];
-            if (t + BS < ne) xl[t + BS] = x[vo + ex1];
+            const XT g0 = x[vo + (t < ne ? ex0 : 0u)];
+            const XT g1 = x[vo + (t + BS < ne ? ex1 : 0u)];
+            const XT br = b[vo + (active ? row : 0u)];
+            if (t < ne) xl[t] = g0;
+            if (t + BS < ne) xl[t + BS] = g1;
             for (int c = t + 2 * BS; c < ne; c += BS) xl[c] = x[vo + ext[e0 + c]];
-            const XT br = active ? b[vo + row] : XT(0);
             __syncthreads();
             if (k == k0 && active) {
                 rowL0 = rl0[i];
                 rowL1 = rl0[i + 1];
             }
             for (int w = 0; w < nw; ++w) {
-                if (myWave == w) {
+                if (active && myWave == w) {
                     XT dot = XT(0);
 #pragma unroll
-                    for (int u = 0; u < E; ++u) dot += lv[u] * xl[li[u]];
+                    for (int u = 0; u < E; ++u) dot += valOf<XT>(lv[u]) * xl[li[u]];
                     part[t] = dot;
                 }
                 __syncthreads();
-                if (myWave == w) {
+                if (active && myWave == w) {
                     XT rowDot = XT(0);
                     for (int l = rowL0; l < rowL1; ++l) rowDot += part[l];
                     const XT scale = (XT)(((double)rowDot - (double)br) / rn);
 #pragma unroll
                     for (int u = 0; u < E; ++u)
-                        if (lv[u] != XT(0)) xl[li[u]] -= scale * lv[u];
+                        if (valOf<XT>(lv[u]) != XT(0)) xl[li[u]] -= scale * valOf<XT>(lv[u]);
                 }
                 __syncthreads();
             }
@@ -485,6 +523,7 @@ struct BundleDev {
     DevBuf<long> off;
     DevBuf<uint16_t> sidx;
     DevBuf<VT> sval;
+    long zero = 0;  // index of the zero entry in sidx / sval
     DevBuf<XT> diag;
     DevBuf<double> rowNorm;
     int load(BundleTable&& t) {
@@ -502,6 +541,10 @@ struct BundleDev {
             if ((e = upload(laneSlot, h.laneSlot.data(), h.laneSlot.size()))) return e;
         }
         if ((e = upload(rowId, h.rowId.data(), h.rowId.size()))) return e;
+        // the zero entry: what a lane loads for an entry it does not have
+        zero = (long)h.sidx.size();
+        h.sidx.push_back(0);
+        h.sval.push_back(0.0);
         if ((e = upload(sidx, h.sidx.data(), h.sidx.size()))) return e;
         std::vector<VT> v(h.sval.size());
         for (size_t q = 0; q < v.size(); ++q) v[q] = conv(h.sval[q]);
@@ -552,17 +595,17 @@ void launchBundles(int kind, int nrhs, const BundleDev<VT, XT>& d, int c0, int n
     if (kind == RACE_K_KACZ)
         hipLaunchKernelGGL((k_kacz_bundle<BS, TPR, EE, VT, XT, BATCH>), grid, dim3(BS), d.lds(p, BS), st,
                            d.slot.p, d.wave.p, d.off.p, d.len.p, d.extPtr.p, d.slotWave.p, d.rowId.p,
-                           d.rowNorm.p, d.ext.p, d.sidx.p, d.sval.p, x, b, n, nrhs, d.chainPtr.p, c0, nchain, xcd,
+                           d.rowNorm.p, d.ext.p, d.sidx.p, d.sval.p, d.zero, x, b, n, nrhs, d.chainPtr.p, c0, nchain, xcd,
                            d.h.phLocal[p], bX, bB, bN);
     else if (nrhs == 1)
         hipLaunchKernelGGL((k_gs_bundle<BS, TPR, EE, VT, XT, true, BATCH>), dim3(grid.x), dim3(BS), d.lds(p, BS), st,
                            d.slot.p, d.wave.p, d.off.p, d.len.p, d.extPtr.p, d.slotWave.p, d.rowId.p, d.diag.p,
-                           d.ext.p, d.sidx.p, d.sval.p, x, b, n, nrhs, d.chainPtr.p, c0, nchain, bundle0, xcd, bX,
+                           d.ext.p, d.sidx.p, d.sval.p, d.zero, x, b, n, nrhs, d.chainPtr.p, c0, nchain, bundle0, xcd, bX,
                            bB, bN);
     else
         hipLaunchKernelGGL((k_gs_bundle<BS, TPR, EE, VT, XT, false, BATCH>), grid, dim3(BS), d.lds(p, BS),
                            st, d.slot.p, d.wave.p, d.off.p, d.len.p, d.extPtr.p, d.slotWave.p, d.rowId.p,
-                           d.diag.p, d.ext.p, d.sidx.p, d.sval.p, x, b, n, nrhs, d.chainPtr.p, c0, nchain, bundle0,
+                           d.diag.p, d.ext.p, d.sidx.p, d.sval.p, d.zero, x, b, n, nrhs, d.chainPtr.p, c0, nchain, bundle0,
                            xcd, bX, bB, bN);
 }
 
@@ -574,12 +617,12 @@ void launchBundlesVL(int kind, int nrhs, const BundleDev<VT, XT>& d, int c0, int
     if (kind == RACE_K_KACZ)
         hipLaunchKernelGGL((k_kacz_bundle_vl<BS, VT, XT, BATCH>), grid, dim3(BS), d.lds(p, BS), st, d.slot.p, d.wave.p,
                            d.off.p, d.len.p, d.extPtr.p, d.slotWave.p, d.rowId.p, d.rowNorm.p, d.ext.p, d.sidx.p,
-                           d.sval.p, x, b, n, nrhs, d.chainPtr.p, c0, nchain, xcd, d.h.phLocal[p], d.h.phRows[p],
+                           d.sval.p, d.zero, x, b, n, nrhs, d.chainPtr.p, c0, nchain, xcd, d.h.phLocal[p], d.h.phRows[p],
                            d.lanePtr.p, d.laneSlot.p, bX, bB, bN);
     else
         hipLaunchKernelGGL((k_gs_bundle_vl<BS, VT, XT, BATCH>), grid, dim3(BS), d.lds(p, BS), st, d.slot.p, d.wave.p,
                            d.off.p, d.len.p, d.extPtr.p, d.slotWave.p, d.rowId.p, d.diag.p, d.ext.p, d.sidx.p,
-                           d.sval.p, x, b, n, nrhs, d.chainPtr.p, c0, nchain, xcd, d.h.phLocal[p], d.h.phRows[p],
+                           d.sval.p, d.zero, x, b, n, nrhs, d.chainPtr.p, c0, nchain, xcd, d.h.phLocal[p], d.h.phRows[p],
                            d.lanePtr.p, d.laneSlot.p, bX, bB, bN);
 }
 

@@ -281,6 +281,47 @@ def test_long_rows_variable_lanes(oracle, kind, T, rows):
         assert relerr(dx.download(), xo) <= TOL
 
 
+def _block_diagonal(nblk, blk, seed):
+    """nblk dense blk x blk blocks on the diagonal, diagonally dominant"""
+    rng = np.random.default_rng(seed)
+    rp, col, val = [0], [], []
+    for bk in range(nblk):
+        for r in range(blk):
+            for c in range(blk):
+                col.append(bk * blk + c)
+                val.append(blk + 1.0 + rng.random() if c == r else -rng.random())
+            rp.append(len(col))
+    return np.array(rp, np.int32), np.array(col, np.int32), np.array(val)
+
+
+@pytest.mark.parametrize("kind", ["gs", "kacz"])
+@pytest.mark.parametrize("blk,T", [(1, 16), (3, 16), (70, 4)])
+def test_bundles_without_outside_columns(oracle, kind, blk, T):
+    """Block-diagonal matrices: bundles whose rows reach no column outside
+    the bundle (no outside-column gathers) next to bundles that do; blk = 1
+    (a diagonal matrix) leaves Gauss-Seidel with no entries at all, only the
+    zero entry that lanes past a bundle's width read; blk = 70 takes the
+    variable-lanes kernels."""
+    rp, col, val = _block_diagonal(600 // blk if blk < 70 else 12, blk, seed=blk)
+    n = len(rp) - 1
+    dist = R.ONE if kind == "gs" else R.TWO
+    ce, (prp, pcol, pval), zt = _prepared(rp, col, val, T, dist, oracle)
+    b = M.uniform_vec(n, 17)
+    x0 = M.uniform_vec(n, 18)
+    dx = R.DeviceArray.from_host(x0)
+    db = R.DeviceArray.from_host(b)
+    ce.setOption("layout", 2)
+    k = R.K_GS if kind == "gs" else R.K_KACZ
+    okind = oracle.K_GS if kind == "gs" else oracle.K_KACZ
+    fid = ce.registerDevice(k, prp, pcol, pval, dx, db)
+    xo = x0.copy()
+    for rev in (False, True):
+        print("case", kind, "blk", blk, "rev", rev, flush=True)
+        ce.executeFunction(fid, rev)
+        oracle.run_calls(okind, prp, pcol, pval, xo, b, oracle.leaf_calls(zt, rev))
+        assert relerr(dx.download(), xo) <= TOL
+
+
 # symm_mode: 0 default, 1 colour phases, 2 SELL atomics, 3 expanded full matrix
 @pytest.mark.parametrize("mode", [0, 1, 2, 3])
 @pytest.mark.parametrize("T", [1, 8, 64])
