@@ -53,10 +53,28 @@ __device__ __forceinline__ XT valOf(unsigned v) {
     return (XT)__uint_as_float(v << 16);
 }
 
+// Sum over the TPR lanes of a row.  The lanes of a row lie in one quad
+// (TPR <= 4), so the partner's value comes through a DPP quad permutation,
+// a VALU move with no LDS round trip: lane ^ 2 is quad_perm [2,3,0,1]
+// (0x4E), lane ^ 1 is quad_perm [1,0,3,2] (0xB1).  The partner lanes of a
+// row are always active together.  Same additions, in the same order, as a
+// butterfly over shuffles.
+template <int CTRL>
+__device__ __forceinline__ float quadSwap(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double quadSwap(double v) {
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)u, CTRL, 0xF, 0xF, false);
+    const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
 template <int TPR, typename T>
 __device__ __forceinline__ T laneSum(T v) {
-#pragma unroll
-    for (int o = TPR / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, TPR);
+    static_assert(TPR == 1 || TPR == 2 || TPR == 4, "the lanes of a row must lie in one quad");
+    if (TPR == 4) v += quadSwap<0x4E>(v);
+    if (TPR >= 2) v += quadSwap<0xB1>(v);
     return v;
 }
 
