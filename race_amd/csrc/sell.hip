@@ -25,9 +25,11 @@ namespace {
 constexpr int kC = 64;               // rows per chunk (one wavefront)
 constexpr int kWavesPerBlock = kBlock / kC;
 
+// 16 bytes, one load: entry offset in units of 64 (every chunk stores
+// 64 * width entries, so offsets are multiples of 64)
 struct SellChunk {
-    int row0, rows, width, pad;
-    long off;
+    int row0, rows, width, off64;
+    __host__ __device__ long off() const { return (long)off64 * kC; }
 };
 
 // matrix value storage (double, float, or bfloat16 bits); the arithmetic
@@ -56,6 +58,8 @@ template <typename VT>
 struct SellMatrix {
     int n = 0, nChunk = 0;
     int uniformW = -1;          // >= 0: every chunk has this width, chunk c at c * 64 * width
+    int maxW = 0;               // the widest chunk
+    bool narrow() const { return maxW <= 8; }
     long entries = 0;
     std::vector<int> cutChunk;  // chunk index of every cut
     DevBuf<SellChunk> chunks;
@@ -78,7 +82,7 @@ struct SellMatrix {
                 const int rows = std::min(kC, cuts[k + 1] - r0);
                 int w = 0;
                 for (int i = 0; i < rows; ++i) w = std::max(w, rp[r0 + i + 1] - rp[r0 + i]);
-                ch.push_back(SellChunk{r0, rows, w, 0, total});
+                ch.push_back(SellChunk{r0, rows, w, (int)(total / kC)});
                 total += (long)w * kC;
             }
         }
@@ -92,7 +96,7 @@ struct SellMatrix {
                 total = 0;
                 for (SellChunk& c : ch) {
                     c.width = wmax;
-                    c.off = total;
+                    c.off64 = (int)(total / kC);
                     total += (long)wmax * kC;
                 }
             }
@@ -100,6 +104,9 @@ struct SellMatrix {
         cutChunk.push_back((int)ch.size());
         nChunk = (int)ch.size();
         entries = total;
+        maxW = 0;
+        for (const SellChunk& c : ch) maxW = std::max(maxW, c.width);
+        if (total / kC > (long)INT32_MAX) return ERR_INVALID_ARG;  // offsets in units of 64 must fit an int
         std::vector<int> hc((size_t)total);
         std::vector<VT> hv((size_t)total);
         for (const SellChunk& c : ch)
@@ -107,7 +114,7 @@ struct SellMatrix {
                 const int r = c.row0 + std::min(i, c.rows - 1);
                 const int len = i < c.rows ? rp[r + 1] - rp[r] : 0;
                 for (int j = 0; j < c.width; ++j) {
-                    const long pos = c.off + (long)j * kC + i;
+                    const long pos = c.off() + (long)j * kC + i;
                     if (j < len) {
                         hc[pos] = d.col[rp[r] + j];
                         hv[pos] = toStorage<VT>(d.val[rp[r] + j]);
@@ -125,32 +132,57 @@ struct SellMatrix {
     }
 };
 
+// G entries of a row from entry j0: the column ids and values are loaded
+// together, then the x gathers, so each group is two dependent memory
+// levels and its loads need no branch.  CLAMP (the row's last group): past
+// the chunk's width a lane re-reads the chunk's last entry and multiplies
+// its value by 0 (a product, not a select on the loaded value, which the
+// compiler would turn into a branch around the gathers); t + (+-0) * x == t
+// for finite values (t starts at +0, and a sum that starts at +0 is never
+// -0).
+template <int G, bool CLAMP, typename VT, typename XT>
+__device__ __forceinline__ void sellGroup(const int* __restrict__ cc, const VT* __restrict__ vv, int j0, int width,
+                                          const XT* __restrict__ x, XT& t) {
+    int c[G];
+    XT v[G];
+#pragma unroll
+    for (int u = 0; u < G; ++u) {
+        const int j = CLAMP ? min(j0 + u, width - 1) : j0 + u;
+        c[u] = cc[j * kC];
+        const XT a = fromStorage<XT>(vv[j * kC]);
+        v[u] = CLAMP ? a * (j0 + u < width ? XT(1) : XT(0)) : a;
+    }
+#pragma unroll
+    for (int u = 0; u < G; ++u) t += v[u] * x[c[u]];
+}
+
 // y[row] = sum_j a_j x[c_j] for the rows of one chunk; sequential in j, as
-// the CRS row order
-template <typename VT, typename XT>
+// the CRS row order.  NARROW (no chunk wider than 8 entries): one clamped
+// group of 8, two dependent memory levels per row.  Otherwise whole groups
+// of 8, then clamped groups of 4 (at most 3 entries re-read per row).  The
+// two shapes are separate kernels: in one kernel the compiler merges the
+// tails and gathers part of the group in a second round.  The padding lanes
+// of a chunk hold value-0 entries on a real row, so every lane of the chunk
+// may compute; callers guard the store.
+template <bool NARROW, typename VT, typename XT>
 __device__ __forceinline__ XT sellRow(const SellChunk& k, int lane, const int* __restrict__ col,
                                       const VT* __restrict__ val, const XT* __restrict__ x) {
-    const int* __restrict__ cc = col + k.off + lane;
-    const VT* __restrict__ vv = val + k.off + lane;
+    const int* __restrict__ cc = col + k.off() + lane;
+    const VT* __restrict__ vv = val + k.off() + lane;
     XT t = XT(0);
-    int j = 0;
-    for (; j + 4 <= k.width; j += 4) {
-        const int c0 = cc[(j + 0) * kC], c1 = cc[(j + 1) * kC], c2 = cc[(j + 2) * kC], c3 = cc[(j + 3) * kC];
-        const XT v0 = fromStorage<XT>(vv[(j + 0) * kC]), v1 = fromStorage<XT>(vv[(j + 1) * kC]);
-        const XT v2 = fromStorage<XT>(vv[(j + 2) * kC]), v3 = fromStorage<XT>(vv[(j + 3) * kC]);
-        const XT x0 = x[c0], x1 = x[c1], x2 = x[c2], x3 = x[c3];
-        t += v0 * x0;
-        t += v1 * x1;
-        t += v2 * x2;
-        t += v3 * x3;
+    if (NARROW) {
+        if (k.width > 0) sellGroup<8, true>(cc, vv, 0, k.width, x, t);
+    } else {
+        int j0 = 0;
+        for (; j0 + 8 <= k.width; j0 += 8) sellGroup<8, false>(cc, vv, j0, k.width, x, t);
+        for (; j0 < k.width; j0 += 4) sellGroup<4, true>(cc, vv, j0, k.width, x, t);
     }
-    for (; j < k.width; ++j) t += fromStorage<XT>(vv[j * kC]) * x[cc[j * kC]];
     return t;
 }
 
 // y = A x (ACC: y += A x) for nrhs right-hand sides (vectors column-major,
 // v[r + k n])
-template <bool ACC, typename VT, typename XT>
+template <bool ACC, bool NARROW, typename VT, typename XT>
 __global__ void __launch_bounds__(kBlock) k_sell_spmv(const SellChunk* __restrict__ chunks, int nChunk,
                                                       const int* __restrict__ col, const VT* __restrict__ val,
                                                       const XT* __restrict__ x, XT* __restrict__ y, long n,
@@ -159,9 +191,9 @@ __global__ void __launch_bounds__(kBlock) k_sell_spmv(const SellChunk* __restric
     const int lane = threadIdx.x % kC;
     if (c >= nChunk) return;
     const SellChunk k = chunks[c];
-    if (lane >= k.rows) return;
     for (int q = 0; q < nrhs; ++q) {
-        const XT t = sellRow(k, lane, col, val, x + q * n);
+        const XT t = sellRow<NARROW>(k, lane, col, val, x + q * n);
+        if (lane >= k.rows) continue;
         if (ACC)
             y[q * n + k.row0 + lane] += t;
         else
@@ -171,17 +203,18 @@ __global__ void __launch_bounds__(kBlock) k_sell_spmv(const SellChunk* __restric
 
 // the same over uniform chunks: chunk c holds rows [64 c, 64 c + 64) at
 // c * 64 * W, no chunk records to load
-template <bool ACC, typename VT, typename XT>
+template <bool ACC, bool NARROW, typename VT, typename XT>
 __global__ void __launch_bounds__(kBlock) k_sell_spmv_u(int W, const int* __restrict__ col,
                                                         const VT* __restrict__ val, const XT* __restrict__ x,
                                                         XT* __restrict__ y, int n, int nrhs) {
     const int c = blockIdx.x * kWavesPerBlock + threadIdx.x / kC;
     const int lane = threadIdx.x % kC;
     const int r = c * kC + lane;
-    if (r >= n) return;
-    const SellChunk k{0, kC, W, 0, (long)c * kC * W};
+    if (c * kC >= n) return;  // a whole wave past the last chunk
+    const SellChunk k{0, kC, W, c * W};
     for (int q = 0; q < nrhs; ++q) {
-        const XT t = sellRow(k, lane, col, val, x + (long)q * n);
+        const XT t = sellRow<NARROW>(k, lane, col, val, x + (long)q * n);
+        if (r >= n) continue;  // padding lanes of the last chunk
         if (ACC)
             y[(long)q * n + r] += t;
         else
@@ -207,8 +240,8 @@ __global__ void __launch_bounds__(kBlock) k_sell_symm_atomic(const SellChunk* __
     const SellChunk k = chunks[c];
     if (lane >= k.rows) return;
     const int r = k.row0 + lane;
-    const int* __restrict__ cc = col + k.off + lane;
-    const double* __restrict__ vv = val + k.off + lane;
+    const int* __restrict__ cc = col + k.off() + lane;
+    const double* __restrict__ vv = val + k.off() + lane;
     for (int q = 0; q < nrhs; ++q) {
         const double* __restrict__ xq = x + q * n;
         double* bq = b + q * n;
@@ -234,35 +267,68 @@ struct MpkSeg {
     int c0, c1, q, blockStart;
 };
 
+// Up to kSegArgs segments of a step travel by value in the kernel
+// arguments, so finding a block's segment costs no memory round trip; a
+// step with more segments (powers above 16) reads them from the device
+// table.
+constexpr int kSegArgs = 16;
+struct MpkSegArgs {
+    MpkSeg s[kSegArgs];
+};
+
 // Power q is stored in vector slot q % nslot: nslot = k + 1 keeps every
 // power, nslot = 2 keeps the last two (the only-highest variant).
-template <typename VT, typename XT>
+template <typename VT, typename XT, bool TABLE, bool NARROW>
 __global__ void __launch_bounds__(kBlock) k_sell_mpk_step(const SellChunk* __restrict__ chunks,
                                                           const int* __restrict__ col,
                                                           const VT* __restrict__ val, XT* __restrict__ x, long n,
-                                                          int nslot, const MpkSeg* __restrict__ segs, int nseg) {
-    int s = 0;
-    while (s + 1 < nseg && (int)blockIdx.x >= segs[s + 1].blockStart) ++s;
-    const MpkSeg sg = segs[s];
+                                                          int nslot, const MpkSeg* __restrict__ segs,
+                                                          MpkSegArgs inl, int nseg) {
+    // the block's segment: the last one starting at or before it (by
+    // value: selects over arguments that are all loaded, no branches)
+    MpkSeg sg;
+    if (TABLE) {
+        int s = 0;
+        while (s + 1 < nseg && (int)blockIdx.x >= segs[s + 1].blockStart) ++s;
+        sg = segs[s];
+    } else {
+        sg = inl.s[0];
+#pragma unroll
+        for (int q = 1; q < kSegArgs; ++q) {
+            const MpkSeg o = inl.s[q];
+            const bool take = q < nseg && (int)blockIdx.x >= o.blockStart;
+            sg.c0 = take ? o.c0 : sg.c0;
+            sg.c1 = take ? o.c1 : sg.c1;
+            sg.q = take ? o.q : sg.q;
+            sg.blockStart = take ? o.blockStart : sg.blockStart;
+        }
+    }
     const int c = sg.c0 + ((int)blockIdx.x - sg.blockStart) * kWavesPerBlock + (int)threadIdx.x / kC;
     const int lane = threadIdx.x % kC;
     if (c >= sg.c1) return;
     const SellChunk k = chunks[c];
-    if (lane >= k.rows) return;
     const XT* __restrict__ xin = x + (long)((sg.q - 1) % nslot) * n;
-    x[(long)(sg.q % nslot) * n + k.row0 + lane] = sellRow(k, lane, col, val, xin);
+    const XT t = sellRow<NARROW>(k, lane, col, val, xin);
+    if (lane < k.rows) x[(long)(sg.q % nslot) * n + k.row0 + lane] = t;
 }
 
 // one launch of y = A x or y += A x over a single-range SELL matrix
-template <bool ACC, typename VT, typename XT>
-void launchSell(const SellMatrix<VT>& A, const void* x, void* y, int nrhs, hipStream_t st) {
+template <bool ACC, bool NARROW, typename VT, typename XT>
+void launchSellShape(const SellMatrix<VT>& A, const void* x, void* y, int nrhs, hipStream_t st) {
     const unsigned g = (unsigned)std::max(1, (A.nChunk + kWavesPerBlock - 1) / kWavesPerBlock);
     if (A.uniformW >= 0)
-        hipLaunchKernelGGL((k_sell_spmv_u<ACC, VT, XT>), dim3(g), dim3(kBlock), 0, st, A.uniformW, A.col.p,
-                           A.val.p, (const XT*)x, (XT*)y, A.n, nrhs);
+        hipLaunchKernelGGL((k_sell_spmv_u<ACC, NARROW, VT, XT>), dim3(g), dim3(kBlock), 0, st, A.uniformW,
+                           A.col.p, A.val.p, (const XT*)x, (XT*)y, A.n, nrhs);
     else
-        hipLaunchKernelGGL((k_sell_spmv<ACC, VT, XT>), dim3(g), dim3(kBlock), 0, st, A.chunks.p, A.nChunk,
-                           A.col.p, A.val.p, (const XT*)x, (XT*)y, (long)A.n, nrhs);
+        hipLaunchKernelGGL((k_sell_spmv<ACC, NARROW, VT, XT>), dim3(g), dim3(kBlock), 0, st, A.chunks.p,
+                           A.nChunk, A.col.p, A.val.p, (const XT*)x, (XT*)y, (long)A.n, nrhs);
+}
+template <bool ACC, typename VT, typename XT>
+void launchSell(const SellMatrix<VT>& A, const void* x, void* y, int nrhs, hipStream_t st) {
+    if (A.narrow())
+        launchSellShape<ACC, true, VT, XT>(A, x, y, nrhs, st);
+    else
+        launchSellShape<ACC, false, VT, XT>(A, x, y, nrhs, st);
 }
 
 // b = A x
@@ -381,6 +447,7 @@ class MpkFn : public DeviceFn {
     int nslot = 2;
     // for power p: segments stepSeg[p][L] .. stepSeg[p][L+1] of step L
     std::vector<std::vector<int>> stepSeg, stepBlocks;
+    std::vector<MpkSeg> hostSegs;
     DevBuf<MpkSeg> segs;
     int init(const race_kernel_desc& d, const ExecContext& ctx) {
         kind = d.kind;
@@ -394,7 +461,8 @@ class MpkFn : public DeviceFn {
         int e;
         if ((e = A.build(d, gp))) return e;
         const int G = (int)gp.size() - 1;
-        std::vector<MpkSeg> all;
+        std::vector<MpkSeg>& all = hostSegs;
+        all.clear();
         stepSeg.assign(powerMax + 1, {});
         stepBlocks.assign(powerMax + 1, {});
         for (int p = 1; p <= powerMax; ++p) {
@@ -428,8 +496,23 @@ class MpkFn : public DeviceFn {
         const std::vector<int>& nb = stepBlocks[power];
         for (int L = 0; L + 1 < (int)sg.size(); ++L) {
             if (nb[L] == 0) continue;
-            hipLaunchKernelGGL((k_sell_mpk_step<VT, XT>), dim3(nb[L]), dim3(kBlock), 0, st, A.chunks.p, A.col.p,
-                               A.val.p, (XT*)x, (long)A.n, nslot, segs.p + sg[L], sg[L + 1] - sg[L]);
+            const int ns = sg[L + 1] - sg[L];
+            MpkSegArgs inl{};
+            if (ns <= kSegArgs) std::copy(hostSegs.begin() + sg[L], hostSegs.begin() + sg[L + 1], inl.s);
+            const dim3 grid(nb[L]), block(kBlock);
+            const MpkSeg* tab = segs.p + sg[L];
+            if (ns <= kSegArgs && A.narrow())
+                hipLaunchKernelGGL((k_sell_mpk_step<VT, XT, false, true>), grid, block, 0, st, A.chunks.p, A.col.p,
+                                   A.val.p, (XT*)x, (long)A.n, nslot, tab, inl, ns);
+            else if (ns <= kSegArgs)
+                hipLaunchKernelGGL((k_sell_mpk_step<VT, XT, false, false>), grid, block, 0, st, A.chunks.p,
+                                   A.col.p, A.val.p, (XT*)x, (long)A.n, nslot, tab, inl, ns);
+            else if (A.narrow())
+                hipLaunchKernelGGL((k_sell_mpk_step<VT, XT, true, true>), grid, block, 0, st, A.chunks.p, A.col.p,
+                                   A.val.p, (XT*)x, (long)A.n, nslot, tab, inl, ns);
+            else
+                hipLaunchKernelGGL((k_sell_mpk_step<VT, XT, true, false>), grid, block, 0, st, A.chunks.p,
+                                   A.col.p, A.val.p, (XT*)x, (long)A.n, nslot, tab, inl, ns);
         }
         RACE_HIP(hipGetLastError());
         return OK;
@@ -481,7 +564,7 @@ class SpmtvFn : public DeviceFn {
 // r = b - A x (FP64 product, r stored as RT) and the sums of r^2 and b^2
 // for the refinement; grid-stride over chunks so few workgroups add to the
 // two norm accumulators
-template <typename RT>
+template <typename RT, bool NARROW>
 __global__ void __launch_bounds__(kBlock) k_sell_residual(const SellChunk* __restrict__ chunks, int nChunk,
                                                           int W, const int* __restrict__ col,
                                                           const double* __restrict__ val,
@@ -492,14 +575,14 @@ __global__ void __launch_bounds__(kBlock) k_sell_residual(const SellChunk* __res
     const int lane = threadIdx.x % kC, wv = threadIdx.x / kC;
     double rr = 0.0, bb = 0.0;
     for (int c = blockIdx.x * kWavesPerBlock + wv; c < nChunk; c += gridDim.x * kWavesPerBlock) {
-        const SellChunk k = W >= 0 ? SellChunk{c * kC, (int)std::min<long>(kC, n - (long)c * kC), W, 0,
-                                               (long)c * kC * W}
-                                   : chunks[c];
-        if (lane >= k.rows) continue;
-        const long row = k.row0 + lane;
+        const SellChunk k =
+            W >= 0 ? SellChunk{c * kC, (int)std::min<long>(kC, n - (long)c * kC), W, c * W} : chunks[c];
+        const bool live = lane < k.rows;  // padding lanes compute, store nothing
+        const long row = k.row0 + (live ? lane : 0);
         for (int q = 0; q < nrhs; ++q) {
-            const double t = sellRow(k, lane, col, val, x + q * n);
+            const double t = sellRow<NARROW>(k, lane, col, val, x + q * n);
             const double bv = b[q * n + row];
+            if (!live) continue;
             const double rv = bv - t;
             r[q * n + row] = (RT)rv;
             rr += rv * rv;
@@ -528,15 +611,22 @@ class SellResidual : public ResidualOp {
   public:
     SellMatrix<double> A;
     int init(const race_kernel_desc& d) { return A.build(d, {0, d.nrow}, true); }
+    template <typename RT, bool NARROW>
+    void launch(const double* x, const double* b, void* r, int nrhs, double* norms, hipStream_t st) {
+        const int blocks = std::max(1, std::min(4096, (A.nChunk + kWavesPerBlock - 1) / kWavesPerBlock));
+        hipLaunchKernelGGL((k_sell_residual<RT, NARROW>), dim3(blocks), dim3(kBlock), 0, st, A.chunks.p, A.nChunk,
+                           A.uniformW, A.col.p, A.val.p, x, b, (RT*)r, (long)A.n, nrhs, norms);
+    }
     int run(const double* x, const double* b, void* r, bool rFp32, int nrhs, double* norms,
             hipStream_t st) override {
-        const int blocks = std::max(1, std::min(4096, (A.nChunk + kWavesPerBlock - 1) / kWavesPerBlock));
-        if (rFp32)
-            hipLaunchKernelGGL(k_sell_residual<float>, dim3(blocks), dim3(kBlock), 0, st, A.chunks.p, A.nChunk,
-                               A.uniformW, A.col.p, A.val.p, x, b, (float*)r, (long)A.n, nrhs, norms);
+        if (rFp32 && A.narrow())
+            launch<float, true>(x, b, r, nrhs, norms, st);
+        else if (rFp32)
+            launch<float, false>(x, b, r, nrhs, norms, st);
+        else if (A.narrow())
+            launch<double, true>(x, b, r, nrhs, norms, st);
         else
-            hipLaunchKernelGGL(k_sell_residual<double>, dim3(blocks), dim3(kBlock), 0, st, A.chunks.p,
-                               A.nChunk, A.uniformW, A.col.p, A.val.p, x, b, (double*)r, (long)A.n, nrhs, norms);
+            launch<double, false>(x, b, r, nrhs, norms, st);
         RACE_HIP(hipGetLastError());
         return OK;
     }

@@ -115,6 +115,34 @@ def test_mpk_highest_matches_full_mpk(oracle, k):
     assert relerr(gt[(k - 1) % 2], ref[(k - 1) % 2]) <= TOL
 
 
+def test_mpk_many_powers(oracle):
+    """k = 20 over 44 level groups: wavefront steps with up to 20 segments,
+    more than the 16 that travel in the kernel arguments, so those steps read
+    the device segment table; full and only-highest against the oracle."""
+    k = 20
+    ce, n, (prp, pcol, pval) = _power_setup(16, k, 64 * 1024.0, 5)
+    assert len(ce.mpkGroups()) - 1 >= 33
+    pval = pval / 6.0  # spectral radius ~1: twenty powers stay O(1)
+    x0 = M.uniform_vec(n, 9)
+    full = np.zeros((k + 1) * n)
+    full[:n] = x0
+    dfull = R.DeviceArray.from_host(full)
+    two = np.zeros(2 * n)
+    two[:n] = x0
+    dtwo = R.DeviceArray.from_host(two)
+    f_full = ce.registerDevice(R.K_MPK, prp, pcol, pval, dfull, power=k)
+    f_two = ce.registerDevice(R.K_MPK_HIGHEST, prp, pcol, pval, dtwo, power=k)
+    print("case mpk k=20", flush=True)
+    ce.executeFunction(f_full)
+    ce.executeFunction(f_two)
+    got = dfull.download().reshape(k + 1, n)
+    ref = oracle.mpk_serial(prp, pcol, pval, x0, k)
+    for p in range(1, k + 1):
+        assert relerr(got[p], ref[p]) <= TOL, p
+    gt = dtwo.download().reshape(2, n)
+    assert np.array_equal(gt[k % 2], got[k]) and np.array_equal(gt[(k - 1) % 2], got[k - 1])
+
+
 def test_mpk_set_power(oracle):
     """setPower(p) on a device MPK computes powers 1..p only; above the
     registered power it is rejected."""
