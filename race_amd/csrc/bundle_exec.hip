@@ -283,14 +283,23 @@ __global__ void __launch_bounds__(BS) k_kacz_bundle(
             __syncthreads();
             for (int w = 0; w < nw; ++w) {
                 if (active && myWave == w) {
+                    // the row's x entries, read once in one batch (no other
+                    // row of this wave writes them)
+                    XT cur[E];
+#pragma unroll
+                    for (int u = 0; u < E; ++u) cur[u] = xl[li[u]];
                     XT dot = XT(0);
 #pragma unroll
-                    for (int u = 0; u < E; ++u) dot += valOf<XT>(lv[u]) * xl[li[u]];
+                    for (int u = 0; u < E; ++u) dot += valOf<XT>(lv[u]) * cur[u];
                     dot = laneSum<TPR>(dot);
                     const XT scale = (XT)(((double)dot - (double)br) / rn);
+                    // write back the entries with a nonzero value only: a
+                    // padding entry may name a column another row updates
 #pragma unroll
-                    for (int u = 0; u < E; ++u)
-                        if (valOf<XT>(lv[u]) != XT(0)) xl[li[u]] -= scale * valOf<XT>(lv[u]);
+                    for (int u = 0; u < E; ++u) {
+                        const XT a = valOf<XT>(lv[u]);
+                        if (a != XT(0)) xl[li[u]] = cur[u] - scale * a;
+                    }
                 }
                 __syncthreads();
             }
@@ -488,10 +497,15 @@ __global__ void __launch_bounds__(BS) k_kacz_bundle_vl(
                 rowL1 = rl0[i + 1];
             }
             for (int w = 0; w < nw; ++w) {
+                // the lane's x entries, read once in one batch and kept
+                // across the barrier (no other row of this wave writes them)
+                XT cur[E];
                 if (active && myWave == w) {
+#pragma unroll
+                    for (int u = 0; u < E; ++u) cur[u] = xl[li[u]];
                     XT dot = XT(0);
 #pragma unroll
-                    for (int u = 0; u < E; ++u) dot += valOf<XT>(lv[u]) * xl[li[u]];
+                    for (int u = 0; u < E; ++u) dot += valOf<XT>(lv[u]) * cur[u];
                     part[t] = dot;
                 }
                 __syncthreads();
@@ -499,9 +513,13 @@ __global__ void __launch_bounds__(BS) k_kacz_bundle_vl(
                     XT rowDot = XT(0);
                     for (int l = rowL0; l < rowL1; ++l) rowDot += part[l];
                     const XT scale = (XT)(((double)rowDot - (double)br) / rn);
+                    // nonzero entries only: a padding entry may name a
+                    // column another row updates
 #pragma unroll
-                    for (int u = 0; u < E; ++u)
-                        if (valOf<XT>(lv[u]) != XT(0)) xl[li[u]] -= scale * valOf<XT>(lv[u]);
+                    for (int u = 0; u < E; ++u) {
+                        const XT a = valOf<XT>(lv[u]);
+                        if (a != XT(0)) xl[li[u]] = cur[u] - scale * a;
+                    }
                 }
                 __syncthreads();
             }
