@@ -36,9 +36,12 @@ config.h):
   * split_zones / ZoneTree / level_recursion: invariants only -- zones cover
     the level range contiguously, every zone has >= minGap levels, same-colour
     zones are distance-k independent (brute force), thread counts add up,
-    effRow equals the critical path recomputed from the leaves.  The exact
-    zone boundaries chosen by the variance heuristic are PARITY UNPINNED
-    against the reference: nothing in the reference tree fixes them.
+    effRow equals the critical path recomputed from the leaves; one known
+    answer fixed by symmetry (equal nnz on every level, a level count that
+    is a multiple of the blocks: the split stays uniform).  Beyond that case
+    the exact zone boundaries chosen by the variance heuristic are PARITY
+    UNPINNED against the reference: nothing in the reference tree fixes
+    them.
   * leaf_calls: with one thread the order is the serial order (textbook
     Gauss-Seidel/Kaczmarz); with T threads every pair of concurrently
     scheduled leaves is checked independent.

@@ -48,6 +48,18 @@ def test_product_load_balancer_matches_oracle(oracle, seed):
     assert a["zone_rows"] == b["zone_rows"]
 
 
+@pytest.mark.parametrize("T,dist,m", [(2, 1, 4), (3, 1, 3), (4, 2, 2), (5, 2, 3), (8, 1, 2), (16, 1, 3)])
+def test_product_load_balancer_uniform_levels_known_answer(T, dist, m):
+    """Equal nnz on every level, T x blocks-per-thread x m levels: the zones
+    and blocks stay the uniform split (zero variance from the start; the
+    same known answer pins the oracle in test_oracle.py)."""
+    bpt = 2  # getBlockPerThread for distance 1 and for distance 2 with two blocks
+    L = T * bpt * m
+    b = R.lb_split(np.full(L, 7), np.full(L, 21), T, 40.0, dist)
+    assert list(b["levelPtr"]) == [k * m for k in range(T * bpt + 1)]
+    assert list(b["scale"]) == [1] * T
+
+
 def test_lb_rejects_bad_arguments():
     with pytest.raises(R.RaceError):
         R.lb_split([], [], 4, 40.0, 1)

@@ -395,6 +395,26 @@ def test_split_zones_invariants(oracle, seed):
     assert sum(r["scale"]) in (T - 1, T, T + 1) or r["threads"] < T
 
 
+@pytest.mark.parametrize("T,dist,m", [(2, 1, 4), (3, 1, 3), (4, 2, 2), (5, 2, 3), (8, 1, 2), (6, 1, 5),
+                                      (7, 2, 4), (16, 1, 3), (10, 2, 2)])
+def test_split_zones_uniform_levels_known_answer(oracle, T, dist, m):
+    """A known answer fixed by symmetry: every level carries the same nnz and
+    the level count is T x blocks-per-thread x m.  The zone split
+    (LB::splitZones, src/lb.cpp:296-391) then closes a zone every T-th of
+    the levels, the blocks of a zone split it evenly, every block carries
+    the same work (zero variance), and no move of the variance heuristic
+    (src/lb.cpp:408-566) can lower that, so the result is the uniform split
+    (tests/test_host_logic.py checks the product's load balancer on the same
+    cases)."""
+    bpt = oracle.block_per_thread(dist)
+    L = T * bpt * m
+    lr, ln = np.full(L, 7), np.full(L, 21)
+    expect = [k * m for k in range(T * bpt + 1)]
+    a = oracle.split_zones(lr, ln, T, 40.0, dist)
+    assert a["threads"] == T and list(a["levelPtr"]) == expect
+    assert a["scale"] == [1] * T
+
+
 def test_level_recursion_thread_accounting(oracle):
     """nthreads of every inner node = sum over threads of the max child
     nthreads per colour; the root reports the available threads."""
