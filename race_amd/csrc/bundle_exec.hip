@@ -26,11 +26,11 @@ namespace {
 
 // matrix value storage: double, float, or bfloat16 bits (uint16_t).  The
 // prologue keeps a value as loaded (ValBits) and converts it where it is
-// used.  A bfloat16 shift placed right after its conditional load makes the
-// wave wait for that load before it issues the next entry's loads (13
-// round trips in a row); the empty asm statement, which emits no
-// instruction, keeps the compiler from folding the shift back into the
-// conditional load.
+// used, after every load of the prologue has been issued.  A bfloat16 shift
+// next to its load makes the wave wait for that load before it issues the
+// next entry's loads (13 round trips in a row); the empty asm statement,
+// which emits no instruction, keeps the compiler from moving the shift back
+// next to the load.
 template <typename VT>
 struct ValBits {
     using T = VT;
@@ -114,9 +114,9 @@ struct BundleVec {
 // then, all issued together, its entry block, row ids, inverse diagonals,
 // wave numbers and outside-column ids; then the gathers of x and b.  Row
 // and column ids are unsigned: widening them for the 64-bit address is then
-// a zero high half, which does not wait for the load as a sign extension
-// inside the conditional load would.  A phase whose chains are single
-// bundles (bundle0 >= 0) skips the chain table.
+// a zero high half, not a sign extension that has to wait for the loaded
+// value.  A phase whose chains are single bundles (bundle0 >= 0) skips the
+// chain table.
 template <int BS, int TPR, int E, typename VT, typename XT, bool ONE_RHS, bool BATCH>
 __global__ void __launch_bounds__(BS) k_gs_bundle(
     const int* __restrict__ bSlot, const int* __restrict__ bWave, const long* __restrict__ bOff,
