@@ -21,6 +21,9 @@
  *                                                      examples/src/helpers/kernels.cpp:749-777
  *   or_run_calls       the leaf calls of recursiveCall  src/functionManager.cpp:199-265
  *                      (serialised; concurrent leaves are independent)
+ *   or_run_phases      the same phase by phase, the leaves of a phase on
+ *                      OpenMP threads (RACE's own execution on the host, the
+ *                      CPU baseline)
  *
  * How each function is pinned (the reference cannot be built here; see the
  * test files for the checks):
@@ -37,6 +40,9 @@
  */
 #include <stdint.h>
 #include <stdlib.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <string.h>
 #include <math.h>
 
@@ -385,4 +391,35 @@ void or_run_calls(int kind, int n, const int *rp, const int *col,
         default: break;
         }
     }
+}
+
+/* The leaves of each phase (phase_ptr[p] .. phase_ptr[p+1] into calls) run
+ * concurrently on OpenMP threads, phases one after the other: how RACE runs
+ * them on the host (recursiveCall, src/functionManager.cpp:199-265).  Leaves
+ * of one phase are distance-k independent, so the result equals
+ * or_run_calls over the same calls in order. */
+void or_run_phases(int kind, int n, const int *rp, const int *col, const double *val, double *x,
+                   double *b, int nphases, const int *phase_ptr, const int *calls, int nrhs,
+                   int nthreads)
+{
+#ifdef _OPENMP
+#pragma omp parallel num_threads(nthreads)
+#endif
+    for (int p = 0; p < nphases; ++p) {
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 1)
+#endif
+        for (int c = phase_ptr[p]; c < phase_ptr[p + 1]; ++c)
+            or_run_calls(kind, n, rp, col, val, x, b, 1, calls + 2 * c, nrhs);
+    }
+    (void)nthreads;
+}
+
+int or_max_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
 }

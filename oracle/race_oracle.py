@@ -83,7 +83,7 @@ _f64p = ctypes.POINTER(ctypes.c_double)
 def build():
     """Compile race_oracle.c into oracle/librace_oracle.so."""
     src = os.path.join(_HERE, "race_oracle.c")
-    subprocess.check_call(["gcc", "-O2", "-std=c11", "-fPIC", "-shared", "-o", _LIB, src, "-lm"])
+    subprocess.check_call(["gcc", "-O2", "-std=c11", "-fopenmp", "-fPIC", "-shared", "-o", _LIB, src, "-lm"])
 
 
 def lib():
@@ -102,6 +102,9 @@ def lib():
         L.or_symmetrize.argtypes = [ctypes.c_int, _i32p, _i32p, _i32p, _i32p, ctypes.c_long]
         L.or_run_calls.argtypes = [ctypes.c_int, ctypes.c_int, _i32p, _i32p, _f64p, _f64p, _f64p,
                                    ctypes.c_int, _i32p, ctypes.c_int]
+        L.or_run_phases.argtypes = [ctypes.c_int, ctypes.c_int, _i32p, _i32p, _f64p, _f64p, _f64p,
+                                    ctypes.c_int, _i32p, _i32p, ctypes.c_int, ctypes.c_int]
+        L.or_max_threads.restype = ctypes.c_int
         L.or_mpk_serial.argtypes = [ctypes.c_int, _i32p, _i32p, _f64p, _f64p, ctypes.c_int]
         L.or_mpk_highest_serial.argtypes = [ctypes.c_int, _i32p, _i32p, _f64p, _f64p, ctypes.c_int]
         _lib = L
@@ -512,6 +515,34 @@ class ZoneTree:
         return res["threads"] == request
 
 
+def split_threads(weights, total, parent_threads):
+    """The thread split of LevelRecursion::calculateIdealNthreads
+    (src/level_recursion.cpp:165-245): child group i (the blocks of one
+    thread of the parent's split) gets floor(weights[i] / total *
+    parent_threads) threads, weights being row counts or nnz (ints, total =
+    the parent range's); a group that got 0 ranks with remainder 1, the
+    others with their fractional part, and the leftover threads go one each
+    to the groups in descending rank order (stable sort, ties keep group
+    order)."""
+    rem = []
+    out = []
+    remaining = parent_threads
+    for w in weights:
+        my_d = (int(w) / float(total)) * parent_threads
+        my = int(my_d)
+        out.append(my)
+        rem.append((my_d - my) if my != 0 else 1.0)
+        remaining -= my
+    if remaining > 0:
+        order = stable_argsort_desc(rem)
+        r = 0
+        while remaining != 0:
+            out[order[r]] += 1
+            remaining -= 1
+            r += 1
+    return out
+
+
 def level_recursion(graph, nthreads, dist, d2=TWO_BLOCK, lb_target=LB_NNZ, eff_list=None,
                     lvl_threads=None):
     """LevelRecursion(graph, T, dist).levelBalancing(); eff_list mirrors
@@ -549,30 +580,16 @@ def level_recursion(graph, nthreads, dist, d2=TWO_BLOCK, lb_target=LB_NNZ, eff_l
             s, e = T[p].valueZ[psub], T[p].valueZ[psub + 1]
             parent_row = e - s
             parent_nnz = row_nnz_sum(s, e) if lb_target == LB_NNZ else 0
-            rem = [0.0] * nt
-            remaining = parent_threads
+            weights = []
             for i in range(nt):
                 first = T[ch[2 * psub] + i * bpt].valueZ
                 last = T[ch[2 * psub] + bpt * (i + 1) - 1].valueZ
                 ss, ee = first[0], last[-1]
-                w = (ee - ss) / float(parent_row)
-                if lb_target == LB_NNZ:
-                    w = row_nnz_sum(ss, ee) / float(parent_nnz)
-                my_d = w * parent_threads
-                my = int(my_d)
+                weights.append(row_nnz_sum(ss, ee) if lb_target == LB_NNZ else ee - ss)
+            total = parent_nnz if lb_target == LB_NNZ else parent_row
+            for i, my in enumerate(split_threads(weights, total, parent_threads)):
                 for b in range(bpt):
                     T[ch[2 * psub] + bpt * i + b].idealNthreadsZ = my
-                rem[i] = (my_d - my) if my != 0 else 1.0
-                remaining -= my
-            if remaining > 0:
-                order = stable_argsort_desc(rem)
-                r = 0
-                while remaining != 0:
-                    tgt = order[r]
-                    for b in range(bpt):
-                        T[ch[2 * psub] + bpt * tgt + b].idealNthreadsZ += 1
-                    remaining -= 1
-                    r += 1
         else:
             my = lvl_thr(curr_lvl)
             for i in range(nt):
@@ -674,6 +691,34 @@ def run_calls(kind, rowPtr, col, val, x, b, calls, nrhs=1):
                        len(c) // 2, _ip(c), nrhs)
 
 
+def run_phases(kind, rowPtr, col, val, x, b, phase_ptr, calls, nrhs=1, nthreads=None):
+    """Execute kernel `kind` phase by phase, the leaf calls of a phase on
+    OpenMP threads (x, b modified in place); calls is a flat (start, end)
+    array, phase p owns calls phase_ptr[p] .. phase_ptr[p+1]."""
+    rowPtr = np.ascontiguousarray(rowPtr, np.int32)
+    col = np.ascontiguousarray(col, np.int32)
+    val = np.ascontiguousarray(val, np.float64)
+    c = np.ascontiguousarray(np.asarray(calls, np.int32).reshape(-1))
+    pp = np.ascontiguousarray(phase_ptr, np.int32)
+    n = len(rowPtr) - 1
+    nt = int(nthreads or lib().or_max_threads())
+    lib().or_run_phases(kind, n, _ip(rowPtr), _ip(col), _dp(val), _dp(x), _dp(b), len(pp) - 1, _ip(pp),
+                        _ip(c), nrhs, nt)
+    return nt
+
+
+def phase_calls(zt, rev=False):
+    """phases(zt) flattened for run_phases: (phase_ptr, calls)."""
+    ph = phases(zt, rev)
+    pp = [0]
+    flat = []
+    for p in ph:
+        for a, b in p:
+            flat += [a, b]
+        pp.append(len(flat) // 2)
+    return np.array(pp, np.int32), np.array(flat, np.int32)
+
+
 def mpk_serial(rowPtr, col, val, x0, k):
     n = len(rowPtr) - 1
     x = np.zeros((k + 1) * n, np.float64)
@@ -699,11 +744,21 @@ def mpk_highest_serial(rowPtr, col, val, x0, k):
 # --------------------------------------------------------------------------
 # MPK partition, stage 0 (no hopeless regions)
 # --------------------------------------------------------------------------
-def mpk_partition(graph, k, cache_bytes, safety=2.0, cutoff=50):
+def mpk_partition(graph, k, cache_bytes, safety=2.0, cutoff=50, mtx_type="N"):
     """Stage-0 level groups of mtxPower::findPartition/consolidatePartition.
     Returns dict(levelPtr=[level row offsets], groupPtr=[row offsets of the
-    consolidated groups], hopeless=[levels with factor>cutoff])."""
-    lr, ln = graph.traverse(POWER, 0, graph.n)
+    consolidated groups], hopeless=[levels with factor>cutoff]).
+    mtx_type "L" / "U": no BFS, row i is level i (n-1-i for "U"), and the
+    stable sort by level keeps (reverses) the current order
+    (Traverse::calculateDistance src/traverse_GAP_graphSoA.cpp:316-335)."""
+    if mtx_type in ("L", "U"):
+        if mtx_type == "U":
+            graph.perm = graph.perm[::-1].copy()
+            graph.inv[graph.perm] = np.arange(graph.n, dtype=np.int32)
+        lr = np.ones(graph.n, np.int32)
+        ln = (graph.rowPtr[graph.perm + 1] - graph.rowPtr[graph.perm]).astype(np.int32)
+    else:
+        lr, ln = graph.traverse(POWER, 0, graph.n)
     total_level = len(lr)
     level_ptr = [0]
     for r in lr:

@@ -437,11 +437,18 @@ def test_level_recursion_thread_accounting(oracle):
 
 
 def test_one_thread_is_the_serial_order(oracle):
+    """One RACE thread: the root's two blocks (src/lb.cpp:384-391) run one
+    after the other, so the non-empty leaf calls tile [0, n) in row order
+    and concatenate to the serial sweep; reversed, they tile it backwards."""
     rp, col = M.stencil27(6)
     G = oracle.Graph(rp, col, symm_hint=True)
     zt = oracle.level_recursion(G, 1, 1)
-    assert oracle.leaf_calls(zt) == [(0, G.n)] or sum(b - a for a, b in oracle.leaf_calls(zt)) == G.n
-    assert [(a, b) for a, b in oracle.leaf_calls(zt) if b > a][0][0] == 0
+    calls = [(a, b) for a, b in oracle.leaf_calls(zt) if b > a]
+    assert calls[0][0] == 0 and calls[-1][1] == G.n
+    assert all(b == c for (_, b), (c, _) in zip(calls, calls[1:]))
+    rcalls = [(a, b) for a, b in oracle.leaf_calls(zt, rev=True) if a > b]
+    assert rcalls[0][0] == G.n - 1 and rcalls[-1][1] == -1
+    assert all(b == c for (_, b), (c, _) in zip(rcalls, rcalls[1:]))
 
 
 def test_mpk_partition_respects_cache_bound(oracle):
@@ -462,3 +469,30 @@ def test_mpk_partition_respects_cache_bound(oracle):
     for a, b, c in zip(gl, gl[1:], gl[2:]):
         assert sum(elem[a:b + 1]) >= part["cache_elem"]
     assert part["groupPtr"][-1] == G.n
+
+
+@pytest.mark.parametrize("kind,dist,T", [("gs", 1, 24), ("kacz", 2, 24), ("symmspmv", 2, 40)])
+def test_parallel_phases_equal_the_serial_leaf_order(oracle, kind, dist, T):
+    """or_run_phases (the leaves of a colour phase on OpenMP threads, the
+    CPU baseline) gives bit for bit what the serial leaf order gives: leaves
+    of one phase are independent, so their interleaving cannot matter."""
+    rp, col = M.stencil27(12) if dist == 1 else M.stencil7(16)
+    val = M.gs_values(rp, col, seed=3)
+    n = len(rp) - 1
+    G = oracle.Graph(rp, col, symm_hint=True)
+    zt = oracle.level_recursion(G, T, dist)
+    prp, pcol, pval = M.make_diag_first(*M.permute(rp, col, val, G.perm, G.inv))
+    okind = {"gs": oracle.K_GS, "kacz": oracle.K_KACZ, "symmspmv": oracle.K_SYMMSPMV}[kind]
+    if kind == "symmspmv":
+        prp, pcol, pval = M.upper_triangle(prp, pcol, pval)
+    b = M.uniform_vec(n, 4, nrhs=2)
+    x0 = M.uniform_vec(n, 5, nrhs=2)
+    for rev in (False, True):
+        xa, ba = x0.copy(), b.copy()
+        oracle.run_calls(okind, prp, pcol, pval, xa, ba, oracle.leaf_calls(zt, rev), nrhs=2)
+        xb, bb = x0.copy(), b.copy()
+        pp, calls = oracle.phase_calls(zt, rev)
+        assert len(pp) > 3
+        nt = oracle.run_phases(okind, prp, pcol, pval, xb, bb, pp, calls, nrhs=2, nthreads=8)
+        assert nt == 8
+        assert np.array_equal(xa, xb) and np.array_equal(ba, bb)
