@@ -36,15 +36,36 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def host_info():
+    """CPU model and core count of this host (lscpu, else /proc/cpuinfo)."""
+    import subprocess
+    info = {"nproc": os.cpu_count()}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core", "CPU(s)"):
+                info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    try:
+        info["affinity_cpus"] = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    return info
+
+
 def gs_bytes(n, nnz, sv=8, sx=8, si=4):
     # values + column indices, row pointer, x read + x write + b read
     return nnz * (sv + si) + n * (si + 3 * sx)
 
 
-def bench_symmspmv(R, M, g=64, threads=64, reps=200):
+def bench_symmspmv(R, M, g=64, threads=64, reps=200, O=None):
     """SymmSpMV b += A x, 7-pt g^3 (diag 6, off -1), upper triangle with the
     diagonal first, RACE distance-2 colouring.  Hot: back-to-back calls
-    (working set in the caches); cold: each call after a 1 GiB copy."""
+    (working set in the caches); cold: each call after a 1 GiB copy.  With
+    the oracle O: the FP64 result against the oracle's SymmSpMV in its own
+    RACE order, and the FP32 / BF16 errors against that."""
     rp, col = M.stencil7(g)
     val = M.laplace7_values(rp, col)
     n = len(rp) - 1
@@ -79,6 +100,18 @@ def bench_symmspmv(R, M, g=64, threads=64, reps=200):
     db.upload(np.zeros(n))
     ce.executeFunction(fid)
     ref = db.download()
+    parity = None
+    if O is not None:
+        G = O.Graph(rp, col, symm_hint=True)
+        zt = O.level_recursion(G, threads, O.TWO)
+        bo = np.zeros(n)
+        O.run_calls(O.K_SYMMSPMV, urp, ucol, uval, M.uniform_vec(n, 1234), bo, O.leaf_calls(zt))
+        perr = float(np.linalg.norm(ref - bo) / np.linalg.norm(bo))
+        peq = bool(np.array_equal(G.perm, perm))
+        parity = {"rel_err_vs_oracle": perr, "perm_equal_oracle": peq, "tol": 1e-12,
+                  "ok": bool(peq and perr <= 1e-12)}
+        log(f"parity: SymmSpMV {g}^3 FP64 rel err vs oracle {perr:.2e}, perm equal {peq}")
+        ref = bo
     # FP32 / BF16 values with FP32 vectors and accumulation: same calls,
     # bytes with the smaller values and vectors, error against the FP64 result
     low = {}
@@ -106,7 +139,7 @@ def bench_symmspmv(R, M, g=64, threads=64, reps=200):
         low[name] = {"bytes_per_call": lbyts, "hot_us": round(lh * 1e3, 2), "hot_gbs": round(lbyts / lh / 1e6, 1),
                      "cold_us": round(lc * 1e3, 2), "cold_gbs": round(lbyts / lc / 1e6, 1),
                      "speedup_hot_vs_fp64": round(hot / lh, 3), "speedup_cold_vs_fp64": round(cold / lc, 3),
-                     "rel_err_vs_fp64": err}
+                     ("rel_err_vs_oracle" if parity else "rel_err_vs_fp64"): err}
         log(f"SymmSpMV {g}^3 {name}: hot {low[name]['hot_us']} us, cold {low[name]['cold_us']} us, "
             f"err {err:.1e}")
         del lx, lb
@@ -116,15 +149,18 @@ def bench_symmspmv(R, M, g=64, threads=64, reps=200):
            "cold_us": round(cold * 1e3, 2), "cold_gbs": round(byts / cold / 1e6, 1),
            "cold_method": "1 GiB device copy enqueued right before each timed call",
            "cold_frac_hbm": round(byts / cold / 1e6 / HBM_PEAK_GBS, 4),
-           "low_precision": low}
+           "low_precision": low, "parity": parity}
     log(f"SymmSpMV {g}^3: hot {out['hot_us']} us ({out['hot_gbs']} GB/s), cold {out['cold_us']} us "
         f"({out['cold_gbs']} GB/s)")
     return out
 
 
-def bench_mpk(R, M, g=256, k=8, cache_mb=512.0, reps=10):
+def bench_mpk(R, M, g=256, k=8, cache_mb=256.0, reps=10, O=None):
     """Matrix powers x_q = A x_{q-1}, q = 1..k, 7-pt g^3 (diag 6, off -1,
-    jitter 10 %), RACE level groups for a cache of cache_mb MB, safety 2."""
+    jitter 10 %), RACE level groups for a cache of cache_mb MB (the 256 MB
+    Infinity Cache), safety 2.  With the oracle O: the permutation and level
+    groups against the oracle's, every FP64 power against the oracle's
+    repeated SpMV, and the FP32 / BF16 errors against that."""
     rp, col = M.stencil7(g)
     val = M.jittered_values(rp, col, 0.1, seed=9)
     n = len(rp) - 1
@@ -133,6 +169,12 @@ def bench_mpk(R, M, g=256, k=8, cache_mb=512.0, reps=10):
     ce.RACEColor(k, cache_mb * 1e6)
     groups = len(ce.mpkGroups()) - 1
     perm, inv = ce.getPerm(), ce.getInvPerm()
+    part_equal = None
+    if O is not None:
+        G = O.Graph(rp, col, symm_hint=True)
+        part = O.mpk_partition(G, k, cache_mb * 1e6)
+        part_equal = bool(np.array_equal(G.perm, perm) and np.array_equal(ce.mpkGroups(), part["groupPtr"]))
+        del G, part
     prp, pcol, pval = M.permute(rp, col, val, perm, inv)
     del rp, col, val
     xs = np.zeros((k + 1) * n)
@@ -163,6 +205,16 @@ def bench_mpk(R, M, g=256, k=8, cache_mb=512.0, reps=10):
     # FP32 / BF16 values, FP32 powers and accumulation: time and the error
     # of every power against the FP64 path (itself within 1e-12 of the oracle)
     ref = dx.download().reshape(k + 1, n)
+    parity = None
+    if O is not None:
+        oref = O.mpk_serial(prp, pcol, pval, ref[0], k)
+        perrs = [float(np.linalg.norm(ref[p] - oref[p]) / np.linalg.norm(oref[p])) for p in range(1, k + 1)]
+        parity = {"rel_err_per_power_vs_oracle": perrs, "groups_and_perm_equal_oracle": part_equal,
+                  "tol": 1e-12, "ok": bool(part_equal and max(perrs) <= 1e-12)}
+        log(f"parity: MPK {g}^3 k={k} FP64 worst power rel err vs oracle {max(perrs):.2e}, "
+            f"groups/perm equal {part_equal}")
+        ref = oref
+        del oref
     low = {}
     for name, vp, sv in (("fp32", R.FP32, 4), ("bf16", R.BF16, 2)):
         xl = np.zeros((k + 1) * n, np.float32)
@@ -180,7 +232,8 @@ def bench_mpk(R, M, g=256, k=8, cache_mb=512.0, reps=10):
         ml = ce.timerEnd() / reps
         leff = k * (nnz * (sv + 4) + n * (4 + 2 * 4))
         low[name] = {"ms_per_call": round(ml, 4), "effective_gbs": round(leff / ml / 1e6, 1),
-                     "speedup_vs_fp64": round(ms / ml, 3), "rel_err_per_power_vs_fp64": errs}
+                     "speedup_vs_fp64": round(ms / ml, 3),
+                     ("rel_err_per_power_vs_oracle" if parity else "rel_err_per_power_vs_fp64"): errs}
         log(f"MPK {g}^3 k={k} {name}: {ml:.3f} ms/call ({low[name]['effective_gbs']} GB/s), "
             f"error power 1 / {k}: {errs[0]:.1e} / {errs[-1]:.1e}")
         del dl
@@ -200,7 +253,7 @@ def bench_mpk(R, M, g=256, k=8, cache_mb=512.0, reps=10):
            "only_highest_ms": round(msh, 4), "only_highest_effective_gbs": round(eff / msh / 1e6, 1),
            "spmv_ms": round(ms1, 4),
            "spmv_gbs": round((nnz * 12 + n * 20) / ms1 / 1e6, 1), "speedup_vs_k_spmv": round(k * ms1 / ms, 3),
-           "low_precision": low}
+           "low_precision": low, "parity": parity}
     log(f"MPK {g}^3 k={k}: {ms:.3f} ms/call, effective {out['effective_gbs']} GB/s; "
         f"only-highest {msh:.3f} ms; SpMV {ms1:.3f} ms ({out['spmv_gbs']} GB/s)")
     return out
@@ -450,7 +503,7 @@ def main():
     prp, pcol, pval = M.permute(rp, col, val, perm, inv)
     prp, pcol, pval = M.make_diag_first(prp, pcol, pval)
     pb = M.permute_vec(b, perm)
-    del rp, col, val
+    del val  # the pattern stays for the oracle's own colouring
 
     dx = R.DeviceArray.from_host(np.zeros(n))
     db = R.DeviceArray.from_host(pb)
@@ -477,6 +530,7 @@ def main():
         ce.executeAsync(fid, True)
     ms = ce.timerEnd()
     R.interface.L.load().race_device_synchronize()
+    x_gpu = dx.download()  # after warmup + steps fwd+bwd steps from x = 0
     ms_step = ms / args.steps
     bytes_step = 2 * gs_bytes(n, nnz)
     gbs = bytes_step / (ms_step * 1e-3) / 1e9
@@ -514,30 +568,67 @@ def main():
 
     extra = None
     if not args.no_extra:
-        extra = {"symmspmv": bench_symmspmv(R, M), "mpk": bench_mpk(R, M), "spmtv": bench_spmtv(R, M)}
+        Ox = None
+        if not args.no_cpu:
+            from oracle import race_oracle as Ox
+            Ox.build()
+        extra = {"symmspmv": bench_symmspmv(R, M, O=Ox), "mpk": bench_mpk(R, M, O=Ox),
+                 "spmtv": bench_spmtv(R, M)}
 
-    # CPU oracle beside it: the same sweeps in RACE order, single thread
+    # CPU oracle beside it, and the parity of the timed GPU state: the oracle
+    # builds its own RACE colouring of the same matrix (LevelRecursion in the
+    # oracle restatement), runs the same warmup + steps fwd+bwd sweeps from
+    # x = 0 in its RACE order on one thread and compares; then the same sweeps
+    # phase by phase with the leaves of a phase on all host cores (RACE's own
+    # OpenMP execution) for the CPU baseline.
     cpu = None
+    parity = None
     if not args.no_cpu:
         from oracle import race_oracle as O
         O.build()
-        pp_f, lo_f, hi_f = ce.schedule(False)
-        pp_r, lo_r, hi_r = ce.schedule(True)
-        fcalls = np.stack([lo_f, hi_f], axis=1).reshape(-1)
-        rcalls = np.stack([hi_r - 1, lo_r - 1], axis=1).reshape(-1)
-        xo = np.zeros(n)
-        sweeps = 0
         t0 = time.time()
-        while time.time() - t0 < args.cpu_seconds or sweeps == 0:
+        G = O.Graph(rp, col, symm_hint=True)
+        zt = O.level_recursion(G, args.threads, O.ONE)
+        t_orc = time.time() - t0
+        perm_equal = bool(np.array_equal(G.perm, perm))
+        fcalls = np.asarray(O.leaf_calls(zt), np.int32).reshape(-1)
+        rcalls = np.asarray(O.leaf_calls(zt, True), np.int32).reshape(-1)
+        xo = np.zeros(n)
+        nsteps = args.warmup + args.steps
+        t0 = time.time()
+        for _ in range(nsteps):
             O.run_calls(O.K_GS, prp, pcol, pval, xo, pb, fcalls)
             O.run_calls(O.K_GS, prp, pcol, pval, xo, pb, rcalls)
-            sweeps += 1
-        tc = (time.time() - t0) / sweeps
-        cpu = {"value": round(bytes_step / tc / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
-               "sample": f"{sweeps} fwd+bwd GS steps on the full {g}^3 27-pt matrix in RACE order "
-                         f"(oracle C restatement, 1 thread, {tc * 1e3:.0f} ms/step)"}
-        # parity of the timed GPU state against the oracle after the same number of steps
-        log(f"CPU oracle: {tc * 1e3:.1f} ms/step")
+        t1 = (time.time() - t0) / nsteps
+        err = float(np.linalg.norm(x_gpu - xo) / np.linalg.norm(xo))
+        parity = {"gs_rel_err_vs_oracle": err, "steps": nsteps, "perm_equal_oracle": perm_equal,
+                  "oracle_threads": zt.available_threads, "gpu_threads": ce.getNumThreads(),
+                  "tol": 1e-12, "ok": bool(perm_equal and err <= 1e-12)}
+        log(f"parity: GS {g}^3 after {nsteps} steps, rel err vs oracle {err:.2e}, perm equal {perm_equal} "
+            f"(oracle colouring {t_orc:.1f}s)")
+        ppf, cf = O.phase_calls(zt)
+        ppr, cr = O.phase_calls(zt, True)  # reversed leaves (hi-1, lo-1), colours in reverse
+        xo2 = np.zeros(n)
+        nt = O.run_phases(O.K_GS, prp, pcol, pval, xo2, pb, ppf, cf)
+        O.run_phases(O.K_GS, prp, pcol, pval, xo2, pb, ppr, cr)
+        times = []
+        t0 = time.time()
+        while time.time() - t0 < args.cpu_seconds or len(times) < 3:
+            ts = time.time()
+            O.run_phases(O.K_GS, prp, pcol, pval, xo2, pb, ppf, cf)
+            O.run_phases(O.K_GS, prp, pcol, pval, xo2, pb, ppr, cr)
+            times.append(time.time() - ts)
+        tc = float(np.median(times))
+        cpu = {"value": round(bytes_step / tc / 1e9, 3), "unit": "GB/s", "cores": nt, "kind": "port",
+               "sample": f"{len(times)} fwd+bwd GS steps on the full {g}^3 27-pt matrix, the oracle's RACE "
+                         f"colouring ({zt.available_threads} threads), leaves of a phase on {nt} OpenMP "
+                         f"threads, median {tc * 1e3:.1f} ms/step (oracle C restatement)",
+               "host": host_info(),
+               "single_thread": {"value": round(bytes_step / t1 / 1e9, 3), "ms_per_step": round(t1 * 1e3, 1),
+                                 "steps": nsteps}}
+        log(f"CPU oracle: {tc * 1e3:.1f} ms/step on {nt} threads, {t1 * 1e3:.1f} ms/step on 1 thread")
+        del xo, xo2
+    del rp, col
 
     # HBM bytes per launch measured by the FETCH_SIZE/WRITE_SIZE passes recorded
     # under profiles/ for this exact configuration (null when it differs)
@@ -568,11 +659,17 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(gbs, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 5), "traffic": traffic},
         "cpu_baseline": cpu,
+        "parity": parity,
         "race_tuner": race_tuner,
         "mixed_precision": mixed,
         "other_kernels": extra,
     }
     print(json.dumps(out), flush=True)
+    bad = [k for k, v in (("gs", parity), ("symmspmv", (extra or {}).get("symmspmv", {}).get("parity")),
+                          ("mpk", (extra or {}).get("mpk", {}).get("parity"))) if v and not v.get("ok")]
+    if bad:
+        log(f"PARITY FAILED for {bad}: the GPU result differs from the oracle")
+        sys.exit(3)
 
 
 if __name__ == "__main__":

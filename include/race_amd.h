@@ -79,7 +79,10 @@ typedef void (*race_color_fn)(int start, int end, void *args);
 typedef void (*race_power_fn)(int start, int end, int pow, int sub_pow, int numa, void *args);
 /* Interface::registerFunction(void(*)(int,int,void*), void*)
  * (include/interface.h:158, src/interface.cpp:397-420).  start>end asks for
- * reverse order, as in the reference. */
+ * reverse order, as in the reference.  Host callbacks run on OpenMP threads:
+ * the leaves of one colour phase in parallel (min(RACE threads, OpenMP
+ * threads) of them), phases in order; power callbacks on row slices of the
+ * independent (level group, power) units of a wavefront step. */
 int race_register_function(race_handle h, race_color_fn f, void *args, int *id);
 /* Interface::registerFunction(void(*)(int,int,int,int,int,void*), void*, power,
  * subPower, numaSplit) (include/interface.h:163, src/interface.cpp:435-458) */
@@ -126,8 +129,14 @@ typedef struct {
     int nrhs;            /* right-hand sides, vectors are column-major v[r + k*nrow] */
     void *x;             /* device pointer */
     void *b;             /* device pointer (unused for MPK) */
-    int power;           /* MPK: number of powers; x holds (power+1)*nrow entries
-                            (RACE_K_MPK_HIGHEST: 2*nrow) */
+    int power;           /* MPK: number of powers; x holds (power*sub_power+1)*nrow
+                            entries (RACE_K_MPK_HIGHEST: 2*nrow) */
+    int sub_power;       /* MPK: sub-powers per power (0 or 1: none).  The reference's
+                            power executor calls the kernel for every (pow, subPow) pair
+                            in order (registerFunction(..., power, subPower),
+                            include/interface.h:153-157, src/interface.cpp:435-458); on
+                            the device each pair is one product with A, so product t =
+                            (pow-1)*sub_power + subPow lands in slot t of x */
 } race_kernel_desc;
 
 /* Typed device registration: the GPU counterpart of registerFunction; the
@@ -187,6 +196,43 @@ int race_execute_async(race_handle h, int id, int rev, void *stream);
  * 1..power; power may not exceed the power it was registered with. */
 int race_set_power(race_handle h, int id, int power);
 int race_get_power(race_handle h, int id, int *power);
+/* Interface::setSubPower / getSubPower (include/interface.h:168-170,
+ * src/interface.cpp:579-592): sub-powers per power of a POWER function; a
+ * device MPK accepts power * sub_power up to its registered products. */
+int race_set_sub_power(race_handle h, int id, int sub_power);
+int race_get_sub_power(race_handle h, int id, int *sub_power);
+/* Interface::setSerial / unsetSerial (include/interface.h:171-172,
+ * src/interface.cpp:594-602, FuncManager::setSerial
+ * src/functionManager.cpp:1561-1580): serial != 0 runs a host callback
+ * function on one thread, 0 on the handle's threads again.  Device
+ * functions have no host threads (accepted, no effect). */
+int race_set_serial(race_handle h, int id, int serial);
+/* Interface::resetTime (src/interface.cpp:649-652) clears the zone tree's
+ * per-leaf timers; no timers are kept here: accepted, no effect. */
+int race_reset_time(race_handle h);
+/* Interface::sleep (src/interface.cpp:817-822) parks RACE's own thread pool,
+ * which exists only without RACE_KERNEL_THREAD_OMP; host callbacks run on
+ * OpenMP threads here, as in the reference's default build: no effect. */
+int race_sleep(race_handle h);
+/* Interface::pinThread (src/interface.cpp:845-848): pins the calling thread
+ * to the (thread_id * smt mod n)-th CPU of the process's affinity mask
+ * (RACE_ERR_PIN when the OS refuses). */
+int race_pin_thread(race_handle h, int thread_id);
+/* Interface::numaInitRowPtr / numaInitMtxVec, both overloads
+ * (src/interface.cpp:852-1040): first-touch initialisation through a power
+ * callback registered with power 1 and executed once (it takes a function id,
+ * as in the reference): rowPtr[r] = rowPtr[r+1] = 0; val, col of each row and
+ * x[r], x[nrow + r] = 0; the _local forms index domain 0 of the per-domain
+ * arrays (one device = one domain).  POWER handles only
+ * (RACE_ERR_NOT_IMPLEMENTED otherwise, "NUMA not implemented for
+ * coloring"). */
+int race_numa_init_row_ptr(race_handle h, int *rowPtr);
+int race_numa_init_row_ptr_local(race_handle h, int **rowPtr);
+int race_numa_init_mtx_vec(race_handle h, int *rowPtr, int *col, double *val, double *x, int power);
+int race_numa_init_mtx_vec_local(race_handle h, int **rowPtr, int **col, double **val, int power);
+/* Interface::getNumaSplitting (src/interface.cpp:1042-1052): row boundaries
+ * of the NUMA domains; one device is one domain, [0, nrow] (len 2). */
+int race_get_numa_splitting(race_handle h, int *split, int cap, int *len);
 /* Interface::tuneFunction (src/interface.cpp:605-647): best power by timing */
 int race_tune_function(race_handle h, int id, int rev, int *best_power);
 /* Interface::checkPowerCoverage (src/interface.cpp:1060-1095): returns
@@ -246,6 +292,12 @@ int race_get_preprocess_stats(race_handle h, double *bfs_ms, int *traversals);
 int race_lb_split(const int *level_row, const int *level_nnz, int nlevels, int nthreads,
                   double efficiency, int dist, int d2_method, int lb_target, int cap,
                   int *nblocks, int *zone_rows, int *level_ptr, int *nthreads_out, int *scale);
+
+/* The thread split of LevelRecursion::calculateIdealNthreads
+ * (src/level_recursion.cpp:149-263) for one partitioned range: weights[i]
+ * is the nnz (or row count) of child group i, total the range's; out[i] =
+ * the ideal thread count of group i. */
+int race_split_threads(const int *weights, int ngroups, int total, int parent_threads, int *out);
 
 /* ---- batches: one function on many matrices (no counterpart in the
  * reference) ----

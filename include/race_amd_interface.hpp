@@ -9,6 +9,7 @@
 #define RACE_AMD_INTERFACE_HPP
 
 #include <cstdio>
+#include <cstdlib>
 #include <functional>
 #include <memory>
 #include <string>
@@ -49,8 +50,11 @@ class Interface {
     RACE_error RACEColor(int highestPower, double cacheSize, double safetyFactor = 2,
                          std::string mtxType = "N", int highestSubPower = 1) {
         if (!h_) return RACE_ERR_INVALID_ARG;
-        return (RACE_error)race_color_power(h_, highestPower, cacheSize, safetyFactor,
-                                            mtxType.empty() ? 'N' : mtxType[0], highestSubPower);
+        if (mtxType.size() != 1) return RACE_ERR_INVALID_ARG;
+        RACE_error e = (RACE_error)race_color_power(h_, highestPower, cacheSize, safetyFactor, mtxType[0],
+                                                    highestSubPower);
+        if (e == RACE_SUCCESS) highestSubPower_ = highestSubPower;
+        return e;
     }
     RACE_error RACEColor() { return h_ ? (RACE_error)race_color(h_) : RACE_ERR_INVALID_ARG; }
 
@@ -137,12 +141,19 @@ class Interface {
                                              subPower, numaSplit, &id);
         return s == RACE_SUCCESS ? id : s;
     }
-    // simplified power callback f(start, end, pow, numa, args) when subPower == 1
-    int registerFunction(void (*f)(int, int, int, int, void*), void* args, int power = 1,
+    // simplified power callback f(start, end, pow, numa, args), only when
+    // highestSubPower == 1 (src/interface.cpp:468-485, else
+    // RACE_ERR_INCOMPATIBILITY)
+    int registerFunction(std::function<void(int, int, int, int, void*)> f, void* args, int power = 1,
                          int numaSplit = false) {
+        if (highestSubPower_ != 1) return RACE_ERR_INCOMPATIBILITY;
         std::function<void(int, int, int, int, int, void*)> g =
             [f](int s, int e, int p, int, int numa, void* a) { f(s, e, p, numa, a); };
         return registerFunction(g, args, power, 1, numaSplit);
+    }
+    int registerFunction(void (*f)(int, int, int, int, void*), void* args, int power = 1,
+                         int numaSplit = false) {
+        return registerFunction(std::function<void(int, int, int, int, void*)>(f), args, power, numaSplit);
     }
     // Interface::attachCommunicationToFunction (include/interface.h:161-165):
     // stored, never called on one device (no remote rows), as in the
@@ -150,8 +161,16 @@ class Interface {
     int attachCommunicationToFunction(int funcId, void (*f)(int, int, void*), void* args) {
         return race_attach_communication(h_, funcId, f, args);
     }
+    int attachCommunicationToFunction(int funcId, std::function<void(int, int, void*)> f, void* args) {
+        commFns_.push_back(std::make_unique<CommThunk>(CommThunk{std::move(f), args}));
+        return race_attach_communication(h_, funcId, &CommThunk::call, commFns_.back().get());
+    }
     int attachCommunicationToFunction(int funcId, void (*f)(int, void*), void* args) {
         return race_attach_communication_simple(h_, funcId, f, args);
+    }
+    int attachCommunicationToFunction(int funcId, std::function<void(int, void*)> f, void* args) {
+        commSimpleFns_.push_back(std::make_unique<CommSimpleThunk>(CommSimpleThunk{std::move(f), args}));
+        return race_attach_communication_simple(h_, funcId, &CommSimpleThunk::call, commSimpleFns_.back().get());
     }
     // GPU kernels: the typed device registration (no reference counterpart)
     int registerDevice(const race_kernel_desc& d) {
@@ -171,6 +190,45 @@ class Interface {
         int p = 0;
         race_get_power(h_, funcId, &p);
         return p;
+    }
+    // include/interface.h:168-172
+    void setSubPower(int funcId, int subPow) { race_set_sub_power(h_, funcId, subPow); }
+    int getSubPower(int funcId) {
+        int p = 0;
+        race_get_sub_power(h_, funcId, &p);
+        return p;
+    }
+    void setSerial(int funcId) { race_set_serial(h_, funcId, 1); }
+    void unsetSerial(int funcId) { race_set_serial(h_, funcId, 0); }
+    // include/interface.h:176: per-leaf timers (none are kept here)
+    void resetTime() { race_reset_time(h_); }
+    // include/interface.h:141: parks RACE's own pool (OpenMP threads here)
+    void sleep() { race_sleep(h_); }
+    // include/interface.h:141-142: MPI helpers, empty in the reference too
+    // (src/interface.cpp:63-67, 70-202)
+    void passGlobalRows(int globalStartRow, int globalEndRow) {
+        (void)globalStartRow;
+        (void)globalEndRow;
+    }
+    void compressColIdx() {}
+    // include/interface.h:189-201
+    void pinThread(int threadId) { race_pin_thread(h_, threadId); }
+    void numaInitRowPtr(int* rowPtr) { report(race_numa_init_row_ptr(h_, rowPtr), "numaInitRowPtr"); }
+    void numaInitMtxVec(int* rowPtr, int* col, double* val, double* x, int power = 1) {
+        report(race_numa_init_mtx_vec(h_, rowPtr, col, val, x, power), "numaInitMtxVec");
+    }
+    void numaInitRowPtr(int** rowPtr) { report(race_numa_init_row_ptr_local(h_, rowPtr), "numaInitRowPtr"); }
+    void numaInitMtxVec(int** rowPtr, int** col, double** val, int power = 1) {
+        report(race_numa_init_mtx_vec_local(h_, rowPtr, col, val, power), "numaInitMtxVec");
+    }
+    // split is malloc'ed, as in the reference (src/interface.cpp:1042-1052)
+    void getNumaSplitting(int** split, int* splitLen) {
+        int n = 0;
+        *split = nullptr;
+        *splitLen = 0;
+        if (race_get_numa_splitting(h_, nullptr, 0, &n) != RACE_SUCCESS) return;
+        *split = (int*)std::malloc(sizeof(int) * n);
+        if (*split && race_get_numa_splitting(h_, *split, n, &n) == RACE_SUCCESS) *splitLen = n;
     }
     int tuneFunction(int funcId, bool rev = false) {
         int p = 1;
@@ -224,10 +282,33 @@ class Interface {
             t->f(s, e, p, sp, numa, t->args);
         }
     };
+    struct CommThunk {
+        std::function<void(int, int, void*)> f;
+        void* args;
+        static void call(int p, int sp, void* self) {
+            CommThunk* t = static_cast<CommThunk*>(self);
+            t->f(p, sp, t->args);
+        }
+    };
+    struct CommSimpleThunk {
+        std::function<void(int, void*)> f;
+        void* args;
+        static void call(int p, void* self) {
+            CommSimpleThunk* t = static_cast<CommSimpleThunk*>(self);
+            t->f(p, t->args);
+        }
+    };
+    // the reference prints its errors from these void methods
+    static void report(int s, const char* what) {
+        if (s != RACE_SUCCESS) std::fprintf(stderr, "RACE::Interface::%s: %s\n", what, race_error_string(s));
+    }
     race_handle h_ = nullptr;
     int nrow_ = 0;
+    int highestSubPower_ = 1;
     std::vector<std::unique_ptr<ColorThunk>> colorFns_;
     std::vector<std::unique_ptr<PowerThunk>> powerFns_;
+    std::vector<std::unique_ptr<CommThunk>> commFns_;
+    std::vector<std::unique_ptr<CommSimpleThunk>> commSimpleFns_;
 };
 
 }  // namespace RACE

@@ -40,6 +40,7 @@ class KernelDesc(ctypes.Structure):
         ("kind", ctypes.c_int), ("nrow", ctypes.c_int), ("rowPtr", i32p), ("col", i32p),
         ("val", f64p), ("val_precision", ctypes.c_int), ("vec_precision", ctypes.c_int),
         ("nrhs", ctypes.c_int), ("x", vp), ("b", vp), ("power", ctypes.c_int),
+        ("sub_power", ctypes.c_int),
     ]
 
 
@@ -83,6 +84,18 @@ PROTOTYPES = {
     "race_get_power": (ctypes.c_int, [vp, ctypes.c_int, i32p]),
     "race_tune_function": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, i32p]),
     "race_check_power_coverage": (ctypes.c_int, [vp]),
+    "race_set_sub_power": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int]),
+    "race_get_sub_power": (ctypes.c_int, [vp, ctypes.c_int, i32p]),
+    "race_set_serial": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int]),
+    "race_reset_time": (ctypes.c_int, [vp]),
+    "race_sleep": (ctypes.c_int, [vp]),
+    "race_pin_thread": (ctypes.c_int, [vp, ctypes.c_int]),
+    "race_numa_init_row_ptr": (ctypes.c_int, [vp, i32p]),
+    "race_numa_init_row_ptr_local": (ctypes.c_int, [vp, ctypes.POINTER(i32p)]),
+    "race_numa_init_mtx_vec": (ctypes.c_int, [vp, i32p, i32p, f64p, f64p, ctypes.c_int]),
+    "race_numa_init_mtx_vec_local": (ctypes.c_int, [vp, ctypes.POINTER(i32p), ctypes.POINTER(i32p),
+                                                    ctypes.POINTER(f64p), ctypes.c_int]),
+    "race_get_numa_splitting": (ctypes.c_int, [vp, i32p, ctypes.c_int, i32p]),
     "race_get_schedule": (ctypes.c_int, [vp, ctypes.c_int, i32p, i32p, i32p, i32p, i32p, ctypes.c_int]),
     "race_get_tree": (ctypes.c_int, [vp, i32p, i32p, ctypes.c_int]),
     "race_get_mpk_groups": (ctypes.c_int, [vp, i32p, i32p, ctypes.c_int]),
@@ -98,6 +111,7 @@ PROTOTYPES = {
     "race_lb_split": (ctypes.c_int, [i32p, i32p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                      ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, i32p, i32p,
                                      i32p, i32p, i32p]),
+    "race_split_threads": (ctypes.c_int, [i32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, i32p]),
     "race_batch_create": (ctypes.c_int, [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                          ctypes.c_int]),
     "race_batch_add": (ctypes.c_int, [vp, vp, ctypes.POINTER(KernelDesc)]),

@@ -27,11 +27,16 @@ COMMON = [
 ]
 
 
+# host callbacks run on OpenMP threads (capi.cpp), as in the reference
+OPENMP_SOURCES = {"capi.cpp"}
+
+
 def _compile(src):
     out = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
-    cmd = [HIPCC] + COMMON + ["-c", os.path.join(CSRC, src), "-o", out]
+    flags = COMMON + (["-fopenmp"] if src in OPENMP_SOURCES else [])
+    cmd = [HIPCC] + flags + ["-c", os.path.join(CSRC, src), "-o", out]
     if src.endswith(".cpp"):
-        cmd = [HIPCC] + COMMON + ["-x", "hip", "-c", os.path.join(CSRC, src), "-o", out]
+        cmd = [HIPCC] + flags + ["-x", "hip", "-c", os.path.join(CSRC, src), "-o", out]
     subprocess.check_call(cmd)
     return out
 
@@ -41,7 +46,7 @@ def build(verbose=False):
     with ThreadPoolExecutor(max_workers=min(4, len(SOURCES))) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
-    subprocess.check_call([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp] + objs)
+    subprocess.check_call([HIPCC, "-shared", "-fPIC", "-fopenmp", f"--offload-arch={ARCH}", "-o", tmp] + objs)
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)

@@ -1,6 +1,7 @@
 // bfs.hip -- GPU frontier BFS with RACE's traversal semantics.
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
 #include <chrono>
 #include <cstring>
 
@@ -494,6 +495,20 @@ int GpuTraverser::init(int n_, const int* rowPtr, const int* col, const int* ini
     hipLaunchKernelGGL(k_fill_i32, dim3(gridFor(n / 4 + 1)), dim3(kBlock), 0, st, dVisit.p, -1, (long)n);
     RACE_HIP(hipGetLastError());
     RACE_HIP(hipStreamSynchronize(st));
+    return OK;
+}
+
+int GpuTraverser::orderByRows(bool reverse, LevelData* out) {
+    if (reverse) {
+        std::reverse(hPerm.begin(), hPerm.end());
+        for (int i = 0; i < n; ++i) hInv[hPerm[i]] = i;
+        RACE_HIP(hipMemcpyAsync(dPerm.p, hPerm.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+        RACE_HIP(hipMemcpyAsync(dInv.p, hInv.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+        RACE_HIP(hipStreamSynchronize(st));
+    }
+    out->levelRow.assign(n, 1);
+    out->levelNnz.resize(n);
+    for (int i = 0; i < n; ++i) out->levelNnz[i] = hRowLen[hPerm[i]];
     return OK;
 }
 

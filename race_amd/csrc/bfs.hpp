@@ -35,6 +35,7 @@ class GpuTraverser : public Traverser {
     // disjoint ranges, traversed concurrently (one workgroup per range)
     int traverseBatch(int dist, const std::vector<std::pair<int, int>>& ranges,
                       std::vector<LevelData>* outs) override;
+    int orderByRows(bool reverse, LevelData* out) override;
     const std::vector<int>& perm() const override { return hPerm; }
     const std::vector<int>& invPerm() const override { return hInv; }
     int rowLen(int origRow) const override { return hRowLen[origRow]; }

@@ -4,7 +4,10 @@
 // src/interface.cpp): it owns the graph (copied to the device), the GPU
 // traverser, the zone tree or MPK partition and the registered functions.
 #include <hip/hip_runtime.h>
+#include <omp.h>
+#include <sched.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -30,6 +33,7 @@ struct HostFn {
     race_power_fn powerFn = nullptr;
     void* args = nullptr;
     int power = 1, subPower = 1;
+    bool serial = false;  // setSerial: one host thread
 };
 
 struct Function {
@@ -118,6 +122,13 @@ static int checkCrs(const race_kernel_desc* d) {
     const int nnz = d->rowPtr[d->nrow];
     for (int j = 0; j < nnz; ++j)
         if (d->col[j] < 0 || d->col[j] >= d->nrow) return ERR_INVALID_ARG;
+    // Gauss-Seidel divides by the first entry of a row and SymmSpMV takes it
+    // as the diagonal (GS_KERNEL / SYMM_SPMV_KERNEL_BODY,
+    // examples/src/helpers/kernels.cpp:174-199, 403-427): every row must
+    // hold its diagonal first
+    if (d->kind == RACE_K_GS || d->kind == RACE_K_SYMMSPMV)
+        for (int r = 0; r < d->nrow; ++r)
+            if (d->rowPtr[r + 1] == d->rowPtr[r] || d->col[d->rowPtr[r]] != r) return ERR_INVALID_ARG;
     return OK;
 }
 
@@ -292,7 +303,10 @@ int race_color_power(race_handle h, int highest_power, double cache_size, double
     if (checkHandle(h)) return ERR_INVALID_ARG;
     if (highest_sub_power < 1 || highest_power < 1) return ERR_INVALID_ARG;
     if (h->dist != DIST_POWER) return ERR_INVALID_ARG;  // src/interface.cpp:220-224
-    if (mtx_type != 'N') return ERR_NOT_IMPLEMENTED;
+    // "N" (BFS levels), "L" / "U" (row order / reversed row order as levels,
+    // src/traverse_GAP_graphSoA.cpp:316-335); any other matrix type is an
+    // error in the reference (src/matrixPower.cpp:55-66)
+    if (mtx_type != 'N' && mtx_type != 'L' && mtx_type != 'U') return ERR_INVALID_ARG;
     h->highestPower = highest_power;
     h->highestSubPower = highest_sub_power;
     int e = ensureTraverser(h);
@@ -305,7 +319,7 @@ int race_color_power(race_handle h, int highest_power, double cache_size, double
     int maxRec = mr.empty() ? std::max(0, (int)cut.size()) : mr[0];
     int cutoff0 = (0 >= maxRec) ? 0x7fffffff : (cut.size() > 1 ? cut[1] : cut[0]);
     e = mpkPartition(h->trav.get(), h->nrow, h->nnzInput, highest_power * highest_sub_power,
-                     cache_size, safety, cutoff0, &h->mpk);
+                     cache_size, safety, cutoff0, mtx_type, &h->mpk);
     if (e) return e;
     h->perm = h->trav->perm();
     h->inv = h->trav->invPerm();
@@ -514,36 +528,69 @@ int race_set_vectors(race_handle h, int id, void* x, void* b) {
     return OK;
 }
 
-// Host colouring callbacks run phase by phase; leaves of one phase are
-// independent, so they are called in order on the calling thread.
+// OpenMP threads for host callbacks: RACE's thread count (the reference runs
+// one OpenMP thread per RACE thread), at most the host's, 1 when serial.
+static int hostThreads(race_handle h, const HostFn& f) {
+    if (f.serial) return 1;
+    const int avail = h->availableThreads > 0 ? h->availableThreads : h->requested;
+    return std::max(1, std::min(avail, omp_get_max_threads()));
+}
+
+// Host colouring callbacks run phase by phase, the leaves of one phase in
+// parallel on OpenMP threads (they are distance-k independent, the
+// concurrency of recursiveCall src/functionManager.cpp:199-265), with the
+// implicit barrier of the worksharing loop between phases.
 static void runHostColor(race_handle h, HostFn& f, bool rev) {
     const Schedule& s = rev ? h->rev : h->fwd;
-    for (int p = 0; p < s.numPhases(); ++p)
+    const int nt = hostThreads(h, f);
+#pragma omp parallel num_threads(nt) if (nt > 1)
+    for (int p = 0; p < s.numPhases(); ++p) {
+#pragma omp for schedule(dynamic, 1)
         for (int l = s.phasePtr[p]; l < s.phasePtr[p + 1]; ++l) {
             if (rev)
                 f.color(s.leafHi[l] - 1, s.leafLo[l] - 1, f.args);
             else
                 f.color(s.leafLo[l], s.leafHi[l], f.args);
         }
+    }
     // serial part (empty unless dense rows were split off)
-    if (rev)
-        f.color(h->nrow, h->nrow, f.args);
-    else
-        f.color(h->nrow, h->nrow, f.args);
+    f.color(h->nrow, h->nrow, f.args);
 }
 
-// Host power callbacks in the wavefront order of the level groups.
+// Host power callbacks over the level groups in a slope-2 wavefront: at step
+// L, product t (= (pow-1) subPower + subPow, 1-based) runs on group
+// L - 2 (t-1).  Product t of group g reads product t-1 of groups g-1..g+1,
+// finished at steps <= L-1, and two units of one step are two or more groups
+// apart, so the units of a step are independent; the rows of a unit are cut
+// into slices for the threads (the reference's per-thread slices of a level,
+// FuncManager::powerCallGeneral src/functionManager.cpp:618-822).
 static void runHostPower(race_handle h, HostFn& f) {
     const std::vector<int>& g = h->mpk.groupPtr;
     const int G = (int)g.size() - 1;
     const int k = f.power * f.subPower;
-    for (int L = 0; L < G + k - 1; ++L)
-        for (int p = 0; p < k; ++p) {
-            const int grp = L - p;
+    const int nt = hostThreads(h, f);
+    struct Slice {
+        int lo, hi, t;
+    };
+    std::vector<Slice> work;
+    for (int L = 0; L < G + 2 * (k - 1); ++L) {
+        work.clear();
+        for (int t = 1; t <= k; ++t) {
+            const int grp = L - 2 * (t - 1);
             if (grp < 0 || grp >= G) continue;
-            const int pw = p / f.subPower + 1, sp = p % f.subPower + 1;
-            f.powerFn(g[grp], g[grp + 1], pw, sp, 0, f.args);
+            const int lo = g[grp], hi = g[grp + 1];
+            const int parts = std::max(1, std::min(nt, hi - lo));
+            for (int q = 0; q < parts; ++q)
+                work.push_back({lo + (int)((long)(hi - lo) * q / parts), lo + (int)((long)(hi - lo) * (q + 1) / parts),
+                                t});
         }
+        const int nw = (int)work.size();
+#pragma omp parallel for schedule(static) num_threads(nt) if (nt > 1 && nw > 1)
+        for (int w = 0; w < nw; ++w) {
+            const int t = work[w].t - 1;
+            f.powerFn(work[w].lo, work[w].hi, t / f.subPower + 1, t % f.subPower + 1, 0, f.args);
+        }
+    }
 }
 
 int race_execute_async(race_handle h, int id, int rev, void* stream) {
@@ -577,9 +624,159 @@ int race_set_power(race_handle h, int id, int power) {
     if (fn.host) {
         fn.host->power = power;
     } else {
-        // a device MPK holds vector slots for the registered power only
-        if (power > fn.dev->powerMax) return ERR_INVALID_ARG;
+        // a device MPK holds vector slots for the registered products only
+        if ((long)power * fn.dev->subPower > fn.dev->powerMax) return ERR_INVALID_ARG;
         fn.dev->power = power;
+    }
+    return OK;
+}
+
+int race_set_sub_power(race_handle h, int id, int sub_power) {
+    if (checkHandle(h) || id < 0 || id >= (int)h->funcs.size() || sub_power < 1) return ERR_INVALID_ARG;
+    Function& fn = h->funcs[id];
+    if (fn.host) {
+        fn.host->subPower = sub_power;
+    } else {
+        if ((long)fn.dev->power * sub_power > fn.dev->powerMax) return ERR_INVALID_ARG;
+        fn.dev->subPower = sub_power;
+    }
+    return OK;
+}
+
+int race_get_sub_power(race_handle h, int id, int* sub_power) {
+    if (checkHandle(h) || id < 0 || id >= (int)h->funcs.size() || !sub_power) return ERR_INVALID_ARG;
+    Function& fn = h->funcs[id];
+    *sub_power = fn.host ? fn.host->subPower : fn.dev->subPower;
+    return OK;
+}
+
+int race_set_serial(race_handle h, int id, int serial) {
+    if (checkHandle(h) || id < 0 || id >= (int)h->funcs.size()) return ERR_INVALID_ARG;
+    Function& fn = h->funcs[id];
+    if (fn.host) fn.host->serial = serial != 0;  // device functions have no host threads
+    return OK;
+}
+
+int race_reset_time(race_handle h) { return checkHandle(h) ? ERR_INVALID_ARG : OK; }
+
+int race_sleep(race_handle h) { return checkHandle(h) ? ERR_INVALID_ARG : OK; }
+
+int race_pin_thread(race_handle h, int thread_id) {
+    if (checkHandle(h) || thread_id < 0) return ERR_INVALID_ARG;
+    cpu_set_t allowed;
+    CPU_ZERO(&allowed);
+    if (sched_getaffinity(0, sizeof(allowed), &allowed) != 0) return ERR_PIN;
+    const int ncpu = CPU_COUNT(&allowed);
+    if (ncpu < 1) return ERR_PIN;
+    // thread i goes to the (i * smt mod ncpu)-th CPU the process may use
+    // (RACE's FILL order over its allowed cores, one hardware thread per core
+    // with smt > 1)
+    int target = (int)(((long)thread_id * std::max(1, h->smt)) % ncpu), seen = 0, cpu = -1;
+    for (int c = 0; c < CPU_SETSIZE; ++c)
+        if (CPU_ISSET(c, &allowed) && seen++ == target) {
+            cpu = c;
+            break;
+        }
+    if (cpu < 0) return ERR_PIN;
+    cpu_set_t one;
+    CPU_ZERO(&one);
+    CPU_SET(cpu, &one);
+    return sched_setaffinity(0, sizeof(one), &one) == 0 ? OK : ERR_PIN;
+}
+
+// The reference's first-touch initialisers (src/interface.cpp:852-1040): a
+// power callback registered with power 1 and executed once, so every row
+// range is written by the thread that will later compute it.
+namespace {
+struct NumaArgs {
+    int nrow;
+    int* rowPtr;
+    int* col;
+    double* val;
+    double* x;
+    int** rowPtrL;
+    int** colL;
+    double** valL;
+};
+void numaRowPtrFn(int start, int end, int pow, int subPow, int, void* a) {
+    NumaArgs* n = (NumaArgs*)a;
+    if (pow == 1 && subPow == 1)
+        for (int r = start; r < end; ++r) n->rowPtr[r] = n->rowPtr[r + 1] = 0;
+}
+void numaRowPtrLocalFn(int start, int end, int pow, int subPow, int numa, void* a) {
+    NumaArgs* n = (NumaArgs*)a;
+    if (pow == 1 && subPow == 1)
+        for (int r = start; r < end; ++r) n->rowPtrL[numa][r] = 0;
+}
+void numaMtxVecFn(int start, int end, int pow, int subPow, int, void* a) {
+    NumaArgs* n = (NumaArgs*)a;
+    for (int r = start; r < end; ++r) {
+        if (pow == 1 && subPow == 1)
+            for (int j = n->rowPtr[r]; j < n->rowPtr[r + 1]; ++j) {
+                n->val[j] = 0;
+                n->col[j] = 0;
+            }
+        if (n->x) {
+            n->x[(long)pow * n->nrow + r] = 0;
+            n->x[(long)(pow - 1) * n->nrow + r] = 0;
+        }
+    }
+}
+void numaMtxVecLocalFn(int start, int end, int pow, int subPow, int numa, void* a) {
+    NumaArgs* n = (NumaArgs*)a;
+    if (pow == 1 && subPow == 1)
+        for (int r = start; r < end; ++r)
+            for (int j = n->rowPtrL[numa][r]; j < n->rowPtrL[numa][r + 1]; ++j) {
+                n->valL[numa][j] = 0;
+                n->colL[numa][j] = 0;
+            }
+}
+}  // namespace
+
+static int runNumaInit(race_handle h, race_power_fn fn, NumaArgs* a) {
+    if (checkHandle(h)) return ERR_INVALID_ARG;
+    if (h->dist != DIST_POWER) return ERR_NOT_IMPLEMENTED;  // "NUMA not implemented for coloring"
+    int id = -1;
+    int e = race_register_power_function(h, fn, a, 1, 1, 0, &id);
+    if (e) return e;
+    return race_execute(h, id, 0);
+}
+
+int race_numa_init_row_ptr(race_handle h, int* rowPtr) {
+    if (!rowPtr) return ERR_INVALID_ARG;
+    NumaArgs a{h ? h->nrow : 0, rowPtr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    return runNumaInit(h, numaRowPtrFn, &a);
+}
+
+int race_numa_init_row_ptr_local(race_handle h, int** rowPtr) {
+    if (!rowPtr) return ERR_INVALID_ARG;
+    NumaArgs a{h ? h->nrow : 0, nullptr, nullptr, nullptr, nullptr, rowPtr, nullptr, nullptr};
+    return runNumaInit(h, numaRowPtrLocalFn, &a);
+}
+
+int race_numa_init_mtx_vec(race_handle h, int* rowPtr, int* col, double* val, double* x, int power) {
+    (void)power;  // unused by the reference as well
+    if (!rowPtr || !col || !val) return ERR_INVALID_ARG;
+    NumaArgs a{h ? h->nrow : 0, rowPtr, col, val, x, nullptr, nullptr, nullptr};
+    return runNumaInit(h, numaMtxVecFn, &a);
+}
+
+int race_numa_init_mtx_vec_local(race_handle h, int** rowPtr, int** col, double** val, int power) {
+    (void)power;
+    if (!rowPtr || !col || !val) return ERR_INVALID_ARG;
+    NumaArgs a{h ? h->nrow : 0, nullptr, nullptr, nullptr, nullptr, rowPtr, col, val};
+    return runNumaInit(h, numaMtxVecLocalFn, &a);
+}
+
+int race_get_numa_splitting(race_handle h, int* split, int cap, int* len) {
+    if (checkHandle(h) || !len) return ERR_INVALID_ARG;
+    if (h->dist != DIST_POWER || !h->colored) return ERR_INVALID_ARG;
+    // one device, one domain: the main rows [0, nrow) (src/interface.cpp:1042-1052)
+    *len = 2;
+    if (split) {
+        if (cap < 2) return ERR_INVALID_ARG;
+        split[0] = h->mpk.levelPtr.empty() ? 0 : h->mpk.levelPtr.front();
+        split[1] = h->mpk.levelPtr.empty() ? h->nrow : h->mpk.levelPtr.back();
     }
     return OK;
 }
@@ -598,7 +795,7 @@ int race_tune_function(race_handle h, int id, int rev, int* best_power) {
     double bestT = 1e300;
     int best = 1;
     const Function& fn = h->funcs[id];
-    const int pmax = fn.dev ? std::min(h->highestPower, fn.dev->powerMax) : h->highestPower;
+    const int pmax = fn.dev ? std::min(h->highestPower, fn.dev->powerMax / fn.dev->subPower) : h->highestPower;
     for (int p = 1; p <= pmax; ++p) {
         int e0 = race_set_power(h, id, p);
         if (e0) return e0;
@@ -1011,6 +1208,13 @@ int race_lb_split(const int* level_row, const int* level_nnz, int nlevels, int n
     if (zone_rows) std::memcpy(zone_rows, r.zoneRows.data(), sizeof(int) * r.zoneRows.size());
     if (level_ptr) std::memcpy(level_ptr, r.levelPtr.data(), sizeof(int) * r.levelPtr.size());
     if (scale) std::memcpy(scale, r.scale.data(), sizeof(int) * r.scale.size());
+    return OK;
+}
+
+int race_split_threads(const int* weights, int ngroups, int total, int parent_threads, int* out) {
+    if (!weights || !out || ngroups <= 0 || total <= 0 || parent_threads < 0) return ERR_INVALID_ARG;
+    const std::vector<int> r = splitThreads(std::vector<int>(weights, weights + ngroups), total, parent_threads);
+    std::memcpy(out, r.data(), sizeof(int) * ngroups);
     return OK;
 }
 

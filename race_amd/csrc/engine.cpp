@@ -408,27 +408,40 @@ void LevelRecursion::idealNthreads(int parent, int parentSub, int currLvl) {
     }
     const int parentThreads = T[parent].idealNthreadsZ;
     const int s = T[parent].valueZ[parentSub], e = T[parent].valueZ[parentSub + 1];
-    const int parentRow = e - s;
     // the reference accumulates nnz in int
-    const int parentNnz = (lbTarget == LB_NNZ) ? (int)nnzSum(s, e) : 0;
-    std::vector<double> frac(nt);
-    int remaining = parentThreads;
+    const int total = (lbTarget == LB_NNZ) ? (int)nnzSum(s, e) : e - s;
+    std::vector<int> weights(nt);
     for (int i = 0; i < nt; ++i) {
         const int ss = T[c0 + i * bpt].valueZ.front();
         const int ee = T[c0 + bpt * (i + 1) - 1].valueZ.back();
-        double w = (ee - ss) / (double)parentRow;
-        if (lbTarget == LB_NNZ) w = (int)nnzSum(ss, ee) / (double)parentNnz;
-        const double myD = w * parentThreads;
-        const int my = (int)myD;
-        for (int b = 0; b < bpt; ++b) T[c0 + bpt * i + b].idealNthreadsZ = my;
-        frac[i] = (my != 0) ? (myD - my) : 1.0;  // empty shares first
-        remaining -= my;
+        weights[i] = (lbTarget == LB_NNZ) ? (int)nnzSum(ss, ee) : ee - ss;
+    }
+    const std::vector<int> my = splitThreads(weights, total, parentThreads);
+    for (int i = 0; i < nt; ++i)
+        for (int b = 0; b < bpt; ++b) T[c0 + bpt * i + b].idealNthreadsZ = my[i];
+}
+
+// The thread split of LevelRecursion::calculateIdealNthreads
+// (src/level_recursion.cpp:165-245): group i gets floor(weight_i / total *
+// parentThreads); groups left with 0 rank first (remainder 1), the others by
+// their fractional part, and the leftover threads go one each down that
+// ranking (stable: ties keep group order).
+std::vector<int> splitThreads(const std::vector<int>& weights, int total, int parentThreads) {
+    const int nt = (int)weights.size();
+    std::vector<int> my(nt);
+    std::vector<double> frac(nt);
+    int remaining = parentThreads;
+    for (int i = 0; i < nt; ++i) {
+        const double myD = (weights[i] / (double)total) * parentThreads;
+        my[i] = (int)myD;
+        frac[i] = (my[i] != 0) ? (myD - my[i]) : 1.0;
+        remaining -= my[i];
     }
     if (remaining > 0) {
         std::vector<int> order = rankDescending(frac);
-        for (int r = 0; remaining != 0; ++r, --remaining)
-            for (int b = 0; b < bpt; ++b) T[c0 + bpt * order[r] + b].idealNthreadsZ += 1;
+        for (int r = 0; remaining != 0; ++r, --remaining) my[order[r]] += 1;
     }
+    return my;
 }
 
 int LevelRecursion::recursivePartition(int parent, int parentSub, int currLvl) {
@@ -554,9 +567,10 @@ Schedule flattenSchedule(const ZoneTree& zt, bool rev, int nrow) {
 
 // ---------------------------------------------------------------------------
 int mpkPartition(Traverser* t, int n, long nnzInput, int k, double cacheBytes, double safety,
-                 int cutoff, MPKPartition* out) {
+                 int cutoff, char mtxType, MPKPartition* out) {
     LevelData ld;
-    int err = t->traverse(DIST_POWER, 0, n, &ld);
+    int err = (mtxType == 'L' || mtxType == 'U') ? t->orderByRows(mtxType == 'U', &ld)
+                                                 : t->traverse(DIST_POWER, 0, n, &ld);
     if (err != OK) return err;
     const int L = ld.totalLevel();
     out->highestPower = k;

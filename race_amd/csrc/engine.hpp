@@ -67,6 +67,10 @@ struct LBResult {
 LBResult splitZones(const LevelData& ld, int nThreads, double efficiency, int dist, int d2,
                     int lbTarget);
 
+// Threads per child group of a partitioned range
+// (LevelRecursion::calculateIdealNthreads, src/level_recursion.cpp:165-245).
+std::vector<int> splitThreads(const std::vector<int>& weights, int total, int parentThreads);
+
 struct ZoneLeaf {
     std::vector<int> valueZ;    // row boundaries of the leaf's sub-blocks
     std::vector<int> children;  // [begin,end) leaf indices per sub-block
@@ -112,6 +116,11 @@ class Traverser {
         }
         return 0;
     }
+    // mtxType "L" / "U" of the POWER traversal (Traverse::calculateDistance
+    // src/traverse_GAP_graphSoA.cpp:316-335): no BFS, row i is level i
+    // ("L") or level n-1-i ("U"); the stable sort by level then keeps or
+    // reverses the current order, composed into the permutation
+    virtual int orderByRows(bool reverse, LevelData* out) = 0;
     virtual const std::vector<int>& perm() const = 0;     // host mirror, perm[new]=old
     virtual const std::vector<int>& invPerm() const = 0;  // host mirror
     virtual int rowLen(int origRow) const = 0;            // graph row length
@@ -162,7 +171,7 @@ struct MPKPartition {
     int highestPower = 1;
 };
 int mpkPartition(Traverser* t, int n, long nnzInput, int highestPower, double cacheBytes,
-                 double safety, int cutoff, MPKPartition* out);
+                 double safety, int cutoff, char mtxType, MPKPartition* out);
 
 // Parse comma separated integers with atoi semantics (getEnv, utility.h:94-111).
 std::vector<int> envIntList(const char* name);

@@ -456,7 +456,11 @@ int makeDeviceFn(const race_kernel_desc& d, const ExecContext& ctx, std::unique_
         default:
             return ERR_INVALID_ARG;
     }
-    if (*out) (*out)->vecPrecision = d.vec_precision;
+    if (*out) {
+        (*out)->vecPrecision = d.vec_precision;
+        (*out)->nrow = d.nrow;
+        (*out)->nrhs = d.nrhs > 0 ? d.nrhs : 1;
+    }
     return e;
 }
 

@@ -38,9 +38,12 @@ class DeviceFn {
     void* x = nullptr;
     void* b = nullptr;
     int power = 1;
-    int powerMax = 1 << 30;  // largest power setPower accepts (MPK: the registered one)
+    int subPower = 1;        // MPK: products with A per power (setSubPower)
+    int powerMax = 1 << 30;  // largest power x subPower accepted (MPK: the registered one)
     int kind = 0;
     int vecPrecision = 0;  // RACE_FP64 or RACE_FP32 vectors
+    int nrow = 0;          // rows of the matrix the function was registered with
+    int nrhs = 1;          // right-hand sides of its vectors
 };
 
 struct ExecContext {

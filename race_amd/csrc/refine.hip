@@ -54,6 +54,9 @@ class RefineFn : public DeviceFn {
         if (!in || (in->kind != RACE_K_GS && in->kind != RACE_K_KACZ) || innerSweeps < 1)
             return ERR_INVALID_ARG;
         if (d.val_precision != RACE_FP64 || d.vec_precision != RACE_FP64) return ERR_INVALID_ARG;
+        // the inner sweeps run on this function's workspace: same rows and
+        // right-hand sides, or they would index past it
+        if (in->nrow != d.nrow || in->nrhs != (d.nrhs > 0 ? d.nrhs : 1)) return ERR_INVALID_ARG;
         kind = d.kind;
         inner = in;
         sweeps = innerSweeps;
@@ -68,13 +71,9 @@ class RefineFn : public DeviceFn {
         if (inner->vecPrecision == RACE_FP32) {
             RACE_HIP(r32.reserve(len));
             RACE_HIP(d32.reserve(len));
-            inner->x = d32.p;
-            inner->b = r32.p;
         } else {
             RACE_HIP(r64.reserve(len));
             RACE_HIP(d64.reserve(len));
-            inner->x = d64.p;
-            inner->b = r64.p;
         }
         RACE_HIP(hipHostMalloc(reinterpret_cast<void**>(&hostNorms), 2 * sizeof(double)));
         return OK;
@@ -96,20 +95,29 @@ class RefineFn : public DeviceFn {
         return OK;
     }
 
-    // correction sweeps on A d = r and x += d (residual already in r)
+    // correction sweeps on A d = r and x += d (residual already in r).  The
+    // inner function is pointed at the workspace for the sweeps and back at
+    // its own vectors afterwards, so it stays usable on its own.
     int correct(hipStream_t st) {
         const long len = (long)n * nrhs;
         const unsigned g = gridFor(len);
-        if (inner->vecPrecision == RACE_FP32)
+        const bool f32 = inner->vecPrecision == RACE_FP32;
+        if (f32)
             hipLaunchKernelGGL(k_zero<float>, dim3(g), dim3(kBlock), 0, st, d32.p, len);
         else
             hipLaunchKernelGGL(k_zero<double>, dim3(g), dim3(kBlock), 0, st, d64.p, len);
-        for (int s = 0; s < sweeps; ++s) {
-            int e = inner->run(false, st);
-            if (e) return e;
-            e = inner->run(true, st);
-            if (e) return e;
+        void* const ownX = inner->x;
+        void* const ownB = inner->b;
+        inner->x = f32 ? (void*)d32.p : (void*)d64.p;
+        inner->b = f32 ? (void*)r32.p : (void*)r64.p;
+        int e = OK;
+        for (int s = 0; s < sweeps && !e; ++s) {
+            e = inner->run(false, st);
+            if (!e) e = inner->run(true, st);
         }
+        inner->x = ownX;
+        inner->b = ownB;
+        if (e) return e;
         totalSweeps += sweeps;
         if (inner->vecPrecision == RACE_FP32)
             hipLaunchKernelGGL(k_update<float>, dim3(g), dim3(kBlock), 0, st, (double*)x, d32.p, len);

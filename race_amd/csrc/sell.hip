@@ -454,13 +454,32 @@ class MpkFn : public DeviceFn {
         x = d.x;
         b = d.b;
         if (!ctx.mpk) return ERR_INVALID_ARG;
-        power = powerMax = d.power > 0 ? d.power : 1;
+        // powers x sub-powers: every (pow, subPow) pair of the reference's
+        // power executor is one product with A, in order
+        // (FuncManager::powerCallGeneral src/functionManager.cpp:618-822)
+        power = d.power > 0 ? d.power : 1;
+        subPower = d.sub_power > 0 ? d.sub_power : 1;
+        powerMax = power * subPower;
         nslot = d.kind == RACE_K_MPK_HIGHEST ? 2 : powerMax + 1;
         const std::vector<int>& gp = ctx.mpk->groupPtr;
         if (gp.size() < 2 || gp.front() != 0 || gp.back() != d.nrow) return ERR_INVALID_ARG;
+        const int G = (int)gp.size() - 1;
+        // The wavefront needs every entry of group g in groups g-1..g+1.
+        // BFS levels guarantee it; with mtxType "L" / "U" (row order as
+        // levels) it holds only for matrices whose entries stay within
+        // neighbouring groups, so anything else is refused here.
+        {
+            std::vector<int> grp(d.nrow);
+            for (int g = 0; g < G; ++g)
+                for (int r = gp[g]; r < gp[g + 1]; ++r) grp[r] = g;
+            for (int r = 0; r < d.nrow; ++r)
+                for (int j = d.rowPtr[r]; j < d.rowPtr[r + 1]; ++j) {
+                    const int dg = grp[d.col[j]] - grp[r];
+                    if (dg < -1 || dg > 1) return ERR_INCOMPATIBILITY;
+                }
+        }
         int e;
         if ((e = A.build(d, gp))) return e;
-        const int G = (int)gp.size() - 1;
         std::vector<MpkSeg>& all = hostSegs;
         all.clear();
         stepSeg.assign(powerMax + 1, {});
@@ -485,15 +504,17 @@ class MpkFn : public DeviceFn {
     }
     bool graphable() const override { return true; }
     int launches(bool) const override {
-        if (power < 1 || power > powerMax) return 0;
+        const int steps = power * subPower;
+        if (steps < 1 || steps > powerMax) return 0;
         int k = 0;
-        for (int nb : stepBlocks[power]) k += nb > 0;
+        for (int nb : stepBlocks[steps]) k += nb > 0;
         return k;
     }
     int run(bool, hipStream_t st) override {
-        if (power < 1 || power > powerMax) return ERR_INVALID_ARG;
-        const std::vector<int>& sg = stepSeg[power];
-        const std::vector<int>& nb = stepBlocks[power];
+        const int steps = power * subPower;
+        if (steps < 1 || steps > powerMax) return ERR_INVALID_ARG;
+        const std::vector<int>& sg = stepSeg[steps];
+        const std::vector<int>& nb = stepBlocks[steps];
         for (int L = 0; L + 1 < (int)sg.size(); ++L) {
             if (nb[L] == 0) continue;
             const int ns = sg[L + 1] - sg[L];

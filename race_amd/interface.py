@@ -17,7 +17,7 @@ from . import _lib as L
 from ._lib import (BF16, FP32, FP64, K_GS, K_KACZ, K_MPK, K_MPK_HIGHEST, K_SPMTV, K_SPMV, K_SYMMSPMV, LB_NNZ,
                    LB_ROW, ONE, POWER, SCATTER, TWO, TWO_BLOCK, RaceError)
 
-__all__ = ["Interface", "DeviceArray", "lb_split", "RaceError", "ONE", "TWO", "POWER", "K_SYMMSPMV",
+__all__ = ["Interface", "DeviceArray", "lb_split", "split_threads", "RaceError", "ONE", "TWO", "POWER", "K_SYMMSPMV",
            "K_GS", "K_KACZ", "K_SPMV", "K_MPK", "K_SPMTV", "K_MPK_HIGHEST", "FP64", "FP32", "BF16", "LB_NNZ", "LB_ROW",
            "TWO_BLOCK", "device_count", "schedule_info", "rcm"]
 
@@ -179,14 +179,15 @@ class Interface:
         self._keep.append(cb)
 
     def registerDevice(self, kind, rowPtr, col, val, x, b=None, nrhs=1, power=1,
-                       val_precision=FP64, vec_precision=FP64):
+                       val_precision=FP64, vec_precision=FP64, subPower=1):
         rp, c = _i32(rowPtr), _i32(col)
         v = np.ascontiguousarray(val, dtype=np.float64)
         d = L.KernelDesc(kind=int(kind), nrow=self.nrow, rowPtr=_ip(rp), col=_ip(c),
                          val=v.ctypes.data_as(L.f64p), val_precision=int(val_precision),
                          vec_precision=int(vec_precision), nrhs=int(nrhs),
                          x=x.ptr if isinstance(x, DeviceArray) else x,
-                         b=(b.ptr if isinstance(b, DeviceArray) else b), power=int(power))
+                         b=(b.ptr if isinstance(b, DeviceArray) else b), power=int(power),
+                         sub_power=int(subPower))
         fid = ctypes.c_int()
         L.check(self._L.race_register_device(self.h, ctypes.byref(d), ctypes.byref(fid)), "registerDevice")
         # device arrays stay alive while a registered function may use them
@@ -241,6 +242,45 @@ class Interface:
         v = ctypes.c_int()
         L.check(self._L.race_get_power(self.h, fid, ctypes.byref(v)), "getPower")
         return v.value
+
+    def setSubPower(self, fid, sp):
+        L.check(self._L.race_set_sub_power(self.h, fid, int(sp)), "setSubPower")
+
+    def getSubPower(self, fid):
+        v = ctypes.c_int()
+        L.check(self._L.race_get_sub_power(self.h, fid, ctypes.byref(v)), "getSubPower")
+        return v.value
+
+    def setSerial(self, fid):
+        L.check(self._L.race_set_serial(self.h, fid, 1), "setSerial")
+
+    def unsetSerial(self, fid):
+        L.check(self._L.race_set_serial(self.h, fid, 0), "unsetSerial")
+
+    def resetTime(self):
+        L.check(self._L.race_reset_time(self.h), "resetTime")
+
+    def sleep(self):
+        L.check(self._L.race_sleep(self.h), "sleep")
+
+    def pinThread(self, threadId):
+        L.check(self._L.race_pin_thread(self.h, int(threadId)), "pinThread")
+
+    def numaInitRowPtr(self, rowPtr):
+        """first-touch zeroing of rowPtr (an int32 numpy array, modified in place)"""
+        L.check(self._L.race_numa_init_row_ptr(self.h, _ip(rowPtr)), "numaInitRowPtr")
+
+    def numaInitMtxVec(self, rowPtr, col, val, x=None, power=1):
+        """first-touch zeroing of col, val (and x[0:2n]) in place"""
+        xp = None if x is None else x.ctypes.data_as(L.f64p)
+        L.check(self._L.race_numa_init_mtx_vec(self.h, _ip(rowPtr), _ip(col), val.ctypes.data_as(L.f64p), xp,
+                                               int(power)), "numaInitMtxVec")
+
+    def getNumaSplitting(self):
+        split = np.zeros(2, np.int32)
+        n = ctypes.c_int()
+        L.check(self._L.race_get_numa_splitting(self.h, _ip(split), 2, ctypes.byref(n)), "getNumaSplitting")
+        return split[:n.value]
 
     def tuneFunction(self, fid, rev=False):
         v = ctypes.c_int()
@@ -487,3 +527,13 @@ def lb_split(levelRow, levelNnz, nthreads, eff, dist, d2=TWO_BLOCK, lb_target=LB
                                    _ip(sc)), "race_lb_split")
     return dict(zone_rows=zr[:nb.value].tolist(), levelPtr=lp[:nb.value + 1].tolist(),
                 scale=sc[:nt.value].tolist(), threads=nt.value)
+
+
+def split_threads(weights, total, parent_threads):
+    """LevelRecursion::calculateIdealNthreads' thread split through the C
+    ABI (host only)."""
+    w = _i32(weights)
+    out = np.zeros(max(len(w), 1), np.int32)
+    L.check(L.load().race_split_threads(_ip(w), len(w), int(total), int(parent_threads), _ip(out)),
+            "race_split_threads")
+    return out[:len(w)].tolist()

@@ -34,6 +34,16 @@ def test_malformed_matrix_is_rejected_before_any_kernel():
     brp[3], brp[4] = brp[4], brp[3]  # rowPtr not non-decreasing
     with pytest.raises(R.RaceError):
         ce.registerDevice(R.K_GS, brp, col, val, dx, db)
+    # Gauss-Seidel needs the diagonal first in every row (it divides by it)
+    swp = col.copy()
+    swp[rp[4]], swp[rp[4] + 1] = swp[rp[4] + 1], swp[rp[4]]
+    with pytest.raises(R.RaceError):
+        ce.registerDevice(R.K_GS, rp, swp, val, dx, db)
+    # an empty row has no diagonal (a GS row would read past its entries)
+    erp = rp.copy()
+    erp[-1] = erp[-2]
+    with pytest.raises(R.RaceError):
+        ce.registerDevice(R.K_GS, erp, col, val, dx, db)
     # the handle still works afterwards
     fid = ce.registerDevice(R.K_SPMV, rp, col, val, dx, db)
     ce.executeFunction(fid)

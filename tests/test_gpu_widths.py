@@ -22,8 +22,14 @@ OFFSETS = [0, 1, 3, 7, 12, 20, 33, 54, 88, 143, 232, 376, 609, 985, 1597, 2584, 
            28657]
 
 
+def _width(widths, r):
+    """one width per 64-row chunk, cycling through widths: a list of several
+    widths gives chunks of different widths"""
+    return widths[(r // 64) % len(widths)]
+
+
 def _rows_of_widths(n, widths, seed):
-    rows = [sorted({(r + s) % n for s in OFFSETS[:widths[r % len(widths)]]}) for r in range(n)]
+    rows = [sorted({(r + s) % n for s in OFFSETS[:_width(widths, r)]}) for r in range(n)]
     rp = np.zeros(n + 1, np.int32)
     rp[1:] = np.cumsum([len(c) for c in rows])
     col = np.array([c for cs in rows for c in cs], np.int32)
@@ -31,15 +37,20 @@ def _rows_of_widths(n, widths, seed):
     return rp, col, val
 
 
-# [8]: narrow, uniform; [3, 8, 5]: narrow with chunk records; [9], [13],
-# [16], [21]: wide, uniform (8 + 1, 8 + 4 + 1, 8 + 8, 8 + 8 + 4 + 1);
-# [5, 9, 13, 6, 11]: wide with chunk records
+# [8]: narrow, uniform; [3, 8, 5]: narrow chunks of three widths (chunk
+# records); [9], [13], [16], [21]: wide, uniform (8 + 1, 8 + 4 + 1, 8 + 8,
+# 8 + 8 + 4 + 1); [5, 9, 13, 6, 11]: chunks of five widths, narrow and wide
+# (chunk records)
 @pytest.mark.parametrize("widths", [[8], [3, 8, 5], [9], [13], [16], [21], [5, 9, 13, 6, 11]])
 def test_spmv_row_widths(oracle, widths):
     print("case spmv widths", widths, flush=True)
     n = 30000
     rp, col, val = _rows_of_widths(n, widths, seed=len(widths) + widths[0])
-    assert np.all(np.diff(rp) == [widths[r % len(widths)] for r in range(n)])
+    assert np.all(np.diff(rp) == [_width(widths, r) for r in range(n)])
+    # several widths: padding every chunk to the widest would cost more than
+    # 1/8 extra entries, so the layout keeps one record per chunk
+    padded = max(widths) * n
+    assert (padded * 8 > int(rp[-1]) * 9) == (len(widths) > 1)
     x = M.uniform_vec(n, 21)
     ce = R.Interface(n, 1, R.ONE, rp, col, symm_hint=False)
     dx, dy = R.DeviceArray.from_host(x), R.DeviceArray.from_host(np.zeros(n))
