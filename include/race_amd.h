@@ -247,6 +247,10 @@ int race_get_schedule(race_handle h, int rev, int *nphases, int *nleaves, int *p
 int race_get_tree(race_handle h, int *nleaves, int *data6, int cap);
 /* MPK level groups: group_ptr[ngroups+1] (row offsets) */
 int race_get_mpk_groups(race_handle h, int *ngroups, int *group_ptr, int cap);
+/* the recursion stage of every hopeless group (mtxPowerRecursive's stage-1
+ * leaves, src/matrixPowerRecursive.cpp:156-338): per child lo, hi (rows),
+ * its stage-0 group, and the number of its own level groups */
+int race_get_mpk_children(race_handle h, int *nchildren, int *data4, int cap);
 
 /* ---- initial permutation ----
  * Reverse Cuthill-McKee order of a symmetric pattern, perm[new] = old, the
@@ -298,6 +302,20 @@ int race_lb_split(const int *level_row, const int *level_nnz, int nlevels, int n
  * is the nnz (or row count) of child group i, total the range's; out[i] =
  * the ideal thread count of group i. */
 int race_split_threads(const int *weights, int ngroups, int total, int parent_threads, int *out);
+
+/* Hopeless regions of the MPK level grouping
+ * (mtxPower::identifyHopelessRegions src/matrixPower.cpp:147-502, one node):
+ * violated = ascending levels over the cache by more than the cut-off; out
+ * receives [start, end] level pairs (inclusive), *nout entries. */
+int race_mpk_hopeless_regions(const int *violated, int nviolated, int k, int total_level, int *out, int cap,
+                              int *nout);
+/* mtxPower::consolidatePartition (src/matrixPower.cpp:685-880, one node):
+ * level_ptr[nlevels+1], elem[nlevels] (safety * (k+1) * nnz per level),
+ * regions from race_mpk_hopeless_regions; groups[ngroups+1] row pointers,
+ * glevel[ngroups+1] first levels, hid[nhid] the group of each region. */
+int race_mpk_consolidate(const int *level_ptr, const double *elem, int nlevels, int k, double cache_elem,
+                         const int *regions, int nregions, int *groups, int *glevel, int *hid, int cap,
+                         int *ngroups, int *nhid);
 
 /* ---- batches: one function on many matrices (no counterpart in the
  * reference) ----

@@ -249,6 +249,136 @@ int or_traverse(void *wsp, int n, const int *rowPtr, const int *col,
     return totalLevel;
 }
 
+/*
+ * The POWER traversal of a recursion stage (mtxPowerRecursive, a hopeless
+ * region [lo,hi) with its boundary regions): Traverse ctor with a boundary
+ * bitmap (src/traverse_GAP_graphSoA.cpp:76-88), TDStep's boundary branch
+ * (:184-197: a boundary row reached from the frontier gets the level and is
+ * expanded, but does not count towards the range), island restarts over the
+ * range only (:277-306, adjacent levels for POWER), createLevelData
+ * (:449-495: untouched boundary rows go to one extra last level, at least
+ * one level), findLevelData for the range and every boundary region, and
+ * permuteGraph (:506-575: each region stably sorted by level, composed into
+ * totalPerm).
+ * nb boundary regions [blo[b], bhi[b]); levelRow/levelNnz hold cap entries,
+ * bLevelRow/bLevelNnz nb * cap (region b at b * cap).  Returns totalLevel,
+ * or -1 when rows go missing or cap is too small.
+ */
+int or_traverse_boundary(void *wsp, int n, const int *rowPtr, const int *col, int *totalPerm,
+                         int *totalInvPerm, int lo, int hi, int nb, const int *blo, const int *bhi,
+                         int *levelRow, int *levelNnz, int *bLevelRow, int *bLevelNnz, int cap)
+{
+    or_ws *w = (or_ws *)wsp;
+    if (lo >= hi) return 0;
+    char *inB = (char *)calloc((size_t)n, 1);
+    for (int b = 0; b < nb; ++b)
+        for (int i = blo[b]; i < bhi[b]; ++i) inB[i] = 1;
+    int *q = w->queue;
+    long qhead = 0, qtail = 0;
+    long counter = 0;
+    int currLvl = 0;
+    w->ntouched = 0;
+    q[qtail++] = lo;
+    w->dist[lo] = currLvl;
+    ws_visit(w, lo, 0);
+    counter++;
+    int prevIslandStop = lo;
+    int status = 0;
+    while (qtail > qhead) {
+        currLvl += 1;
+        long fEnd = qtail;
+        for (long qi = qhead; qi < fEnd; ++qi) {
+            int u = q[qi];
+            int pu = totalPerm[u];
+            for (int j = rowPtr[pu]; j < rowPtr[pu + 1]; ++j) {
+                int c = col[j];
+                if (c >= n) continue;
+                c = totalInvPerm[c];
+                if (w->parent[c] != -1) continue;
+                if (c >= lo && c < hi) {
+                    ws_visit(w, c, u);
+                    q[qtail++] = c;
+                    w->dist[c] = currLvl;
+                    counter++;
+                } else if (inB[c]) {
+                    ws_visit(w, c, u);
+                    q[qtail++] = c;
+                    w->dist[c] = currLvl;
+                }
+            }
+        }
+        qhead = fEnd;
+        if (qtail == qhead && counter != (long)(hi - lo)) {
+            for (int i = prevIslandStop; i < hi; ++i) {
+                if (w->dist[i] == -1) {
+                    q[qtail++] = i;
+                    counter++;
+                    w->dist[i] = currLvl;
+                    ws_visit(w, i, i);
+                    prevIslandStop = i;
+                    break;
+                }
+            }
+            if (qtail == qhead) { status = -1; break; }
+        }
+    }
+    int totalLevel = currLvl;
+    if (status == 0) {
+        int untouched = 0;
+        for (int b = 0; b < nb; ++b)
+            for (int i = blo[b]; i < bhi[b]; ++i)
+                if (w->dist[i] == -1) {
+                    w->dist[i] = totalLevel;
+                    w->touched[w->ntouched++] = i;
+                    untouched = 1;
+                }
+        if (untouched) totalLevel += 1;
+        if (totalLevel < 1) totalLevel = 1;
+        if (totalLevel > cap) status = -1;
+    }
+    if (status == 0) {
+        for (int L = 0; L < totalLevel; ++L) { levelRow[L] = 0; levelNnz[L] = 0; }
+        for (int i = lo; i < hi; ++i) {
+            int d = w->dist[i], pr = totalPerm[i];
+            levelRow[d] += 1;
+            levelNnz[d] += rowPtr[pr + 1] - rowPtr[pr];
+        }
+        for (int b = 0; b < nb; ++b) {
+            int *lr = bLevelRow + (size_t)b * cap, *ln = bLevelNnz + (size_t)b * cap;
+            for (int L = 0; L < totalLevel; ++L) { lr[L] = 0; ln[L] = 0; }
+            for (int i = blo[b]; i < bhi[b]; ++i) {
+                int d = w->dist[i], pr = totalPerm[i];
+                lr[d] += 1;
+                ln[d] += rowPtr[pr + 1] - rowPtr[pr];
+            }
+        }
+        /* stable sort of every region by level, composed into totalPerm */
+        int *off = w->cnt, *perm = w->perm, *tmp = (int *)malloc(sizeof(int) * (size_t)n);
+        for (int region = -1; region < nb; ++region) {
+            const int a = region < 0 ? lo : blo[region], e = region < 0 ? hi : bhi[region];
+            if (a >= e) continue;
+            for (int L = 0; L <= totalLevel; ++L) off[L] = 0;
+            for (int i = a; i < e; ++i) off[w->dist[i] + 1]++;
+            for (int L = 0; L < totalLevel; ++L) off[L + 1] += off[L];
+            for (int i = a; i < e; ++i) perm[a + off[w->dist[i]]++] = i;
+            for (int i = a; i < e; ++i) tmp[i - a] = totalPerm[perm[i]];
+            for (int i = a; i < e; ++i) {
+                totalPerm[i] = tmp[i - a];
+                totalInvPerm[totalPerm[i]] = i;
+            }
+        }
+        free(tmp);
+    }
+    for (long t = 0; t < w->ntouched; ++t) {
+        int c = w->touched[t];
+        w->dist[c] = -1;
+        w->parent[c] = -1;
+    }
+    w->ntouched = 0;
+    free(inB);
+    return status == 0 ? totalLevel : -1;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Kernel callback bodies (sequential over the given row range).             */
 /* CRS arrays are the *permuted* matrix with the diagonal stored first.      */

@@ -98,6 +98,10 @@ def lib():
         L.or_traverse.restype = ctypes.c_int
         L.or_traverse.argtypes = [ctypes.c_void_p, ctypes.c_int, _i32p, _i32p, _i32p, _i32p,
                                   ctypes.c_int, ctypes.c_int, ctypes.c_int, _i32p, _i32p, _i32p]
+        L.or_traverse_boundary.restype = ctypes.c_int
+        L.or_traverse_boundary.argtypes = [ctypes.c_void_p, ctypes.c_int, _i32p, _i32p, _i32p, _i32p,
+                                           ctypes.c_int, ctypes.c_int, ctypes.c_int, _i32p, _i32p, _i32p,
+                                           _i32p, _i32p, _i32p, ctypes.c_int]
         L.or_symmetrize.restype = ctypes.c_long
         L.or_symmetrize.argtypes = [ctypes.c_int, _i32p, _i32p, _i32p, _i32p, ctypes.c_long]
         L.or_run_calls.argtypes = [ctypes.c_int, ctypes.c_int, _i32p, _i32p, _f64p, _f64p, _f64p,
@@ -208,6 +212,27 @@ class Graph:
             raise RuntimeError("traversal lost rows")
         out = (lr[:tl].copy(), ln[:tl].copy())
         return out + ((dd[:hi - lo].copy(),) if want_dist else ())
+
+    def traverse_boundary(self, lo, hi, bounds):
+        """POWER traversal of [lo,hi) with boundary regions bounds = [(blo,
+        bhi), ...] (the recursion stage of mtxPowerRecursive); returns
+        (levelRow, levelNnz, [(levelRow_b, levelNnz_b) per region]) and
+        composes the permutation of the range and of every region."""
+        nb = len(bounds)
+        cap = (hi - lo) + sum(b - a for a, b in bounds) + 2
+        lr = np.zeros(cap, np.int32)
+        ln = np.zeros(cap, np.int32)
+        blr = np.zeros(max(nb, 1) * cap, np.int32)
+        bln = np.zeros(max(nb, 1) * cap, np.int32)
+        blo = np.array([a for a, _ in bounds] or [0], np.int32)
+        bhi = np.array([b for _, b in bounds] or [0], np.int32)
+        tl = lib().or_traverse_boundary(self.ws, self.n, _ip(self.rowPtr), _ip(self.col), _ip(self.perm),
+                                        _ip(self.inv), lo, hi, nb, _ip(blo), _ip(bhi), _ip(lr), _ip(ln),
+                                        _ip(blr), _ip(bln), cap)
+        if tl < 0:
+            raise RuntimeError("traversal lost rows")
+        regs = [(blr[b * cap:b * cap + tl].copy(), bln[b * cap:b * cap + tl].copy()) for b in range(nb)]
+        return lr[:tl].copy(), ln[:tl].copy(), regs
 
 
 # --------------------------------------------------------------------------
@@ -742,15 +767,161 @@ def mpk_highest_serial(rowPtr, col, val, x0, k):
 
 
 # --------------------------------------------------------------------------
-# MPK partition, stage 0 (no hopeless regions)
+# MPK partition: stage 0 and the recursion into hopeless regions
 # --------------------------------------------------------------------------
-def mpk_partition(graph, k, cache_bytes, safety=2.0, cutoff=50, mtx_type="N"):
-    """Stage-0 level groups of mtxPower::findPartition/consolidatePartition.
-    Returns dict(levelPtr=[level row offsets], groupPtr=[row offsets of the
-    consolidated groups], hopeless=[levels with factor>cutoff]).
-    mtx_type "L" / "U": no BFS, row i is level i (n-1-i for "U"), and the
-    stable sort by level keeps (reverses) the current order
-    (Traverse::calculateDistance src/traverse_GAP_graphSoA.cpp:316-335)."""
+INT_MAX = 2 ** 31 - 1
+
+
+def _region(regs, i):
+    """getHopelessStartEnd src/matrixPower.cpp:674-683: (start, end) of
+    region i of a flat [start, end, ...] list, (-1, -1) outside it."""
+    if i >= 0 and 2 * i < len(regs) - 1:
+        return regs[2 * i], regs[2 * i + 1]
+    return -1, -1
+
+
+def identify_hopeless_regions(violated, k, total_level):
+    """mtxPower::identifyHopelessRegions src/matrixPower.cpp:147-502 for one
+    node.  violated: ascending levels whose cache factor exceeds the cut-off.
+    Only runs of at least two consecutive violated levels form a region
+    (:149-217); regions closer than k-1 levels merge (:225-250); regions are
+    kept k-1 levels away from the first and last level, or dropped when
+    nothing is left (:268-398).  Returns the flat [start, end, ...] list
+    (inclusive level ranges)."""
+    cvl = list(violated)
+    if len(cvl) > 1:
+        cvl.append(cvl[-1] + 2)
+    flag = [0] * len(cvl)
+    prev, block = -2, 1
+    for i, lv in enumerate(cvl):
+        if lv == prev + 1:
+            flag[i - 1] = block
+            flag[i] = block
+        else:
+            block += 1
+        prev = lv
+    regs = []
+    pf = 0
+    for i, f in enumerate(flag):
+        if f != pf:
+            if f > 0 and pf == 0:
+                regs.append(cvl[i])
+            elif f > 0 and pf > 0:
+                regs += [cvl[i - 1], cvl[i]]
+            elif f == 0:
+                regs.append(cvl[i - 1])
+        pf = f
+    # merge regions less than k-1 levels apart
+    n_h = len(regs) // 2
+    merged = []
+    cs, ce = _region(regs, 0)
+    ns, ne = _region(regs, 1)
+    i = 0
+    while i < n_h:
+        if cs != -1:
+            while (ns - ce - 1) < (k - 1) and ns != -1:
+                ce = ne
+                ns, ne = _region(regs, i + 2)
+                i += 1
+            merged += [cs, ce]
+            cs, ce = _region(regs, i + 1)
+            ns, ne = _region(regs, i + 2)
+        i += 1
+    # one node [0, total_level): split at its ends (nothing to split), then
+    # keep k-1 levels to the first and the last level
+    lvl_start, lvl_end = 0, total_level
+    split = []
+    for i in range(len(merged) // 2):
+        cs, ce = _region(merged, i)
+        if cs >= lvl_start:
+            if cs < lvl_end:
+                split += [cs, lvl_end - 1] if ce > lvl_end else [cs, ce]
+        else:
+            if ce > lvl_end:
+                split += [lvl_start, lvl_end - 1]
+            elif ce >= lvl_start:
+                split += [lvl_start, ce]
+    drop = []
+    cs, ce = _region(split, 0)
+    if cs != -1 and (cs - lvl_start - 1) < (k - 1):
+        new_cs = lvl_start + (k - 1)
+        if ce < new_cs:
+            drop.append(0)
+        else:
+            split[0] = new_cs
+    last = len(split) // 2 - 1
+    cs, ce = _region(split, last)
+    if ce != -1 and ((lvl_end - 1) - ce - 1) < (k - 1):
+        new_ce = lvl_end - (k - 1) - 1
+        if cs > new_ce:
+            drop.append(last)
+        else:
+            split[2 * last + 1] = new_ce
+    out = []
+    for i in range(len(split) // 2):
+        if i not in drop:
+            out += [split[2 * i], split[2 * i + 1]]
+    return out
+
+
+def consolidate_levels(level_ptr, elem, k, cache_elem, regions):
+    """mtxPower::consolidatePartition src/matrixPower.cpp:685-880 for one
+    node: consecutive levels merge into a group until the running sum of
+    elem reaches cache_elem; around a hopeless region every level from
+    k-1 before its start to k-1 after its end is a group of its own, and the
+    region's levels form one group.  Returns (group row pointers, group
+    first levels, group index of each hopeless region)."""
+    total_level = len(level_ptr) - 1
+    bl = k - 1
+    groups = [level_ptr[0]]
+    glevel = [0]
+    hid = []
+    ctr = 0
+    hc = 0
+    hs, he = _region(regions, 0)
+    sum_elem = 0.0
+    cur = 0
+    for level in range(total_level):
+        e = elem[level]
+        sum_elem += e
+        hopeless = False
+        force = False
+        if hs != -1:
+            if hs - bl <= level <= hs:
+                force = True
+            elif he < level <= he + bl:
+                force = True
+            elif hs < level <= he:
+                hopeless = True
+            if level == he + bl + 1:
+                hc += 1
+                hs, he = _region(regions, hc)
+                force = True
+            if level == hs:
+                hid.append(ctr + 1)
+        if cur > 0 and not hopeless:
+            if sum_elem >= cache_elem or force:
+                sum_elem = e
+                groups.append(level_ptr[level])
+                glevel.append(level)
+                ctr += 1
+        cur += 1
+    groups.append(level_ptr[total_level])
+    glevel.append(total_level)
+    return groups, glevel, hid
+
+
+def mpk_partition(graph, k, cache_bytes, safety=2.0, cutoff=50, mtx_type="N", cutoff1=INT_MAX):
+    """mtxPowerRecursive::findPartition src/matrixPowerRecursive.cpp:340-445
+    with mtxPower::findPartition src/matrixPower.cpp:505-604.  Stage 0:
+    traversal (or mtx_type "L" / "U": row i is level i, or n-1-i, no BFS,
+    src/traverse_GAP_graphSoA.cpp:316-335), levels whose elem exceeds the
+    cache by more than `cutoff` times, hopeless regions, level groups.  For
+    mtx_type "N", every hopeless region's group is traversed again with its
+    boundary groups (recursivePartition :156-338, working radius
+    ceil(k/2) - 1) and grouped with cut-off `cutoff1` (a further stage is
+    not restated: NotImplementedError).  Returns dict(levelPtr, groupPtr,
+    groupLevel, violated, hopelessRegions, hopelessGroups, children, ...)."""
     if mtx_type in ("L", "U"):
         if mtx_type == "U":
             graph.perm = graph.perm[::-1].copy()
@@ -767,31 +938,43 @@ def mpk_partition(graph, k, cache_bytes, safety=2.0, cutoff=50, mtx_type="N"):
     byte_per_nnz = (1 + k / nnzr) * 8 + (1 + 1 / nnzr) * 4
     cache_elem = cache_bytes / byte_per_nnz
 
-    def elem(level):
-        return safety * (k + 1) * int(ln[level])
+    def violated_levels(elem, cut):
+        out = []
+        for L, e in enumerate(elem):
+            if e > cache_elem and int(np.ceil(e / cache_elem)) > cut:
+                out.append(L)
+        return out
 
-    hopeless = []
-    for L in range(total_level):
-        e = elem(L)
-        if e > cache_elem:
-            factor = int(np.ceil(e / cache_elem))
-            if factor > cutoff:
-                hopeless.append(L)
-    groups = [level_ptr[0]]
-    glevels = [0]
-    sum_elem = 0.0
-    ctr = 0
-    for L in range(total_level):
-        e = elem(L)
-        sum_elem += e
-        if ctr > 0 and sum_elem >= cache_elem:
-            sum_elem = e
-            groups.append(level_ptr[L])
-            glevels.append(L)
-        ctr += 1
-    groups.append(level_ptr[total_level])
-    glevels.append(total_level)
-    return dict(levelPtr=level_ptr, groupPtr=groups, groupLevel=glevels, hopeless=hopeless,
+    elem = [safety * (k + 1) * int(v) for v in ln]
+    violated = violated_levels(elem, cutoff)
+    regions = identify_hopeless_regions(violated, k, total_level)
+    groups, glevel, hid = consolidate_levels(level_ptr, elem, k, cache_elem, regions)
+    children = []
+    if mtx_type == "N":
+        wbl = int(np.ceil(k / 2.0)) - 1
+        for h in hid:
+            lo, hi = groups[h], groups[h + 1]
+            bounds = []
+            for r in range(-wbl, wbl + 1):
+                if r != 0 and groups[h + r + 1] > groups[h + r]:
+                    bounds.append((groups[h + r], groups[h + r + 1]))
+            clr, cln, blds = graph.traverse_boundary(lo, hi, bounds)
+            cptr = [lo]
+            for r in clr:
+                cptr.append(cptr[-1] + int(r))
+            celem = []
+            for L in range(len(clr)):
+                nz = int(cln[L]) + sum(int(b[1][L]) for b in blds)
+                celem.append(safety * (k + 1) * nz)
+            cviol = violated_levels(celem, cutoff1)
+            cregs = identify_hopeless_regions(cviol, k, len(clr))
+            if cregs:
+                raise NotImplementedError("hopeless regions inside a recursion stage")
+            cgroups, _, _ = consolidate_levels(cptr, celem, k, cache_elem, [])
+            children.append(dict(lo=lo, hi=hi, group=h, boundaries=bounds, groupPtr=cgroups,
+                                 levelRow=clr, levelNnz=cln))
+    return dict(levelPtr=level_ptr, groupPtr=groups, groupLevel=glevel, violated=violated, hopeless=violated,
+                hopelessRegions=regions, hopelessGroups=hid, children=children,
                 levelRow=lr, levelNnz=ln, cache_elem=cache_elem, byte_per_nnz=byte_per_nnz)
 
 

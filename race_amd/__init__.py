@@ -7,8 +7,8 @@ script.
 """
 from .interface import (BF16, FP32, FP64, K_GS, K_KACZ, K_MPK, K_MPK_HIGHEST, K_SPMTV, K_SPMV, K_SYMMSPMV,
                         LB_NNZ, LB_ROW, ONE, POWER, TWO, TWO_BLOCK, Batch, DeviceArray, Interface, RaceError,
-                        device_count, lb_split, rcm, split_threads, schedule_info, simdify_resolve, simdifyD1)
+                        device_count, lb_split, mpk_consolidate, mpk_hopeless_regions, rcm, split_threads, schedule_info, simdify_resolve, simdifyD1)
 
-__all__ = ["Interface", "Batch", "DeviceArray", "RaceError", "lb_split", "split_threads", "device_count", "simdify_resolve",
+__all__ = ["Interface", "Batch", "DeviceArray", "RaceError", "lb_split", "split_threads", "mpk_hopeless_regions", "mpk_consolidate", "device_count", "simdify_resolve",
            "simdifyD1", "schedule_info", "rcm", "ONE", "TWO", "POWER", "K_SYMMSPMV", "K_GS", "K_KACZ", "K_SPMV", "K_MPK",
            "K_SPMTV", "K_MPK_HIGHEST", "FP64", "FP32", "BF16", "LB_NNZ", "LB_ROW", "TWO_BLOCK"]

@@ -112,6 +112,11 @@ PROTOTYPES = {
                                      ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, i32p, i32p,
                                      i32p, i32p, i32p]),
     "race_split_threads": (ctypes.c_int, [i32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, i32p]),
+    "race_mpk_hopeless_regions": (ctypes.c_int, [i32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, i32p,
+                                                 ctypes.c_int, i32p]),
+    "race_mpk_consolidate": (ctypes.c_int, [i32p, f64p, ctypes.c_int, ctypes.c_int, ctypes.c_double, i32p,
+                                            ctypes.c_int, i32p, i32p, i32p, ctypes.c_int, i32p, i32p]),
+    "race_get_mpk_children": (ctypes.c_int, [vp, i32p, i32p, ctypes.c_int]),
     "race_batch_create": (ctypes.c_int, [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                          ctypes.c_int]),
     "race_batch_add": (ctypes.c_int, [vp, vp, ctypes.POINTER(KernelDesc)]),

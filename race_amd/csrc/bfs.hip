@@ -31,11 +31,14 @@ __global__ void k_seed(int* visit, int* dist, int* q, int row, int parent, int l
 
 // One BFS level: every frontier row claims its unvisited neighbours.
 // counters[0] = next frontier length, counters[1] = in-range rows reached.
+// bmask (POWER recursion stages, else null): rows of the boundary regions;
+// such a row is claimed with the level and expanded like a range row but
+// not counted (Traverse::TDStep src/traverse_GAP_graphSoA.cpp:184-197).
 __global__ void __launch_bounds__(kBlock) k_bfs_step(
     const int* __restrict__ rowPtr, const int* __restrict__ col, const int* __restrict__ perm,
     const int* __restrict__ inv, int* __restrict__ visit, int* __restrict__ dist,
     const int* __restrict__ qin, int qlen, int* __restrict__ qout, int* __restrict__ counters,
-    int lo, int hi, int lvl, int twoDist, int n) {
+    int lo, int hi, int lvl, int twoDist, int n, const unsigned char* __restrict__ bmask) {
     __shared__ int sCnt, sIn, sBase;
     if (threadIdx.x == 0) {
         sCnt = 0;
@@ -61,6 +64,16 @@ __global__ void __launch_bounds__(kBlock) k_bfs_step(
                 if (atomicCAS(&visit[c], -1, u) == -1) {
                     dist[c] = lvl;
                     ++inCnt;
+                    if (j - s < 64) {
+                        won |= 1ull << (j - s);
+                        ++cnt;
+                    } else {
+                        qout[atomicAdd(&counters[0], 1)] = c;
+                    }
+                }
+            } else if (bmask && bmask[c]) {
+                if (atomicCAS(&visit[c], -1, u) == -1) {
+                    dist[c] = lvl;
                     if (j - s < 64) {
                         won |= 1ull << (j - s);
                         ++cnt;
@@ -370,6 +383,26 @@ __global__ void __launch_bounds__(kBlock) k_compose_batch(
 }
 
 // smallest row in [start,hi) without a level
+__global__ void k_fill_u8(unsigned char* __restrict__ a, unsigned char v, int lo, int hi) {
+    int i = lo + blockIdx.x * blockDim.x + threadIdx.x;
+    const int stride = gridDim.x * blockDim.x;
+    for (; i < hi; i += stride) a[i] = v;
+}
+
+// boundary rows no traversal reached get level `lvl` (createLevelData
+// src/traverse_GAP_graphSoA.cpp:455-466); *flag = 1 if there was one
+__global__ void k_untouched_to_level(int* __restrict__ dist, int lo, int hi, int lvl, int* __restrict__ flag) {
+    int i = lo + blockIdx.x * blockDim.x + threadIdx.x;
+    const int stride = gridDim.x * blockDim.x;
+    bool any = false;
+    for (; i < hi; i += stride)
+        if (dist[i] == -1) {
+            dist[i] = lvl;
+            any = true;
+        }
+    if (any) atomicOr(flag, 1);
+}
+
 __global__ void k_first_unvisited(const int* __restrict__ dist, int start, int hi, int* out) {
     int i = start + blockIdx.x * blockDim.x + threadIdx.x;
     int stride = gridDim.x * blockDim.x;
@@ -529,6 +562,69 @@ int GpuTraverser::traverse(int dist, int lo, int hi, LevelData* out) {
 // A large range, level by level over the whole device.
 int GpuTraverser::traverseLevels(int dist, int lo, int hi, LevelData* out) {
     auto t0 = std::chrono::steady_clock::now();
+    int totalLevel = 0;
+    int e = bfsLevels(dist, lo, hi, nullptr, &totalLevel);
+    if (e) return e;
+    if ((e = histogram(lo, hi, totalLevel, out))) return e;
+    if ((e = sortByLevel(lo, hi, totalLevel))) return e;
+    RACE_HIP(hipStreamSynchronize(st));
+    auto t1 = std::chrono::steady_clock::now();
+    lastTraverseMs += std::chrono::duration<double, std::milli>(t1 - t0).count();
+    ++traversals;
+    return OK;
+}
+
+// The recursion stage's POWER traversal of [lo,hi) with boundary regions.
+int GpuTraverser::traverseBoundary(int lo, int hi, const std::vector<std::pair<int, int>>& bounds,
+                                   LevelData* out, std::vector<LevelData>* bout) {
+    auto t0 = std::chrono::steady_clock::now();
+    out->levelRow.clear();
+    out->levelNnz.clear();
+    bout->assign(bounds.size(), LevelData());
+    if (lo >= hi) return OK;
+    if (!dMask.p) {
+        RACE_HIP(dMask.reserve(n));
+        hipLaunchKernelGGL(k_fill_u8, dim3(gridFor(n / 4 + 1)), dim3(kBlock), 0, st, dMask.p, (unsigned char)0, 0, n);
+    }
+    for (const auto& b : bounds)
+        if (b.second > b.first)
+            hipLaunchKernelGGL(k_fill_u8, dim3(gridFor(b.second - b.first)), dim3(kBlock), 0, st, dMask.p,
+                               (unsigned char)1, b.first, b.second);
+    int totalLevel = 0;
+    int e = bfsLevels(DIST_POWER, lo, hi, dMask.p, &totalLevel);
+    for (const auto& b : bounds)
+        if (b.second > b.first)
+            hipLaunchKernelGGL(k_fill_u8, dim3(gridFor(b.second - b.first)), dim3(kBlock), 0, st, dMask.p,
+                               (unsigned char)0, b.first, b.second);
+    if (e) return e;
+    // untouched boundary rows: one more level at the end; at least one level
+    RACE_HIP(hipMemsetAsync(dCount.p + 3, 0, sizeof(int), st));
+    for (const auto& b : bounds)
+        if (b.second > b.first)
+            hipLaunchKernelGGL(k_untouched_to_level, dim3(gridFor(b.second - b.first)), dim3(kBlock), 0, st,
+                               dDist.p, b.first, b.second, totalLevel, dCount.p + 3);
+    RACE_HIP(hipGetLastError());
+    RACE_HIP(hipMemcpyAsync(hPinned + 3, dCount.p + 3, sizeof(int), hipMemcpyDeviceToHost, st));
+    RACE_HIP(hipStreamSynchronize(st));
+    if (hPinned[3]) totalLevel += 1;
+    totalLevel = std::max(1, totalLevel);
+    if ((e = histogram(lo, hi, totalLevel, out))) return e;
+    for (size_t r = 0; r < bounds.size(); ++r)
+        if ((e = histogram(bounds[r].first, bounds[r].second, totalLevel, &(*bout)[r]))) return e;
+    // every region stably sorted by level, composed into the permutation
+    if ((e = sortByLevel(lo, hi, totalLevel))) return e;
+    for (const auto& b : bounds)
+        if ((e = sortByLevel(b.first, b.second, totalLevel))) return e;
+    RACE_HIP(hipStreamSynchronize(st));
+    auto t1 = std::chrono::steady_clock::now();
+    lastTraverseMs += std::chrono::duration<double, std::milli>(t1 - t0).count();
+    ++traversals;
+    return OK;
+}
+
+// Level-synchronous BFS of [lo,hi) from lo with island restarts; dDist holds
+// the levels afterwards (-1: not reached).
+int GpuTraverser::bfsLevels(int dist, int lo, int hi, const unsigned char* bmask, int* levels) {
     const int len = hi - lo;
     // fresh visit/level state (Traverse ctor sets both to -1 for every row)
     hipLaunchKernelGGL(k_fill_i32, dim3(gridFor(n / 4 + 1)), dim3(kBlock), 0, st, dVisit.p, -1, (long)n);
@@ -546,7 +642,7 @@ int GpuTraverser::traverseLevels(int dist, int lo, int hi, LevelData* out) {
         RACE_HIP(hipMemsetAsync(dCount.p, 0, 2 * sizeof(int), st));
         hipLaunchKernelGGL(k_bfs_step, dim3(gridFor(qlen)), dim3(kBlock), 0, st, dRowPtr.p, dCol.p,
                            dPerm.p, dInv.p, dVisit.p, dDist.p, qin, qlen, qout, dCount.p, lo, hi,
-                           lvl, two, n);
+                           lvl, two, n, bmask);
         RACE_HIP(hipGetLastError());
         RACE_HIP(hipMemcpyAsync(hPinned, dCount.p, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
         RACE_HIP(hipStreamSynchronize(st));
@@ -574,9 +670,17 @@ int GpuTraverser::traverseLevels(int dist, int lo, int hi, LevelData* out) {
             prevIslandStop = root;
         }
     }
-    const int totalLevel = lvl;
+    *levels = lvl;
+    return OK;
+}
 
-    // histograms
+// levelRow / levelNnz of the rows [lo,hi) over totalLevel levels
+// (findLevelData src/traverse_GAP_graphSoA.cpp:385-447)
+int GpuTraverser::histogram(int lo, int hi, int totalLevel, LevelData* out) {
+    const int len = hi - lo;
+    out->levelRow.assign(totalLevel, 0);
+    out->levelNnz.assign(totalLevel, 0);
+    if (len <= 0) return OK;
     RACE_HIP(dLevelRow.reserve(totalLevel + 1));
     RACE_HIP(dLevelNnz.reserve(totalLevel + 1));
     RACE_HIP(hipMemsetAsync(dLevelRow.p, 0, sizeof(int) * totalLevel, st));
@@ -590,8 +694,15 @@ int GpuTraverser::traverseLevels(int dist, int lo, int hi, LevelData* out) {
                             hipMemcpyDeviceToHost, st));
     RACE_HIP(hipMemcpyAsync(out->levelNnz.data(), dLevelNnz.p, sizeof(int) * totalLevel,
                             hipMemcpyDeviceToHost, st));
+    RACE_HIP(hipStreamSynchronize(st));
+    return OK;
+}
 
-    // stable sort of the range by level (LSD radix sort is stable)
+// stable sort of [lo,hi) by level (LSD radix sort is stable), composed into
+// perm / invPerm on the device and the host mirror
+int GpuTraverser::sortByLevel(int lo, int hi, int totalLevel) {
+    const int len = hi - lo;
+    if (len <= 0) return OK;
     hipLaunchKernelGGL(k_sort_input, dim3(gridFor(len)), dim3(kBlock), 0, st, dDist.p, lo, len,
                        dKeys.p, dVals.p);
     int endBit = 1;
@@ -612,9 +723,6 @@ int GpuTraverser::traverseLevels(int dist, int lo, int hi, LevelData* out) {
                             hipMemcpyDeviceToHost, st));
     RACE_HIP(hipStreamSynchronize(st));
     for (int i = lo; i < hi; ++i) hInv[hPerm[i]] = i;
-    auto t1 = std::chrono::steady_clock::now();
-    lastTraverseMs += std::chrono::duration<double, std::milli>(t1 - t0).count();
-    ++traversals;
     return OK;
 }
 

@@ -36,6 +36,8 @@ class GpuTraverser : public Traverser {
     int traverseBatch(int dist, const std::vector<std::pair<int, int>>& ranges,
                       std::vector<LevelData>* outs) override;
     int orderByRows(bool reverse, LevelData* out) override;
+    int traverseBoundary(int lo, int hi, const std::vector<std::pair<int, int>>& bounds, LevelData* out,
+                         std::vector<LevelData>* bout) override;
     const std::vector<int>& perm() const override { return hPerm; }
     const std::vector<int>& invPerm() const override { return hInv; }
     int rowLen(int origRow) const override { return hRowLen[origRow]; }
@@ -52,6 +54,10 @@ class GpuTraverser : public Traverser {
 
   private:
     int traverseLevels(int dist, int lo, int hi, LevelData* out);
+    int bfsLevels(int dist, int lo, int hi, const unsigned char* bmask, int* levels);
+    int histogram(int lo, int hi, int totalLevel, LevelData* out);
+    int sortByLevel(int lo, int hi, int totalLevel);
+    DevBuf<unsigned char> dMask;  // boundary rows of a recursion stage's traversal
     hipStream_t st = nullptr;
     int stampNext = 0, bvisitSlots = 0;
     DevBuf<int> dBVisit, dBatchQ, dLvRow, dLvNnz, dLvStart, dPackRow, dPackNnz, dNLev;

@@ -317,19 +317,22 @@ int race_color_power(race_handle h, int highest_power, double cache_size, double
     for (int v : envIntList("RACE_CACHE_VIOLATION_CUTOFF")) cut.push_back(v);
     std::vector<int> mr = envIntList("RACE_MAX_RECURSION_STAGES");
     int maxRec = mr.empty() ? std::max(0, (int)cut.size()) : mr[0];
-    int cutoff0 = (0 >= maxRec) ? 0x7fffffff : (cut.size() > 1 ? cut[1] : cut[0]);
+    // mtxPowerRecursive::get_cache_violation_cutoff: stage s uses entry s+1
+    // of the list (the default when the list is shorter), no cut-off from
+    // stage maxRec on
+    auto cutoffOf = [&](int stage) {
+        if (stage >= maxRec) return 0x7fffffff;
+        return stage < (int)cut.size() - 1 ? cut[stage + 1] : cut[0];
+    };
     e = mpkPartition(h->trav.get(), h->nrow, h->nnzInput, highest_power * highest_sub_power,
-                     cache_size, safety, cutoff0, mtx_type, &h->mpk);
+                     cache_size, safety, cutoffOf(0), cutoffOf(1), mtx_type, &h->mpk);
+    if (e == ERR_NOT_IMPLEMENTED)
+        std::fprintf(stderr,
+                     "race_amd: hopeless levels inside a recursion stage (or with mtxType L/U): a second "
+                     "recursion stage is not implemented\n");
     if (e) return e;
     h->perm = h->trav->perm();
     h->inv = h->trav->invPerm();
-    if (!h->mpk.hopeless.empty()) {
-        std::fprintf(stderr,
-                     "race_amd: %zu BFS levels exceed the cache by more than the cut-off; "
-                     "recursive (hopeless-region) MPK partitioning is not implemented\n",
-                     h->mpk.hopeless.size());
-        return ERR_NOT_IMPLEMENTED;
-    }
     h->availableThreads = h->requested;
     h->colored = true;
     return OK;
@@ -1215,6 +1218,49 @@ int race_split_threads(const int* weights, int ngroups, int total, int parent_th
     if (!weights || !out || ngroups <= 0 || total <= 0 || parent_threads < 0) return ERR_INVALID_ARG;
     const std::vector<int> r = splitThreads(std::vector<int>(weights, weights + ngroups), total, parent_threads);
     std::memcpy(out, r.data(), sizeof(int) * ngroups);
+    return OK;
+}
+
+int race_mpk_hopeless_regions(const int* violated, int nviolated, int k, int total_level, int* out, int cap,
+                              int* nout) {
+    if ((nviolated > 0 && !violated) || nviolated < 0 || k < 1 || total_level < 1 || !nout) return ERR_INVALID_ARG;
+    const std::vector<int> r =
+        identifyHopelessRegions(std::vector<int>(violated, violated + nviolated), k, total_level);
+    *nout = (int)r.size();
+    if ((int)r.size() > cap || (!out && !r.empty())) return ERR_INVALID_ARG;
+    if (!r.empty()) std::memcpy(out, r.data(), sizeof(int) * r.size());
+    return OK;
+}
+
+int race_mpk_consolidate(const int* level_ptr, const double* elem, int nlevels, int k, double cache_elem,
+                         const int* regions, int nregions, int* groups, int* glevel, int* hid, int cap,
+                         int* ngroups, int* nhid) {
+    if (!level_ptr || !elem || nlevels < 1 || k < 1 || nregions < 0 || (nregions && !regions) || !ngroups ||
+        !nhid || !groups || !glevel || !hid)
+        return ERR_INVALID_ARG;
+    std::vector<int> g, gl, h;
+    consolidateLevels(std::vector<int>(level_ptr, level_ptr + nlevels + 1), std::vector<double>(elem, elem + nlevels),
+                      k, cache_elem, std::vector<int>(regions, regions + nregions), &g, &gl, &h);
+    *ngroups = (int)g.size() - 1;
+    *nhid = (int)h.size();
+    if ((int)g.size() > cap || (int)h.size() > cap) return ERR_INVALID_ARG;
+    std::memcpy(groups, g.data(), sizeof(int) * g.size());
+    std::memcpy(glevel, gl.data(), sizeof(int) * gl.size());
+    if (!h.empty()) std::memcpy(hid, h.data(), sizeof(int) * h.size());
+    return OK;
+}
+
+int race_get_mpk_children(race_handle h, int* nchildren, int* data4, int cap) {
+    if (checkHandle(h) || h->dist != DIST_POWER || !h->colored || !nchildren) return ERR_INVALID_ARG;
+    const auto& c = h->mpk.children;
+    *nchildren = (int)c.size();
+    if (data4 && (int)c.size() <= cap)
+        for (size_t i = 0; i < c.size(); ++i) {
+            data4[4 * i + 0] = c[i].lo;
+            data4[4 * i + 1] = c[i].hi;
+            data4[4 * i + 2] = c[i].group;
+            data4[4 * i + 3] = (int)c[i].groupPtr.size() - 1;
+        }
     return OK;
 }
 

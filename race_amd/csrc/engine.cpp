@@ -566,8 +566,153 @@ Schedule flattenSchedule(const ZoneTree& zt, bool rev, int nrow) {
 }
 
 // ---------------------------------------------------------------------------
+namespace {
+// getHopelessStartEnd src/matrixPower.cpp:674-683
+void regionAt(const std::vector<int>& r, int i, int* s, int* e) {
+    *s = -1;
+    *e = -1;
+    if (i >= 0 && 2 * i < (int)r.size() - 1) {
+        *s = r[2 * i];
+        *e = r[2 * i + 1];
+    }
+}
+}  // namespace
+
+std::vector<int> identifyHopelessRegions(const std::vector<int>& violated, int k, int totalLevel) {
+    // runs of consecutive violated levels (a single violated level is no
+    // region): flag each run, then read off its first and last level
+    std::vector<int> cvl = violated;
+    if (cvl.size() > 1) cvl.push_back(cvl.back() + 2);
+    std::vector<int> flag(cvl.size(), 0);
+    int prev = -2, block = 1;
+    for (size_t i = 0; i < cvl.size(); ++i) {
+        if (cvl[i] == prev + 1) {
+            flag[i - 1] = block;
+            flag[i] = block;
+        } else {
+            ++block;
+        }
+        prev = cvl[i];
+    }
+    std::vector<int> regs;
+    for (size_t i = 0, pf = 0; i < flag.size(); pf = flag[i], ++i) {
+        const int f = flag[i];
+        if (f == (int)pf) continue;
+        if (f > 0 && pf == 0) {
+            regs.push_back(cvl[i]);
+        } else if (f > 0) {
+            regs.push_back(cvl[i - 1]);
+            regs.push_back(cvl[i]);
+        } else {
+            regs.push_back(cvl[i - 1]);
+        }
+    }
+    // regions closer than k-1 levels become one
+    const int nh = (int)regs.size() / 2;
+    std::vector<int> merged;
+    int cs, ce, ns, ne;
+    regionAt(regs, 0, &cs, &ce);
+    regionAt(regs, 1, &ns, &ne);
+    for (int i = 0; i < nh; ++i) {
+        if (cs == -1) continue;
+        while ((ns - ce - 1) < (k - 1) && ns != -1) {
+            ce = ne;
+            regionAt(regs, i + 2, &ns, &ne);
+            ++i;
+        }
+        merged.push_back(cs);
+        merged.push_back(ce);
+        regionAt(regs, i + 1, &cs, &ce);
+        regionAt(regs, i + 2, &ns, &ne);
+    }
+    // one node [0, totalLevel): keep k-1 levels to its first and last level
+    const int lvlStart = 0, lvlEnd = totalLevel;
+    std::vector<int> split;
+    for (int i = 0; i < (int)merged.size() / 2; ++i) {
+        regionAt(merged, i, &cs, &ce);
+        if (cs >= lvlStart) {
+            if (cs < lvlEnd) {
+                split.push_back(cs);
+                split.push_back(ce > lvlEnd ? lvlEnd - 1 : ce);
+            }
+        } else if (ce > lvlEnd) {
+            split.push_back(lvlStart);
+            split.push_back(lvlEnd - 1);
+        } else if (ce >= lvlStart) {
+            split.push_back(lvlStart);
+            split.push_back(ce);
+        }
+    }
+    std::vector<int> drop;
+    regionAt(split, 0, &cs, &ce);
+    if (cs != -1 && (cs - lvlStart - 1) < (k - 1)) {
+        const int newStart = lvlStart + (k - 1);
+        if (ce < newStart)
+            drop.push_back(0);
+        else
+            split[0] = newStart;
+    }
+    const int last = (int)split.size() / 2 - 1;
+    regionAt(split, last, &cs, &ce);
+    if (ce != -1 && ((lvlEnd - 1) - ce - 1) < (k - 1)) {
+        const int newEnd = lvlEnd - (k - 1) - 1;
+        if (cs > newEnd)
+            drop.push_back(last);
+        else
+            split[2 * last + 1] = newEnd;
+    }
+    std::vector<int> out;
+    for (int i = 0; i < (int)split.size() / 2; ++i)
+        if (std::find(drop.begin(), drop.end(), i) == drop.end()) {
+            out.push_back(split[2 * i]);
+            out.push_back(split[2 * i + 1]);
+        }
+    return out;
+}
+
+void consolidateLevels(const std::vector<int>& levelPtr, const std::vector<double>& elem, int k,
+                       double cacheElem, const std::vector<int>& regions, std::vector<int>* groups,
+                       std::vector<int>* glevel, std::vector<int>* hid) {
+    const int L = (int)levelPtr.size() - 1;
+    const int bl = k - 1;
+    groups->assign(1, levelPtr[0]);
+    glevel->assign(1, 0);
+    hid->clear();
+    int ctr = 0, hc = 0, hs, he;
+    regionAt(regions, 0, &hs, &he);
+    double sum = 0;
+    for (int l = 0; l < L; ++l) {
+        const double e = elem[l];
+        sum += e;
+        bool hopeless = false, force = false;
+        if (hs != -1) {
+            // k-1 levels on each side of a region are groups of their own,
+            // the region itself is one group
+            if (l >= hs - bl && l <= hs)
+                force = true;
+            else if (l > he && l <= he + bl)
+                force = true;
+            else if (l > hs && l <= he)
+                hopeless = true;
+            if (l == he + bl + 1) {
+                regionAt(regions, ++hc, &hs, &he);
+                force = true;
+            }
+            if (l == hs) hid->push_back(ctr + 1);
+        }
+        if (l > 0 && !hopeless && (sum >= cacheElem || force)) {
+            sum = e;
+            groups->push_back(levelPtr[l]);
+            glevel->push_back(l);
+            ++ctr;
+        }
+    }
+    groups->push_back(levelPtr[L]);
+    glevel->push_back(L);
+}
+
 int mpkPartition(Traverser* t, int n, long nnzInput, int k, double cacheBytes, double safety,
-                 int cutoff, char mtxType, MPKPartition* out) {
+                 int cutoff0, int cutoff1, char mtxType, MPKPartition* out) {
     LevelData ld;
     int err = (mtxType == 'L' || mtxType == 'U') ? t->orderByRows(mtxType == 'U', &ld)
                                                  : t->traverse(DIST_POWER, 0, n, &ld);
@@ -582,26 +727,52 @@ int mpkPartition(Traverser* t, int n, long nnzInput, int k, double cacheBytes, d
     const double nnzr = nnzInput / (double)n;
     out->bytePerNnz = (1 + k / nnzr) * sizeof(double) + (1 + 1 / nnzr) * sizeof(int);
     out->cacheElem = cacheBytes / out->bytePerNnz;
-    auto elem = [&](int l) { return safety * (k + 1) * ld.levelNnz[l]; };
-    out->hopeless.clear();
-    for (int l = 0; l < L; ++l) {
-        double e = elem(l);
-        if (e > out->cacheElem && (int)std::ceil(e / out->cacheElem) > cutoff) out->hopeless.push_back(l);
-    }
-    out->groupPtr.assign(1, out->levelPtr[0]);
-    out->groupLevel.assign(1, 0);
-    double sum = 0;
-    for (int l = 0; l < L; ++l) {
-        double e = elem(l);
-        sum += e;
-        if (l > 0 && sum >= out->cacheElem) {
-            sum = e;
-            out->groupPtr.push_back(out->levelPtr[l]);
-            out->groupLevel.push_back(l);
+    auto violatedOf = [&](const std::vector<double>& elem, int cutoff) {
+        std::vector<int> v;
+        for (int l = 0; l < (int)elem.size(); ++l)
+            if (elem[l] > out->cacheElem && (int)std::ceil(elem[l] / out->cacheElem) > cutoff) v.push_back(l);
+        return v;
+    };
+    std::vector<double> elem(L);
+    for (int l = 0; l < L; ++l) elem[l] = safety * (k + 1) * ld.levelNnz[l];
+    out->violated = violatedOf(elem, cutoff0);
+    out->hopelessRegions = identifyHopelessRegions(out->violated, k, L);
+    consolidateLevels(out->levelPtr, elem, k, out->cacheElem, out->hopelessRegions, &out->groupPtr,
+                      &out->groupLevel, &out->hopeless);
+    out->children.clear();
+    if (mtxType != 'N') return out->hopeless.empty() ? OK : ERR_NOT_IMPLEMENTED;
+    // recursion stage: every hopeless group is traversed again together with
+    // the groups within the working radius ceil(k/2) - 1 around it
+    // (mtxPowerRecursive::recursivePartition src/matrixPowerRecursive.cpp:156-338)
+    const int wbl = (int)std::ceil(k / 2.0) - 1;
+    const std::vector<int>& g = out->groupPtr;
+    for (int h : out->hopeless) {
+        if (h - wbl < 0 || h + wbl + 1 >= (int)g.size()) return ERR_INTERNAL;
+        MPKChild c;
+        c.lo = g[h];
+        c.hi = g[h + 1];
+        c.group = h;
+        for (int r = -wbl; r <= wbl; ++r)
+            if (r != 0 && g[h + r + 1] > g[h + r]) c.boundaries.push_back({g[h + r], g[h + r + 1]});
+        LevelData cl;
+        std::vector<LevelData> bl;
+        if ((err = t->traverseBoundary(c.lo, c.hi, c.boundaries, &cl, &bl)) != OK) return err;
+        const int CL = cl.totalLevel();
+        std::vector<int> cptr(CL + 1, c.lo);
+        for (int l = 0; l < CL; ++l) cptr[l + 1] = cptr[l] + cl.levelRow[l];
+        // elem counts the boundary regions' entries of the level too
+        // (mtxPower::getElemUpperLimit src/matrixPower.cpp:88-99)
+        std::vector<double> celem(CL);
+        for (int l = 0; l < CL; ++l) {
+            long nz = cl.levelNnz[l];
+            for (const LevelData& b : bl) nz += b.levelNnz[l];
+            celem[l] = safety * (k + 1) * nz;
         }
+        if (!identifyHopelessRegions(violatedOf(celem, cutoff1), k, CL).empty()) return ERR_NOT_IMPLEMENTED;
+        std::vector<int> gl, hid;
+        consolidateLevels(cptr, celem, k, out->cacheElem, {}, &c.groupPtr, &gl, &hid);
+        out->children.push_back(std::move(c));
     }
-    out->groupPtr.push_back(out->levelPtr[L]);
-    out->groupLevel.push_back(L);
     return OK;
 }
 

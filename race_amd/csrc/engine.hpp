@@ -121,6 +121,14 @@ class Traverser {
     // ("L") or level n-1-i ("U"); the stable sort by level then keeps or
     // reverses the current order, composed into the permutation
     virtual int orderByRows(bool reverse, LevelData* out) = 0;
+    // The POWER traversal of a recursion stage (mtxPowerRecursive, a
+    // hopeless region [lo,hi) with boundary regions): boundary rows reached
+    // from the frontier take the level and are expanded but not counted,
+    // untouched boundary rows go to one extra last level, the range and
+    // every region are sorted by level (src/traverse_GAP_graphSoA.cpp:76-88,
+    // 184-197, 449-575); bout: one histogram per boundary region
+    virtual int traverseBoundary(int lo, int hi, const std::vector<std::pair<int, int>>& bounds,
+                                 LevelData* out, std::vector<LevelData>* bout) = 0;
     virtual const std::vector<int>& perm() const = 0;     // host mirror, perm[new]=old
     virtual const std::vector<int>& invPerm() const = 0;  // host mirror
     virtual int rowLen(int origRow) const = 0;            // graph row length
@@ -159,19 +167,41 @@ struct Schedule {
 // Phases of recursiveCall's execution (src/functionManager.cpp:199-265).
 Schedule flattenSchedule(const ZoneTree& zt, bool rev, int nrow);
 
-// Stage-0 MPK level groups (mtxPower::findPartition / consolidatePartition,
-// src/matrixPower.cpp:505-604,685-940).
+// MPK level groups (mtxPowerRecursive::findPartition / recursivePartition
+// src/matrixPowerRecursive.cpp:156-445 with mtxPower::findPartition /
+// identifyHopelessRegions / consolidatePartition src/matrixPower.cpp:147-940).
+struct MPKChild {
+    int lo = 0, hi = 0;              // rows of the hopeless group
+    int group = 0;                   // its stage-0 group
+    std::vector<int> groupPtr;       // the stage's own level groups over [lo,hi)
+    std::vector<std::pair<int, int>> boundaries;  // boundary regions traversed with it
+};
 struct MPKPartition {
     std::vector<int> levelPtr;    // BFS level -> first row
     std::vector<int> groupPtr;    // level group -> first row
     std::vector<int> groupLevel;  // level group -> first BFS level
     std::vector<int> levelNnz;
-    std::vector<int> hopeless;    // levels violating the cache by more than the cut-off
+    std::vector<int> violated;         // levels over the cache by more than the cut-off
+    std::vector<int> hopelessRegions;  // [start, end] level pairs (inclusive)
+    std::vector<int> hopeless;         // stage-0 group of each hopeless region
+    std::vector<MPKChild> children;    // the recursion stage of each hopeless group
     double cacheElem = 0, bytePerNnz = 0;
     int highestPower = 1;
 };
+// Hopeless regions of one node from the violated levels
+// (mtxPower::identifyHopelessRegions src/matrixPower.cpp:147-502).
+std::vector<int> identifyHopelessRegions(const std::vector<int>& violated, int k, int totalLevel);
+// Level groups around hopeless regions (mtxPower::consolidatePartition
+// src/matrixPower.cpp:685-880): groups (row pointers), first level per group,
+// group index of each hopeless region.
+void consolidateLevels(const std::vector<int>& levelPtr, const std::vector<double>& elem, int k,
+                       double cacheElem, const std::vector<int>& regions, std::vector<int>* groups,
+                       std::vector<int>* glevel, std::vector<int>* hid);
+// cutoff0 / cutoff1: cache-violation cut-offs of stage 0 and of the
+// recursion stage; a hopeless region inside a recursion stage returns
+// ERR_NOT_IMPLEMENTED
 int mpkPartition(Traverser* t, int n, long nnzInput, int highestPower, double cacheBytes,
-                 double safety, int cutoff, char mtxType, MPKPartition* out);
+                 double safety, int cutoff0, int cutoff1, char mtxType, MPKPartition* out);
 
 // Parse comma separated integers with atoi semantics (getEnv, utility.h:94-111).
 std::vector<int> envIntList(const char* name);

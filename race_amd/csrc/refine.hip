@@ -35,6 +35,15 @@ __global__ void k_update(double* __restrict__ x, const T* __restrict__ d, long n
     for (; i < n; i += stride) x[i] += (double)d[i];
 }
 
+// Device workspace of one refinement: the residual r and the correction d
+// (n * nrhs entries each, in the inner function's vector precision) and the
+// two squared norms.
+struct RefineWork {
+    double* norms = nullptr;
+    void* r = nullptr;
+    void* d = nullptr;
+};
+
 class RefineFn : public DeviceFn {
   public:
     int n = 0, nrhs = 1, sweeps = 1;
@@ -48,6 +57,16 @@ class RefineFn : public DeviceFn {
 
     ~RefineFn() override {
         if (hostNorms) (void)hipHostFree(hostNorms);
+    }
+
+    bool innerFp32() const { return inner->vecPrecision == RACE_FP32; }
+    size_t vecBytes() const { return (size_t)n * nrhs * (innerFp32() ? sizeof(float) : sizeof(double)); }
+    RefineWork own() {
+        RefineWork w;
+        w.norms = norms.p;
+        w.r = innerFp32() ? (void*)r32.p : (void*)r64.p;
+        w.d = innerFp32() ? (void*)d32.p : (void*)d64.p;
+        return w;
     }
 
     int init(const race_kernel_desc& d, DeviceFn* in, int innerSweeps) {
@@ -68,7 +87,7 @@ class RefineFn : public DeviceFn {
         if ((e = makeSellResidual(d, &res))) return e;
         RACE_HIP(norms.reserve(2));
         const size_t len = (size_t)n * nrhs;
-        if (inner->vecPrecision == RACE_FP32) {
+        if (innerFp32()) {
             RACE_HIP(r32.reserve(len));
             RACE_HIP(d32.reserve(len));
         } else {
@@ -80,14 +99,12 @@ class RefineFn : public DeviceFn {
     }
 
     // r = b - A x; relres = ||r|| / ||b|| when requested (synchronises)
-    int residual(hipStream_t st, double* relres) {
-        hipLaunchKernelGGL(k_zero<double>, dim3(1), dim3(64), 0, st, norms.p, 2l);
-        const bool f32 = inner->vecPrecision == RACE_FP32;
-        int e = res->run((const double*)x, (const double*)b, f32 ? (void*)r32.p : (void*)r64.p, f32, nrhs,
-                         norms.p, st);
+    int residual(const RefineWork& w, hipStream_t st, double* relres) {
+        hipLaunchKernelGGL(k_zero<double>, dim3(1), dim3(64), 0, st, w.norms, 2l);
+        int e = res->run((const double*)x, (const double*)b, w.r, innerFp32(), nrhs, w.norms, st);
         if (e) return e;
         if (relres) {
-            RACE_HIP(hipMemcpyAsync(hostNorms, norms.p, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+            RACE_HIP(hipMemcpyAsync(hostNorms, w.norms, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
             RACE_HIP(hipStreamSynchronize(st));
             *relres = hostNorms[1] > 0 ? std::sqrt(hostNorms[0] / hostNorms[1]) : std::sqrt(hostNorms[0]);
             lastRelres = *relres;
@@ -98,18 +115,18 @@ class RefineFn : public DeviceFn {
     // correction sweeps on A d = r and x += d (residual already in r).  The
     // inner function is pointed at the workspace for the sweeps and back at
     // its own vectors afterwards, so it stays usable on its own.
-    int correct(hipStream_t st) {
+    int correct(const RefineWork& w, hipStream_t st) {
         const long len = (long)n * nrhs;
         const unsigned g = gridFor(len);
-        const bool f32 = inner->vecPrecision == RACE_FP32;
+        const bool f32 = innerFp32();
         if (f32)
-            hipLaunchKernelGGL(k_zero<float>, dim3(g), dim3(kBlock), 0, st, d32.p, len);
+            hipLaunchKernelGGL(k_zero<float>, dim3(g), dim3(kBlock), 0, st, (float*)w.d, len);
         else
-            hipLaunchKernelGGL(k_zero<double>, dim3(g), dim3(kBlock), 0, st, d64.p, len);
+            hipLaunchKernelGGL(k_zero<double>, dim3(g), dim3(kBlock), 0, st, (double*)w.d, len);
         void* const ownX = inner->x;
         void* const ownB = inner->b;
-        inner->x = f32 ? (void*)d32.p : (void*)d64.p;
-        inner->b = f32 ? (void*)r32.p : (void*)r64.p;
+        inner->x = w.d;
+        inner->b = w.r;
         int e = OK;
         for (int s = 0; s < sweeps && !e; ++s) {
             e = inner->run(false, st);
@@ -119,30 +136,32 @@ class RefineFn : public DeviceFn {
         inner->b = ownB;
         if (e) return e;
         totalSweeps += sweeps;
-        if (inner->vecPrecision == RACE_FP32)
-            hipLaunchKernelGGL(k_update<float>, dim3(g), dim3(kBlock), 0, st, (double*)x, d32.p, len);
+        if (f32)
+            hipLaunchKernelGGL(k_update<float>, dim3(g), dim3(kBlock), 0, st, (double*)x, (const float*)w.d, len);
         else
-            hipLaunchKernelGGL(k_update<double>, dim3(g), dim3(kBlock), 0, st, (double*)x, d64.p, len);
+            hipLaunchKernelGGL(k_update<double>, dim3(g), dim3(kBlock), 0, st, (double*)x, (const double*)w.d,
+                               len);
         RACE_HIP(hipGetLastError());
         return OK;
     }
 
     // one refinement step (asynchronous)
     int run(bool, hipStream_t st) override {
-        int e = residual(st, nullptr);
+        const RefineWork w = own();
+        int e = residual(w, st, nullptr);
         if (e) return e;
-        return correct(st);
+        return correct(w, st);
     }
 
-    int solve(double tol, int maxOuter, hipStream_t st, int* outer, double* relres) {
+    int solve(const RefineWork& w, double tol, int maxOuter, hipStream_t st, int* outer, double* relres) {
         double rr = 0;
         int it = 0;
-        int e = residual(st, &rr);
+        int e = residual(w, st, &rr);
         if (e) return e;
         while (rr > tol && it < maxOuter) {
-            if ((e = correct(st))) return e;
+            if ((e = correct(w, st))) return e;
             ++it;
-            if ((e = residual(st, &rr))) return e;
+            if ((e = residual(w, st, &rr))) return e;
         }
         if (outer) *outer = it;
         if (relres) *relres = rr;
@@ -164,7 +183,7 @@ int refineSolve(DeviceFn* f, double tol, int maxOuter, hipStream_t st, int* oute
     RefineFn* r = dynamic_cast<RefineFn*>(f);
     if (!r) return ERR_INVALID_ARG;
     const long before = r->totalSweeps;
-    int e = r->solve(tol, maxOuter, st, outer, relres);
+    int e = r->solve(r->own(), tol, maxOuter, st, outer, relres);
     if (sweeps) *sweeps = r->totalSweeps - before;
     return e;
 }
@@ -172,7 +191,7 @@ int refineSolve(DeviceFn* f, double tol, int maxOuter, hipStream_t st, int* oute
 int refineResidual(DeviceFn* f, hipStream_t st, double* relres) {
     RefineFn* r = dynamic_cast<RefineFn*>(f);
     if (!r) return ERR_INVALID_ARG;
-    return r->residual(st, relres);
+    return r->residual(r->own(), st, relres);
 }
 
 }  // namespace race

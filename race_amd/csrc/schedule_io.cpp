@@ -5,8 +5,10 @@
 // highestSubPower), u64 pattern hash, i32 vectors as (i32 count, data):
 // perm, inv; i32 hasTree, then per leaf (i32 count of leaves first) seven
 // i32 fields and the vectors valueZ, children; i32 hasMpk, then the vectors
-// levelPtr, groupPtr, groupLevel, levelNnz, hopeless, f64 cacheElem,
-// f64 bytePerNnz, i32 highestPower; u64 FNV-1a checksum of all bytes before.
+// levelPtr, groupPtr, groupLevel, levelNnz, hopeless (the group of each
+// hopeless region), f64 cacheElem, f64 bytePerNnz, i32 highestPower; u64
+// FNV-1a checksum of all bytes before.  The recursion stage's own groups of
+// a hopeless region are not stored (the permutation they produced is).
 #include "schedule_io.hpp"
 
 #include <cstring>
@@ -256,8 +258,12 @@ int decodeSchedule(const void* buf, size_t size, SavedSchedule* out) {
     }
     if (s.hasMpk) {
         const MPKPartition& m = s.mpk;
-        if (!rowPointers(m.levelPtr, n) || !rowPointers(m.groupPtr, n) || !m.hopeless.empty())
-            return ERR_INVALID_ARG;
+        if (!rowPointers(m.levelPtr, n) || !rowPointers(m.groupPtr, n)) return ERR_INVALID_ARG;
+        // hopeless: the group of each hopeless region, ascending
+        for (size_t i = 0; i < m.hopeless.size(); ++i)
+            if (m.hopeless[i] < 0 || m.hopeless[i] + 1 >= (int)m.groupPtr.size() ||
+                (i > 0 && m.hopeless[i] <= m.hopeless[i - 1]))
+                return ERR_INVALID_ARG;
         // groupLevel: level group -> first BFS level, ending with the level count
         const int L = (int)m.levelNnz.size();
         if (m.levelPtr.size() != (size_t)L + 1 || m.groupLevel.size() != m.groupPtr.size() ||

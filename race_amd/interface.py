@@ -17,7 +17,7 @@ from . import _lib as L
 from ._lib import (BF16, FP32, FP64, K_GS, K_KACZ, K_MPK, K_MPK_HIGHEST, K_SPMTV, K_SPMV, K_SYMMSPMV, LB_NNZ,
                    LB_ROW, ONE, POWER, SCATTER, TWO, TWO_BLOCK, RaceError)
 
-__all__ = ["Interface", "DeviceArray", "lb_split", "split_threads", "RaceError", "ONE", "TWO", "POWER", "K_SYMMSPMV",
+__all__ = ["Interface", "DeviceArray", "lb_split", "split_threads", "mpk_hopeless_regions", "mpk_consolidate", "RaceError", "ONE", "TWO", "POWER", "K_SYMMSPMV",
            "K_GS", "K_KACZ", "K_SPMV", "K_MPK", "K_SPMTV", "K_MPK_HIGHEST", "FP64", "FP32", "BF16", "LB_NNZ", "LB_ROW",
            "TWO_BLOCK", "device_count", "schedule_info", "rcm"]
 
@@ -310,6 +310,14 @@ class Interface:
         L.check(self._L.race_get_tree(self.h, ctypes.byref(n), _ip(d), n.value), "tree")
         return d.reshape(n.value, 6)
 
+    def mpkChildren(self):
+        """recursion-stage leaves: list of (lo, hi, stage-0 group, own groups)"""
+        n = ctypes.c_int()
+        L.check(self._L.race_get_mpk_children(self.h, ctypes.byref(n), None, 0), "mpkChildren")
+        d = np.zeros(4 * max(n.value, 1), np.int32)
+        L.check(self._L.race_get_mpk_children(self.h, ctypes.byref(n), _ip(d), n.value), "mpkChildren")
+        return [tuple(int(v) for v in d[4 * i:4 * i + 4]) for i in range(n.value)]
+
     def mpkGroups(self):
         n = ctypes.c_int()
         L.check(self._L.race_get_mpk_groups(self.h, ctypes.byref(n), None, 0), "mpkGroups")
@@ -537,3 +545,29 @@ def split_threads(weights, total, parent_threads):
     L.check(L.load().race_split_threads(_ip(w), len(w), int(total), int(parent_threads), _ip(out)),
             "race_split_threads")
     return out[:len(w)].tolist()
+
+
+def mpk_hopeless_regions(violated, k, total_level):
+    """mtxPower::identifyHopelessRegions through the C ABI (host only)."""
+    v = _i32(violated)
+    cap = 2 * len(v) + 4
+    out = np.zeros(cap, np.int32)
+    n = ctypes.c_int()
+    L.check(L.load().race_mpk_hopeless_regions(_ip(v), len(v), int(k), int(total_level), _ip(out), cap,
+                                               ctypes.byref(n)), "race_mpk_hopeless_regions")
+    return out[:n.value].tolist()
+
+
+def mpk_consolidate(level_ptr, elem, k, cache_elem, regions):
+    """mtxPower::consolidatePartition through the C ABI (host only):
+    (group row pointers, group first levels, group of each region)."""
+    lp = _i32(level_ptr)
+    el = np.ascontiguousarray(elem, np.float64)
+    rg = _i32(regions if len(regions) else [0])
+    cap = len(lp) + 4
+    g, gl, h = np.zeros(cap, np.int32), np.zeros(cap, np.int32), np.zeros(cap, np.int32)
+    ng, nh = ctypes.c_int(), ctypes.c_int()
+    L.check(L.load().race_mpk_consolidate(_ip(lp), el.ctypes.data_as(L.f64p), len(el), int(k), float(cache_elem),
+                                          _ip(rg), len(regions), _ip(g), _ip(gl), _ip(h), cap, ctypes.byref(ng),
+                                          ctypes.byref(nh)), "race_mpk_consolidate")
+    return g[:ng.value + 1].tolist(), gl[:ng.value + 1].tolist(), h[:nh.value].tolist()

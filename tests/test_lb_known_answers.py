@@ -284,5 +284,58 @@ def test_mpk_hopeless_level_by_hand(oracle):
     part = oracle.mpk_partition(G, 3, 100.0)
     assert abs(part["byte_per_nnz"] - bpn) < 1e-12
     assert list(part["levelNnz"]) == [41, 80]
-    assert part["hopeless"] == [0, 1]
-    assert oracle.mpk_partition(G, 3, 100.0, cutoff=100)["hopeless"] == [1]
+    assert part["violated"] == [0, 1]
+    # the run [0, 1] lies within k-1 = 2 levels of the first level: the
+    # clamp (src/matrixPower.cpp:320-334) drops it, so no hopeless region
+    assert part["hopelessRegions"] == [] and part["hopelessGroups"] == []
+    assert oracle.mpk_partition(G, 3, 100.0, cutoff=100)["violated"] == [1]
+
+
+# ---------------------------------------------------------------------------
+# hopeless regions and the level grouping around them
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("violated,k,L,expect", [
+    # a single violated level is no region (only runs of consecutive levels
+    # get a flag, src/matrixPower.cpp:149-217)
+    ([7], 3, 20, []),
+    ([3, 7, 12], 3, 20, []),
+    # a run of two
+    ([5, 6], 3, 20, [5, 6]),
+    # two runs 1 level apart merge when k-1 > 1 (src/matrixPower.cpp:225-250)
+    ([5, 6, 8, 9], 4, 30, [5, 9]),
+    ([5, 6, 8, 9], 2, 30, [5, 6, 8, 9]),
+    # adjacent runs with one gap level: kept apart for k = 2, merged for 3
+    ([3, 4, 6, 7], 2, 20, [3, 4, 6, 7]),
+    ([3, 4, 6, 7], 3, 20, [3, 7]),
+    # isolated levels between runs are ignored
+    ([2, 5, 6, 9, 12, 13, 14], 2, 30, [5, 6, 12, 14]),
+    # start moved to k-1 (src/matrixPower.cpp:320-334) ...
+    ([1, 2, 3, 4], 4, 20, [3, 4]),
+    # ... or the region dropped when nothing is left of it
+    ([1, 2], 4, 20, []),
+    # end moved to L-1-(k-1) (src/matrixPower.cpp:336-352)
+    ([15, 16, 17, 18], 3, 20, [15, 17]),
+    # both ends
+    (list(range(1, 19)), 3, 20, [2, 17]),
+])
+def test_hopeless_regions_known_answers(oracle, violated, k, L, expect):
+    assert oracle.identify_hopeless_regions(violated, k, L) == expect
+    assert R.mpk_hopeless_regions(violated, k, L) == expect
+
+
+@pytest.mark.parametrize("cache,groups,hid", [
+    # a cache no sum of levels reaches: only the forced cuts.  k = 3 (k-1 =
+    # 2), region [5, 6]: every level of [3, 5] and of [7, 8] starts a group,
+    # level 9 = end + k starts one (and ends the region's rules), levels
+    # 5..6 are one group (src/matrixPower.cpp:769-858): group 3 is hopeless
+    (1e9, [0, 3, 4, 5, 7, 8, 9, 12], [3]),
+    # elem 1 per level, cacheElem 2.5: a cut also where the running sum
+    # reaches 2.5 (before level 2 and level 11); the region is group 4
+    (2.5, [0, 2, 3, 4, 5, 7, 8, 9, 11, 12], [4]),
+])
+def test_consolidation_around_a_hopeless_region(oracle, cache, groups, hid):
+    lp = list(range(13))  # one row per level, 12 levels
+    elem = [1.0] * 12
+    g, gl, h = oracle.consolidate_levels(lp, elem, 3, cache, [5, 6])
+    assert g == groups and gl == groups and h == hid
+    assert R.mpk_consolidate(lp, elem, 3, cache, [5, 6]) == (groups, groups, hid)

@@ -496,3 +496,92 @@ def test_parallel_phases_equal_the_serial_leaf_order(oracle, kind, dist, T):
         nt = oracle.run_phases(okind, prp, pcol, pval, xb, bb, pp, calls, nrhs=2, nthreads=8)
         assert nt == 8
         assert np.array_equal(xa, xb) and np.array_equal(ba, bb)
+
+
+def _grid2d(g):
+    rp, col = [0], []
+    for i in range(g):
+        for j in range(g):
+            for di, dj in ((0, 0), (-1, 0), (1, 0), (0, -1), (0, 1)):
+                a, b = i + di, j + dj
+                if 0 <= a < g and 0 <= b < g:
+                    col.append(a * g + b)
+            rp.append(len(col))
+    return np.array(rp, np.int32), np.array(col, np.int32)
+
+
+@pytest.mark.parametrize("g,lv,radius", [(12, (6, 11), 1), (10, (4, 9), 2), (9, (3, 3), 1)])
+def test_traverse_boundary_is_a_bfs_of_the_region_and_its_boundary(oracle, g, lv, radius):
+    """The recursion stage's traversal (Traverse with boundary regions,
+    src/traverse_GAP_graphSoA.cpp:76-88, 184-197, 449-575), checked against
+    an independent computation: after a stage-0 traversal of a 2D grid, the
+    rows of levels lv[0]..lv[1] are traversed again with `radius` levels on
+    each side as boundary regions.  Levels = unweighted shortest paths from
+    the range's first row inside the subgraph of range + boundary rows
+    (scipy); every region is stably sorted by level; the histograms count
+    the range and each region."""
+    rp, col = _grid2d(g)
+    G = oracle.Graph(rp, col, symm_hint=True)
+    lr, ln = G.traverse(oracle.POWER, 0, G.n)
+    lptr = np.concatenate([[0], np.cumsum(lr)])
+    lo, hi = int(lptr[lv[0]]), int(lptr[lv[1] + 1])
+    bounds = []
+    for r in range(-radius, radius + 1):
+        L = (lv[0] + r) if r < 0 else (lv[1] + r)
+        if r != 0 and 0 <= L < len(lr):
+            bounds.append((int(lptr[L]), int(lptr[L + 1])))
+    before = G.perm.copy()
+    # the permuted pattern before the call, restricted to range + boundaries
+    prp, pcol, _ = M.permute(rp, col, None, before, G.inv)
+    nodes = np.concatenate([np.arange(lo, hi)] + [np.arange(a, b) for a, b in bounds])
+    idx = {int(v): i for i, v in enumerate(nodes)}
+    rows, cols = [], []
+    for v in nodes:
+        for c in pcol[prp[v]:prp[v + 1]]:
+            if int(c) in idx:
+                rows.append(idx[int(v)])
+                cols.append(idx[int(c)])
+    A = sp.csr_matrix((np.ones(len(rows)), (rows, cols)), shape=(len(nodes), len(nodes)))
+    d = csg.shortest_path(A, unweighted=True, indices=idx[lo])
+    assert np.all(np.isfinite(d[:hi - lo]))  # the band is connected
+    level = {int(v): int(d[i]) if np.isfinite(d[i]) else -1 for i, v in enumerate(nodes)}
+    top = max(level.values()) + 1  # totalLevel: one past the deepest level
+    unreached = [v for v in level if level[v] < 0]
+    for v in unreached:
+        level[v] = top
+    total = top + (1 if unreached else 0)
+    clr, cln, regs = G.traverse_boundary(lo, hi, bounds)
+    assert len(clr) == total
+    lens = np.diff(rp)
+    exp_lr = np.bincount([level[v] for v in range(lo, hi)], minlength=total)
+    exp_ln = np.bincount([level[v] for v in range(lo, hi)], weights=[lens[before[v]] for v in range(lo, hi)],
+                         minlength=total)
+    assert np.array_equal(clr, exp_lr) and np.array_equal(cln, exp_ln.astype(int))
+    for (a, b), (blr, bln) in zip(bounds, regs):
+        assert np.array_equal(blr, np.bincount([level[v] for v in range(a, b)], minlength=total))
+    # every region stably sorted by level, composed into the permutation
+    expect = before.copy()
+    for a, b in [(lo, hi)] + bounds:
+        order = sorted(range(a, b), key=lambda v: level[v])
+        expect[a:b] = before[order]
+    assert np.array_equal(G.perm, expect)
+    assert np.array_equal(G.inv[G.perm], np.arange(G.n))
+
+
+def test_traverse_boundary_path_by_hand(oracle):
+    """A path of 20 rows, range [2, 18) with boundary rows {1} and {18}:
+    from row 2, row 1 (boundary) and row 3 are level 1, row r is level r-2,
+    row 18 (boundary) is level 16; row 0 is outside both and never reached.
+    17 levels (the last holds only row 18), the permutation is unchanged."""
+    n = 20
+    rp, col = [0], []
+    for r in range(n):
+        col += [c for c in (r - 1, r, r + 1) if 0 <= c < n]
+        rp.append(len(col))
+    G = oracle.Graph(np.array(rp, np.int32), np.array(col, np.int32), symm_hint=True)
+    clr, cln, regs = G.traverse_boundary(2, 18, [(1, 2), (18, 19)])
+    assert list(clr) == [1] * 16 + [0]
+    assert list(cln) == [3] * 16 + [0]
+    assert list(regs[0][0]) == [0, 1] + [0] * 15
+    assert list(regs[1][0]) == [0] * 16 + [1]
+    assert np.array_equal(G.perm, np.arange(n))
