@@ -185,6 +185,19 @@ int race_refine(race_handle h, int id, double tol, int max_outer, int *outer, lo
                 double *relres);
 /* relative residual ||b - A x|| / ||b|| of a refinement function's vectors */
 int race_residual(race_handle h, int id, double *relres);
+/* race_refine without allocation.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  The residual, the
+ * correction and the norms live in one static device array of 64 MiB per
+ * GPU (race_refine_noalloc_max_bytes), held until the process ends: the
+ * workspace needs 256 + 2 * nrow * nrhs * (8 for FP64 inner sweeps, 4 for
+ * FP32 / BF16) bytes (each vector rounded up to 256), and larger problems
+ * give RACE_ERR_INVALID_ARG.  Calls on one GPU share that array: calls that
+ * overlap on one GPU (other threads, other streams) race.  Results equal
+ * race_refine's. */
+int race_refine_noalloc(race_handle h, int id, double tol, int max_outer, int *outer, long *sweeps,
+                        double *relres);
+int race_refine_noalloc_max_bytes(size_t *bytes);
 
 /* Interface::executeFunction (src/interface.cpp:541-572): synchronous. */
 int race_execute(race_handle h, int id, int rev);

@@ -78,6 +78,10 @@ PROTOTYPES = {
     "race_refine": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, i32p,
                                    ctypes.POINTER(ctypes.c_long), f64p]),
     "race_residual": (ctypes.c_int, [vp, ctypes.c_int, f64p]),
+    "race_refine_noalloc": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                           ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_long),
+                                           ctypes.POINTER(ctypes.c_double)]),
+    "race_refine_noalloc_max_bytes": (ctypes.c_int, [ctypes.POINTER(ctypes.c_size_t)]),
     "race_execute": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int]),
     "race_execute_async": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, vp]),
     "race_set_power": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int]),

@@ -517,6 +517,19 @@ int race_refine(race_handle h, int id, double tol, int max_outer, int* outer, lo
     return refineSolve(h->funcs[id].dev.get(), tol, max_outer, h->stream, outer, sweeps, relres);
 }
 
+int race_refine_noalloc(race_handle h, int id, double tol, int max_outer, int* outer, long* sweeps,
+                        double* relres) {
+    if (checkHandle(h) || id < 0 || id >= (int)h->funcs.size() || !h->funcs[id].dev)
+        return ERR_INVALID_ARG;
+    return refineSolveStatic(h->funcs[id].dev.get(), tol, max_outer, h->stream, outer, sweeps, relres);
+}
+
+int race_refine_noalloc_max_bytes(size_t* bytes) {
+    if (!bytes) return ERR_INVALID_ARG;
+    *bytes = refineStaticScratchBytes();
+    return OK;
+}
+
 int race_residual(race_handle h, int id, double* relres) {
     if (checkHandle(h) || id < 0 || id >= (int)h->funcs.size() || !h->funcs[id].dev || !relres)
         return ERR_INVALID_ARG;

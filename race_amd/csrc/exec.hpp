@@ -81,6 +81,11 @@ int makeRefineFn(const race_kernel_desc& d, DeviceFn* inner, int sweeps, std::un
 int refineSolve(DeviceFn* f, double tol, int maxOuter, hipStream_t st, int* outer, long* sweeps,
                 double* relres);
 int refineResidual(DeviceFn* f, hipStream_t st, double* relres);
+// refineSolve with its workspace in the library's static device array (no
+// allocation; calls on one GPU share that array)
+int refineSolveStatic(DeviceFn* f, double tol, int maxOuter, hipStream_t st, int* outer, long* sweeps,
+                      double* relres);
+size_t refineStaticScratchBytes();
 // Gauss-Seidel / Kaczmarz with one workgroup per leaf (exec.hip)
 int makeWaveFn(const race_kernel_desc& d, const ExecContext& ctx, std::unique_ptr<DeviceFn>* out);
 // SELL-C SpMV (one right-hand side) and level-blocked matrix powers (sell.hip)

@@ -13,6 +13,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <mutex>
 
 #include "exec.hpp"
 #include "exec_common.hpp"
@@ -192,6 +194,67 @@ int refineResidual(DeviceFn* f, hipStream_t st, double* relres) {
     RefineFn* r = dynamic_cast<RefineFn*>(f);
     if (!r) return ERR_INVALID_ARG;
     return r->residual(r->own(), st, relres);
+}
+
+// ---------------------------------------------------------------------------
+// The refinement without allocation.  Its workspace (norms, r, d) lives in
+// one static device array of kScratchBytes; every GPU the library runs on
+// holds its own copy until the process ends, and calls on one GPU share it,
+// so concurrent calls on one GPU race.  Host code never uses the array's
+// name as an address: the device address comes from hipGetSymbolAddress,
+// once per device, with that device current.
+namespace {
+constexpr size_t kScratchBytes = size_t(64) << 20;
+constexpr size_t kScratchAlign = 256;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+std::mutex scratchLock;
+void* scratchPtr[kMaxDevices] = {};
+
+int scratchOfCurrentDevice(void** out) {
+    int dev = -1;
+    RACE_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return ERR_INTERNAL;
+    std::lock_guard<std::mutex> g(scratchLock);
+    if (!scratchPtr[dev]) {
+        void* p = nullptr;
+        RACE_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return ERR_INTERNAL;
+        scratchPtr[dev] = p;
+    }
+    *out = scratchPtr[dev];
+    return OK;
+}
+
+size_t alignUp(size_t v) { return (v + kScratchAlign - 1) / kScratchAlign * kScratchAlign; }
+}  // namespace
+
+size_t refineStaticScratchBytes() { return kScratchBytes; }
+
+int refineSolveStatic(DeviceFn* f, double tol, int maxOuter, hipStream_t st, int* outer, long* sweeps,
+                      double* relres) {
+    RefineFn* r = dynamic_cast<RefineFn*>(f);
+    if (!r) return ERR_INVALID_ARG;
+    const size_t vb = alignUp(r->vecBytes());
+    const size_t need = kScratchAlign + 2 * vb;
+    if (need > kScratchBytes) {
+        std::fprintf(stderr,
+                     "race_refine_noalloc: %d rows x %d right-hand sides need %zu bytes of workspace, more "
+                     "than the %zu bytes of the static workspace; use race_refine\n",
+                     r->n, r->nrhs, need, kScratchBytes);
+        return ERR_INVALID_ARG;
+    }
+    void* base = nullptr;
+    int e = scratchOfCurrentDevice(&base);
+    if (e) return e;
+    RefineWork w;
+    w.norms = (double*)base;
+    w.r = (unsigned char*)base + kScratchAlign;
+    w.d = (unsigned char*)base + kScratchAlign + vb;
+    const long before = r->totalSweeps;
+    e = r->solve(w, tol, maxOuter, st, outer, relres);
+    if (sweeps) *sweeps = r->totalSweeps - before;
+    return e;
 }
 
 }  // namespace race
