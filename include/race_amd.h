@@ -46,7 +46,12 @@ enum race_pin_method { RACE_FILL = 0, RACE_SCATTER = 1 };
 typedef struct race_handle_s *race_handle;
 
 /* Interface::Interface (include/interface.h:127, src/interface.cpp:33-36).
- * rowPtr/col are copied (the reference borrows them). */
+ * rowPtr/col are copied (the reference borrows them).  RACE_MAX_DENSE (read
+ * here, as the reference's Graph does, src/graph_SoA.cpp:321-435): rows
+ * longer than that many times nnz/nrow (integer division) leave the graph
+ * and run last, one after the other, in every sweep (a last phase of one
+ * leaf, [n', n) of the permutation); colouring only (RACE_ERR_NOT_IMPLEMENTED
+ * for POWER, whose executor has no serial part). */
 int race_create(race_handle *out, int nrow, int nthreads, int dist, const int *rowPtr,
                 const int *col, int symm_hint, int smt, int pin_method, const int *init_perm,
                 const int *init_inv_perm, int d2_method, int lb_target);
@@ -282,7 +287,8 @@ int race_rcm(int n, const int *rowPtr, const int *col, int *perm, int *inv_perm)
  * src/matrixPowerRecursive.cpp:447-490).  The byte string holds what
  * RACEColor computed: permutation, zone tree (colouring) or level groups
  * (POWER), thread count, and a hash of the matrix pattern it was built on.
- * race_schedule_save: buf NULL returns the size only; needs a coloured handle.
+ * race_schedule_save: buf NULL returns the size only; needs a coloured handle
+ * (RACE_ERR_NOT_IMPLEMENTED when RACE_MAX_DENSE moved rows to a serial tail).
  * race_schedule_load: instead of RACEColor, on a handle created with the
  * same matrix pattern, nrow, nthreads, dist, d2 method and LB target, and
  * before any function is registered; anything else, a damaged byte string

@@ -524,8 +524,9 @@ void or_run_calls(int kind, int n, const int *rp, const int *col,
 }
 
 /* The leaves of each phase (phase_ptr[p] .. phase_ptr[p+1] into calls) run
- * concurrently on OpenMP threads, phases one after the other: how RACE runs
- * them on the host (recursiveCall, src/functionManager.cpp:199-265).  Leaves
+ * concurrently on OpenMP threads, each thread a contiguous run of leaves
+ * (RACE's threads own neighbouring zones), phases one after the other: how
+ * RACE runs them on the host (recursiveCall, src/functionManager.cpp:199-265).  Leaves
  * of one phase are distance-k independent, so the result equals
  * or_run_calls over the same calls in order. */
 void or_run_phases(int kind, int n, const int *rp, const int *col, const double *val, double *x,
@@ -537,7 +538,7 @@ void or_run_phases(int kind, int n, const int *rp, const int *col, const double 
 #endif
     for (int p = 0; p < nphases; ++p) {
 #ifdef _OPENMP
-#pragma omp for schedule(dynamic, 1)
+#pragma omp for schedule(static)
 #endif
         for (int c = phase_ptr[p]; c < phase_ptr[p + 1]; ++c)
             or_run_calls(kind, n, rp, col, val, x, b, 1, calls + 2 * c, nrhs);

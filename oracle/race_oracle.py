@@ -167,11 +167,12 @@ class Graph:
     """RACE::Graph (SoA, RACE_PERMUTE_ON_FLY): borrowed CRS in original
     numbering plus totalPerm/totalInvPerm (src/graph_SoA.cpp:48-216,437-477)."""
 
-    def __init__(self, rowPtr, col, symm_hint=False, init_perm=None, init_inv_perm=None):
+    def __init__(self, rowPtr, col, symm_hint=False, init_perm=None, init_inv_perm=None, max_dense=None):
         rowPtr = np.ascontiguousarray(rowPtr, dtype=np.int32)
         col = np.ascontiguousarray(col, dtype=np.int32)
         n = len(rowPtr) - 1
         self.n = n
+        self.nfull = n
         self.nnz_input = int(rowPtr[-1])  # Graph::NNZ = rowPtr[NROW] of the input
         if not symm_hint:
             cap = int(rowPtr[-1]) * 2 + 16
@@ -187,7 +188,48 @@ class Graph:
         else:
             self.perm = np.arange(n, dtype=np.int32)
             self.inv = np.arange(n, dtype=np.int32)
-        self.ws = lib().or_ws_create(n)
+        self.graph_rows = None
+        self.dense_rows = []
+        if max_dense is not None:
+            self._split_dense(float(max_dense))
+        self.ws = lib().or_ws_create(self.n)
+
+    def _split_dense(self, max_dense):
+        """Graph::getStatistics + permuteAndRemoveSerialPart
+        (src/graph_SoA.cpp:321-435, the build without RACE_PERMUTE_ON_FLY):
+        rows longer than max_dense x (NNZ / NROW, integer division) move, in
+        the current order, to a serial tail; the graph keeps the other rows
+        in order and drops every entry pointing to a tail row."""
+        n = self.n
+        mean = self.nnz_input // n
+        lens = np.diff(self.rowPtr)
+        cur = self.perm  # current row r = original row perm[r]
+        dense = [int(o) for o in cur if lens[o] > max_dense * mean]
+        if not dense:
+            return
+        keep = [int(o) for o in cur if not lens[o] > max_dense * mean]
+        new_id = np.full(n, -1, np.int64)
+        new_id[keep] = np.arange(len(keep))
+        rp, cl = [0], []
+        for o in keep:
+            for c in self.col[self.rowPtr[o]:self.rowPtr[o + 1]]:
+                if new_id[c] >= 0:
+                    cl.append(int(new_id[c]))
+            rp.append(len(cl))
+        self.rowPtr = np.array(rp, np.int32)
+        self.col = np.array(cl, np.int32)
+        self.graph_rows = np.array(keep, np.int32)
+        self.dense_rows = dense
+        self.n = len(keep)
+        self.perm = np.arange(self.n, dtype=np.int32)
+        self.inv = np.arange(self.n, dtype=np.int32)
+
+    def full_perm(self):
+        """perm[new] = old over all rows: the graph rows in their RACE
+        order, then the dense tail"""
+        if self.graph_rows is None:
+            return self.perm.copy()
+        return np.concatenate([self.graph_rows[self.perm], np.array(self.dense_rows, np.int32)])
 
     def __del__(self):
         try:
@@ -655,7 +697,11 @@ def level_recursion(graph, nthreads, dist, d2=TWO_BLOCK, lb_target=LB_NNZ, eff_l
 def leaf_calls(zt, rev=False, serial_part=None):
     """Serialised order of recursiveCall's leaf invocations (start,end).
     Concurrent leaves (same parent, same colour block) are independent, so
-    any serialisation gives the reference's results."""
+    any serialisation gives the reference's results.  serial_part (lo, hi):
+    the dense-row tail, called last in both directions
+    (FuncManager::Run src/functionManager.cpp:1493-1500; reversed here as
+    (hi-1, lo-1), the reverse-leaf convention of recursiveCall -- the
+    reference passes (hi, lo), one row off)."""
     out = []
     bpt = zt.bpt
 
@@ -673,6 +719,9 @@ def leaf_calls(zt, rev=False, serial_part=None):
                     for tid in range(n):
                         rec(ch[2 * sb] + bpt * tid + b)
     rec(0)
+    if serial_part is not None and serial_part[1] > serial_part[0]:
+        lo, hi = serial_part
+        out.append((hi - 1, lo - 1) if rev else (lo, hi))
     return out
 
 

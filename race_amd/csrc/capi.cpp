@@ -11,6 +11,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -85,6 +86,11 @@ void symmetrizePattern(int n, const int* rp, const int* col, std::vector<int>& o
 
 struct race_handle_s {
     int nrow = 0, requested = 1, dist = DIST_ONE, d2 = TWO_BLOCK, lbTarget = LB_NNZ;
+    // rows the colouring traverses: nrow minus the dense rows moved to the
+    // serial tail (RACE_MAX_DENSE); graphRows[q] = original row of graph row
+    // q, denseRows = original rows of the tail, in order
+    int nGraph = 0;
+    std::vector<int> graphRows, denseRows;
     int smt = 1, pin = 1;
     bool symmHint = false;
     std::vector<int> rowPtr, col;  // graph pattern used for traversal (copied)
@@ -254,6 +260,47 @@ int race_create(race_handle* out, int nrow, int nthreads, int dist, const int* r
         h->initPerm.assign(init_perm, init_perm + nrow);
         h->initInv.assign(init_inv_perm, init_inv_perm + nrow);
     }
+    h->nGraph = nrow;
+    // Dense rows (RACE_MAX_DENSE, Graph::getStatistics /
+    // permuteAndRemoveSerialPart src/graph_SoA.cpp:321-435, 173-200): rows
+    // longer than RACE_MAX_DENSE x (nnz / nrow, integer division) go to a
+    // serial tail after the coloured rows, in the row order of the initial
+    // permutation, and the graph loses them and every entry pointing to them.
+    // This follows the reference built without RACE_PERMUTE_ON_FLY; with it
+    // (its default) the reference reads past its arrays (it skips columns by
+    // their unpermuted index, src/traverse_GAP_graphSoA.cpp:161, and warns
+    // that the removal is not implemented, src/graph_SoA.cpp:389).
+    if (const char* md = std::getenv("RACE_MAX_DENSE")) {
+        if (dist == DIST_POWER) return ERR_NOT_IMPLEMENTED;  // the power executor has no serial part
+        const double maxDense = std::atof(md);
+        const int mean = (int)(h->nnzInput / nrow);
+        std::vector<int> cur(nrow), newId(nrow, -1);
+        for (int r = 0; r < nrow; ++r) cur[r] = init_perm ? init_perm[r] : r;
+        for (int r = 0; r < nrow; ++r) {
+            const int o = cur[r];
+            if (h->rowPtr[o + 1] - h->rowPtr[o] > maxDense * mean) {
+                h->denseRows.push_back(o);
+            } else {
+                newId[o] = (int)h->graphRows.size();
+                h->graphRows.push_back(o);
+            }
+        }
+        if (!h->denseRows.empty()) {
+            std::vector<int> rp(1, 0), cl;
+            for (int o : h->graphRows) {
+                for (int j = h->rowPtr[o]; j < h->rowPtr[o + 1]; ++j)
+                    if (newId[h->col[j]] >= 0) cl.push_back(newId[h->col[j]]);
+                rp.push_back((int)cl.size());
+            }
+            h->rowPtr.swap(rp);
+            h->col.swap(cl);
+            h->nGraph = (int)h->graphRows.size();
+            h->initPerm.clear();  // applied: graph row q is original row graphRows[q]
+            h->initInv.clear();
+        } else {
+            h->graphRows.clear();
+        }
+    }
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) return ERR_INTERNAL;
     *out = h.release();
     return OK;
@@ -268,11 +315,40 @@ int race_destroy(race_handle h) {
     return OK;
 }
 
+// Full-length permutation (graph rows, then the dense tail) and the colour
+// schedules, with the tail as one last phase in both directions: the
+// reference calls the serial part after the coloured part, f(n', n) forward
+// and reverse (FuncManager::Run src/functionManager.cpp:1493-1500).
+static void appendSerialPhase(race_handle h, Schedule* s) {
+    if (h->nGraph >= h->nrow) return;
+    s->leafLo.push_back(h->nGraph);
+    s->leafHi.push_back(h->nrow);
+    s->phasePtr.push_back((int)s->leafLo.size());
+}
+
+static void setPermAndSchedules(race_handle h) {
+    const std::vector<int>& tp = h->trav->perm();
+    if (h->graphRows.empty()) {
+        h->perm = tp;
+        h->inv = h->trav->invPerm();
+    } else {
+        h->perm.resize(h->nrow);
+        for (int i = 0; i < h->nGraph; ++i) h->perm[i] = h->graphRows[tp[i]];
+        for (size_t d = 0; d < h->denseRows.size(); ++d) h->perm[h->nGraph + d] = h->denseRows[d];
+        h->inv.resize(h->nrow);
+        for (int i = 0; i < h->nrow; ++i) h->inv[h->perm[i]] = i;
+    }
+    h->fwd = flattenSchedule(*h->tree, false, h->nGraph);
+    h->rev = flattenSchedule(*h->tree, true, h->nGraph);
+    appendSerialPhase(h, &h->fwd);
+    appendSerialPhase(h, &h->rev);
+}
+
 static int ensureTraverser(race_handle h) {
     if (h->trav) return OK;
     h->trav = std::make_unique<GpuTraverser>();
     h->trav->rangeBlockMax = h->optBfsBlockMax;
-    return h->trav->init(h->nrow, h->rowPtr.data(), h->col.data(),
+    return h->trav->init(h->nGraph, h->rowPtr.data(), h->col.data(),
                          h->initPerm.empty() ? nullptr : h->initPerm.data(),
                          h->initInv.empty() ? nullptr : h->initInv.data(), h->stream);
 }
@@ -286,14 +362,11 @@ int race_color(race_handle h) {
     std::vector<double> eff(effI.begin(), effI.end());
     std::vector<int> lvlThreads = envIntList("RACE_THREADS");
     h->tree = std::make_unique<ZoneTree>(h->dist, h->d2, h->lbTarget);
-    LevelRecursion lr(h->trav.get(), h->nrow, h->requested, h->dist, h->d2, h->lbTarget, eff, lvlThreads);
+    LevelRecursion lr(h->trav.get(), h->nGraph, h->requested, h->dist, h->d2, h->lbTarget, eff, lvlThreads);
     e = lr.levelBalancing(h->tree.get());
     if (e) return e;
     h->availableThreads = lr.availableThreads;
-    h->perm = h->trav->perm();
-    h->inv = h->trav->invPerm();
-    h->fwd = flattenSchedule(*h->tree, false, h->nrow);
-    h->rev = flattenSchedule(*h->tree, true, h->nrow);
+    setPermAndSchedules(h);
     h->colored = true;
     return OK;
 }
@@ -569,8 +642,9 @@ static void runHostColor(race_handle h, HostFn& f, bool rev) {
                 f.color(s.leafLo[l], s.leafHi[l], f.args);
         }
     }
-    // serial part (empty unless dense rows were split off)
-    f.color(h->nrow, h->nrow, f.args);
+    // the serial part is the schedule's last phase when dense rows were
+    // split off; otherwise the reference's empty call f(n, n)
+    if (h->nGraph >= h->nrow) f.color(h->nrow, h->nrow, f.args);
 }
 
 // Host power callbacks over the level groups in a slope-2 wavefront: at step
@@ -887,6 +961,7 @@ int race_get_tree(race_handle h, int* nleaves, int* data6, int cap) {
 
 int race_schedule_save(race_handle h, void* buf, size_t cap, size_t* size) {
     if (checkHandle(h) || !size || !h->colored) return ERR_INVALID_ARG;
+    if (h->nGraph < h->nrow) return ERR_NOT_IMPLEMENTED;  // a dense-row tail is not stored
     SavedSchedule s;
     s.nrow = h->nrow;
     s.dist = h->dist;
@@ -929,8 +1004,10 @@ int race_schedule_load(race_handle h, const void* buf, size_t size) {
     if (s.hasTree) {
         h->tree = std::make_unique<ZoneTree>(h->dist, h->d2, h->lbTarget);
         h->tree->tree = std::move(s.tree);
-        h->fwd = flattenSchedule(*h->tree, false, h->nrow);
-        h->rev = flattenSchedule(*h->tree, true, h->nrow);
+        h->fwd = flattenSchedule(*h->tree, false, h->nGraph);
+        h->rev = flattenSchedule(*h->tree, true, h->nGraph);
+        appendSerialPhase(h, &h->fwd);
+        appendSerialPhase(h, &h->rev);
     } else {
         h->tree.reset();
         h->mpk = std::move(s.mpk);

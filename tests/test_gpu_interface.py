@@ -317,3 +317,65 @@ def test_fp32_kaczmarz_fp64_norms_error_vs_fp64_oracle(oracle):
             worst = max(worst, e)
         print(name, n, "worst rhs error vs FP64 oracle", worst, flush=True)
     assert worst <= 2e-5
+
+
+def _grid_hubs(g, hubs):
+    n0 = g * g
+    rows = [[] for _ in range(n0 + hubs)]
+    for i in range(g):
+        for j in range(g):
+            r = i * g + j
+            for di, dj in ((0, 0), (-1, 0), (1, 0), (0, -1), (0, 1)):
+                a, b = i + di, j + dj
+                if 0 <= a < g and 0 <= b < g:
+                    rows[r].append(a * g + b)
+    for h in range(hubs):
+        rows[n0 + h].append(n0 + h)
+        for r in range(0, n0, 3):  # every third grid row
+            rows[r].append(n0 + h)
+            rows[n0 + h].append(r)
+    rp = np.concatenate([[0], np.cumsum([len(x) for x in rows])]).astype(np.int32)
+    col = np.array([c for x in rows for c in sorted(x)], np.int32)
+    return rp, col
+
+
+@pytest.mark.parametrize("dist,kind,T", [(R.ONE, R.K_GS, 32), (R.TWO, R.K_KACZ, 16)])
+def test_dense_rows_run_as_a_serial_tail(oracle, monkeypatch, dist, kind, T):
+    """RACE_MAX_DENSE: the hub rows leave the colouring and run last, in
+    order, in both sweep directions.  Permutation bit for bit with the
+    oracle; device sweeps and a host callback against the oracle's leaf
+    calls plus the serial tail."""
+    monkeypatch.setenv("RACE_MAX_DENSE", "4")
+    rp, col = _grid_hubs(40, 3)
+    n = len(rp) - 1
+    val = M.gs_values(rp, col, seed=6)
+    ce = R.Interface(n, T, dist, rp, col, symm_hint=True)
+    ce.RACEColor()
+    G = oracle.Graph(rp, col, symm_hint=True, max_dense=4)
+    assert G.dense_rows == [1600, 1601, 1602]
+    zt = oracle.level_recursion(G, T, dist)
+    perm = ce.getPerm()
+    assert np.array_equal(perm, G.full_perm())
+    ns = G.n
+    prp, pcol, pval = M.make_diag_first(*M.permute(rp, col, val, perm, ce.getInvPerm()))
+    b = M.uniform_vec(n, 4)
+    x0 = M.uniform_vec(n, 5)
+    dx, db = R.DeviceArray.from_host(x0), R.DeviceArray.from_host(b)
+    fid = ce.registerDevice(kind, prp, pcol, pval, dx, db)
+    okind = oracle.K_GS if kind == R.K_GS else oracle.K_KACZ
+    xo = x0.copy()
+    xh = x0.copy()
+
+    def host(start, end, x):
+        oracle.run_calls(okind, prp, pcol, pval, x, b, [(start, end)])
+
+    hid = ce.registerFunction(host, xh)
+    ce.setSerial(hid)
+    for rev in (False, True):
+        ce.executeFunction(fid, rev)
+        ce.executeFunction(hid, rev)
+        oracle.run_calls(okind, prp, pcol, pval, xo, b, oracle.leaf_calls(zt, rev, serial_part=(ns, n)))
+        assert relerr(dx.download(), xo) <= TOL
+        assert relerr(xh, xo) <= TOL
+    with pytest.raises(R.RaceError):
+        ce.saveSchedule()

@@ -90,6 +90,12 @@ def test_entry_points_reject_bad_arguments_without_a_device():
     assert lib.race_create(ctypes.byref(h), 2, 0, 1, ip(rp), ip(col), 1, 1, 1, None, None, 0, 1) == INVALID
     assert lib.race_create(ctypes.byref(h), 2, 1, 7, ip(rp), ip(col), 1, 1, 1, None, None, 0, 1) == INVALID
     assert lib.race_create(ctypes.byref(h), 2, 1, 1, ip(rp), ip(col), 1, 1, 1, ip(rp), None, 0, 1) == INVALID
+    # RACE_MAX_DENSE has no serial part for POWER (checked before any device call)
+    os.environ["RACE_MAX_DENSE"] = "4"
+    try:
+        assert lib.race_create(ctypes.byref(h), 2, 1, 3, ip(rp), ip(col), 1, 1, 1, None, None, 0, 1) == 10
+    finally:
+        del os.environ["RACE_MAX_DENSE"]
     # batches and the tuner's model
     assert lib.race_batch_create(ctypes.byref(h), 99, 0, 0, 1) == INVALID
     assert lib.race_batch_create(ctypes.byref(h), 2, 0, 0, 0) == INVALID

@@ -585,3 +585,58 @@ def test_traverse_boundary_path_by_hand(oracle):
     assert list(regs[0][0]) == [0, 1] + [0] * 15
     assert list(regs[1][0]) == [0] * 16 + [1]
     assert np.array_equal(G.perm, np.arange(n))
+
+
+def _grid_with_hubs(g, hubs):
+    """a g x g 5-point grid plus `hubs` rows joined to every grid row"""
+    n0 = g * g
+    rows = [[] for _ in range(n0 + hubs)]
+    for i in range(g):
+        for j in range(g):
+            r = i * g + j
+            for di, dj in ((0, 0), (-1, 0), (1, 0), (0, -1), (0, 1)):
+                a, b = i + di, j + dj
+                if 0 <= a < g and 0 <= b < g:
+                    rows[r].append(a * g + b)
+    for h in range(hubs):
+        rows[n0 + h].append(n0 + h)
+        for r in range(n0):
+            rows[r].append(n0 + h)
+            rows[n0 + h].append(r)
+    rp = np.concatenate([[0], np.cumsum([len(x) for x in rows])]).astype(np.int32)
+    col = np.array([c for x in rows for c in sorted(x)], np.int32)
+    return rp, col
+
+
+@pytest.mark.parametrize("reverse_init", [False, True])
+def test_dense_rows_move_to_a_serial_tail(oracle, reverse_init):
+    """RACE_MAX_DENSE (Graph::getStatistics / permuteAndRemoveSerialPart,
+    src/graph_SoA.cpp:321-435): a 10 x 10 grid with two hub rows joined to
+    all 100 grid rows has 862 entries over 102 rows, mean 862 // 102 = 8;
+    with RACE_MAX_DENSE = 4 the hubs (101 entries > 32) leave the graph, the
+    grid rows keep at most 7 entries.  The remaining graph is the plain grid:
+    its one-thread BFS order is (i + j, index); the hubs follow in the order
+    they have under the initial permutation."""
+    rp, col = _grid_with_hubs(10, 2)
+    n = len(rp) - 1
+    assert int(rp[-1]) == 862
+    ip = np.arange(n, dtype=np.int32)[::-1].copy() if reverse_init else None
+    G = oracle.Graph(rp, col, symm_hint=True, init_perm=ip, init_inv_perm=ip, max_dense=4)
+    assert G.n == 100 and G.nfull == 102
+    assert G.dense_rows == ([101, 100] if reverse_init else [100, 101])
+    zt = oracle.level_recursion(G, 1, oracle.ONE)
+    grid = np.arange(100)
+    if reverse_init:
+        grid = grid[::-1]  # graph row q is original grid row 99 - q
+    q = np.arange(100)
+    i, j = (grid[q] // 10), (grid[q] % 10)
+    if reverse_init:
+        # BFS from graph row 0 = original (9, 9): level (9 - i) + (9 - j)
+        lvl = (9 - i) + (9 - j)
+    else:
+        lvl = i + j
+    expect = grid[np.lexsort((q, lvl))]
+    assert np.array_equal(G.full_perm(), np.concatenate([expect, G.dense_rows]))
+    calls = oracle.leaf_calls(zt, serial_part=(100, 102))
+    assert calls[-1] == (100, 102)
+    assert oracle.leaf_calls(zt, rev=True, serial_part=(100, 102))[-1] == (101, 99)
