@@ -51,7 +51,10 @@ typedef struct race_handle_s *race_handle;
  * longer than that many times nnz/nrow (integer division) leave the graph
  * and run last, one after the other, in every sweep (a last phase of one
  * leaf, [n', n) of the permutation); colouring only (RACE_ERR_NOT_IMPLEMENTED
- * for POWER, whose executor has no serial part). */
+ * for POWER, whose executor has no serial part).  Device kernels that write
+ * the columns of their rows (Kaczmarz, colour-phase SymmSpMV) are then
+ * refused with RACE_ERR_INCOMPATIBILITY: concurrent rows may share a dense
+ * column. */
 int race_create(race_handle *out, int nrow, int nthreads, int dist, const int *rowPtr,
                 const int *col, int symm_hint, int smt, int pin_method, const int *init_perm,
                 const int *init_inv_perm, int d2_method, int lb_target);

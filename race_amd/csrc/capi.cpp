@@ -514,6 +514,14 @@ int race_register_device(race_handle h, const race_kernel_desc* d, int* id) {
     // src/interface.cpp:449-452): the wavefront is correct for any power,
     // only the cache blocking was sized for highestPower
     if (power && d->power < 1) return ERR_INVALID_ARG;
+    // A kernel that writes the columns of its rows (Kaczmarz, colour-phase
+    // SymmSpMV) relies on distance-2 independence of concurrent rows; rows
+    // removed as dense are no graph vertices, so two concurrent rows may
+    // share a dense column and write it at once (the reference's OpenMP
+    // execution races there the same way).  Refused instead.
+    if (h->nGraph < h->nrow &&
+        (d->kind == RACE_K_KACZ || (d->kind == RACE_K_SYMMSPMV && h->optSymmMode == 1)))
+        return ERR_INCOMPATIBILITY;
     ExecContext ctx;
     ctx.fwd = &h->fwd;
     ctx.rev = &h->rev;

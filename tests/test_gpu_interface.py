@@ -339,7 +339,7 @@ def _grid_hubs(g, hubs):
     return rp, col
 
 
-@pytest.mark.parametrize("dist,kind,T", [(R.ONE, R.K_GS, 32), (R.TWO, R.K_KACZ, 16)])
+@pytest.mark.parametrize("dist,kind,T", [(R.ONE, R.K_GS, 32), (R.ONE, R.K_GS, 4)])
 def test_dense_rows_run_as_a_serial_tail(oracle, monkeypatch, dist, kind, T):
     """RACE_MAX_DENSE: the hub rows leave the colouring and run last, in
     order, in both sweep directions.  Permutation bit for bit with the
@@ -379,3 +379,23 @@ def test_dense_rows_run_as_a_serial_tail(oracle, monkeypatch, dist, kind, T):
         assert relerr(xh, xo) <= TOL
     with pytest.raises(R.RaceError):
         ce.saveSchedule()
+
+
+def test_dense_rows_refuse_column_writing_kernels(oracle, monkeypatch):
+    """Kaczmarz writes every column of its row: two concurrent rows of a
+    distance-2 phase may share a dense column once dense rows are no graph
+    vertices, so the device function is refused (the colouring itself and
+    the permutation still follow the reference)."""
+    monkeypatch.setenv("RACE_MAX_DENSE", "4")
+    rp, col = _grid_hubs(20, 2)
+    n = len(rp) - 1
+    val = M.gs_values(rp, col, seed=2)
+    ce = R.Interface(n, 8, R.TWO, rp, col, symm_hint=True)
+    ce.RACEColor()
+    G = oracle.Graph(rp, col, symm_hint=True, max_dense=4)
+    oracle.level_recursion(G, 8, oracle.TWO)
+    assert np.array_equal(ce.getPerm(), G.full_perm())
+    prp, pcol, pval = M.make_diag_first(*M.permute(rp, col, val, ce.getPerm(), ce.getInvPerm()))
+    dx, db = R.DeviceArray(n), R.DeviceArray(n)
+    with pytest.raises(R.RaceError):
+        ce.registerDevice(R.K_KACZ, prp, pcol, pval, dx, db)
