@@ -449,6 +449,8 @@ def main():
                     help="skip the SymmSpMV 64^3 and MPK 256^3 measurements")
     ap.add_argument("--graph", type=int, choices=[0, 1], default=0,
                     help="replay each sweep's phase launches as a HIP graph")
+    ap.add_argument("--flow", type=int, choices=[0, 1], default=0,
+                    help="one launch per sweep with per-chain dependency flags instead of one per colour phase")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="run a plain copy kernel of this many MiB first (byte-counter reference)")
     args = ap.parse_args()
@@ -496,6 +498,7 @@ def main():
     ce.RACEColor()
     t_color = time.time() - t0
     ce.setOption("graph", args.graph)
+    ce.setOption("flow", args.flow)
     bfs_ms, ntrav = ce.preprocessStats()
     perm, inv = ce.getPerm(), ce.getInvPerm()
     log(f"RACEColor T={args.threads}: {t_color:.2f}s (GPU BFS {bfs_ms:.0f} ms in {ntrav} traversals), "

@@ -173,6 +173,12 @@ int race_register_device(race_handle h, const race_kernel_desc *desc, int *id);
  *   "bfs_block_max" ranges of at most this many rows are traversed one
  *                  workgroup each, batched (default 131072; 0 = every range
  *                  level by level over the whole device); set before colouring
+ *   "flow"         Gauss-Seidel with one right-hand side: 1 = one launch per
+ *                  sweep; the chains of all colour phases are handed out in
+ *                  phase order and each waits only for the earlier chains it
+ *                  conflicts with (per-chain completion flags) instead of a
+ *                  launch boundary per phase (default 0); set before
+ *                  registering; results are identical
  * No counterpart in the reference (its executor has no layout choice). */
 int race_set_option(race_handle h, const char *key, int value);
 /* re-point a registered device function at other vectors */
@@ -306,6 +312,11 @@ int race_get_wave_count(race_handle h, int id, int rev, int *waves);
 /* kernel launches one execution of a registered device function issues
  * (colour phases, MPK wavefront steps) */
 int race_get_launch_count(race_handle h, int id, int rev, int *launches);
+/* one-launch ("flow") sweeps of a device function whose dependency waits gave
+ * up after their poll budget (always 0 for a consistent schedule; a nonzero
+ * count means those sweeps' results are invalid).  Synchronises with the
+ * device.  No counterpart in the reference. */
+int race_get_flow_aborts(race_handle h, int id, int *aborts);
 /* HIP-event timer on the handle's stream (brackets work launched there) */
 int race_timer_begin(race_handle h);
 int race_timer_end(race_handle h, float *ms);

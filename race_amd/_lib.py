@@ -104,6 +104,7 @@ PROTOTYPES = {
     "race_get_tree": (ctypes.c_int, [vp, i32p, i32p, ctypes.c_int]),
     "race_get_mpk_groups": (ctypes.c_int, [vp, i32p, i32p, ctypes.c_int]),
     "race_get_launch_count": (ctypes.c_int, [vp, ctypes.c_int, ctypes.c_int, i32p]),
+    "race_get_flow_aborts": (ctypes.c_int, [vp, ctypes.c_int, i32p]),
     "race_rcm": (ctypes.c_int, [ctypes.c_int, i32p, i32p, i32p, i32p]),
     "race_schedule_save": (ctypes.c_int, [vp, vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
     "race_schedule_load": (ctypes.c_int, [vp, vp, ctypes.c_size_t]),

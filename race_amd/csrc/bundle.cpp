@@ -223,6 +223,7 @@ BundleTable buildBundles(const Schedule& s, int kind, int n, const int* rp, cons
         t.phRows.back() = std::max(t.phRows.back(), nb);
         t.phLocal.back() = std::max(t.phLocal.back(), local);
         t.phWaves.back() = std::max(t.phWaves.back(), nw);
+        t.phExt.back() = std::max(t.phExt.back(), (int)extCols.size());
         cur.clear();
         curRows = 0;
         curLocal = 0;
@@ -247,6 +248,7 @@ BundleTable buildBundles(const Schedule& s, int kind, int n, const int* rp, cons
         t.phRows.push_back(0);
         t.phLocal.push_back(0);
         t.phWaves.push_back(0);
+        t.phExt.push_back(0);
     };
     auto flushMark = [&]() {
         const bool had = !cur.empty();
@@ -357,11 +359,12 @@ BundleTable concatBundles(const std::vector<const BundleTable*>& tabs, std::vect
     for (int p = 0; p < P; ++p) {
         t.phaseBundle.push_back((int)t.slot.size() - 1);
         t.phaseChain.push_back((int)t.chainPtr.size());
-        int rows = 0, local = 0, waves = 0;
+        int rows = 0, local = 0, waves = 0, extMax = 0;
         for (size_t m = 0; m < tabs.size(); ++m) {
             const BundleTable& a = *tabs[m];
             if (p >= a.numPhases()) continue;
             rows = std::max(rows, a.phRows[p]);
+            extMax = std::max(extMax, a.phExt[p]);
             local = std::max(local, a.phLocal[p]);
             waves = std::max(waves, a.phWaves[p]);
             // chains keep their bundles, renumbered
@@ -398,6 +401,7 @@ BundleTable concatBundles(const std::vector<const BundleTable*>& tabs, std::vect
         t.phRows.push_back(rows);
         t.phLocal.push_back(local);
         t.phWaves.push_back(waves);
+        t.phExt.push_back(extMax);
         t.critical += waves;
     }
     t.phaseChain.push_back((int)t.chainPtr.size());
@@ -405,6 +409,80 @@ BundleTable concatBundles(const std::vector<const BundleTable*>& tabs, std::vect
     t.phaseBundle.push_back((int)t.slot.size() - 1);
     t.waveSlot.push_back(t.slot.back());
     return t;
+}
+
+}  // namespace race
+
+namespace race {
+
+long chainDeps(const BundleTable& t, int n, std::vector<int>* depPtr, std::vector<int>* dep) {
+    const int nchain = (int)t.chainPtr.size() - 1;
+    std::vector<int> phaseOf(std::max(nchain, 0), 0);
+    for (int p = 0; p < t.numPhases(); ++p)
+        for (int c = t.phaseChain[p]; c < t.phaseChain[p + 1]; ++c) phaseOf[c] = p;
+    // (waiting chain, chain it waits for)
+    std::vector<std::pair<int, int>> edge;
+    long same = 0;
+    auto add = [&](int waiter, int first) {
+        if (phaseOf[waiter] == phaseOf[first]) ++same;
+        edge.push_back({waiter, first});
+    };
+    if (t.kind != RACE_K_KACZ) {
+        // owner[r]: chain that writes row r
+        std::vector<int> owner(n, -1);
+        for (int c = 0; c < nchain; ++c)
+            for (int b = t.chainPtr[c]; b < t.chainPtr[c + 1]; ++b)
+                for (int s = t.slot[b]; s < t.slot[b + 1]; ++s) owner[t.rowId[s]] = c;
+        for (int c = 0; c < nchain; ++c)
+            for (int b = t.chainPtr[c]; b < t.chainPtr[c + 1]; ++b)
+                for (int e = t.extPtr[b]; e < t.extPtr[b + 1]; ++e) {
+                    const int o = owner[t.ext[e]];
+                    if (o < 0 || o == c) continue;  // not written in the sweep, or by this chain
+                    if (o < c)
+                        add(c, o);  // c reads what o wrote
+                    else
+                        add(o, c);  // o overwrites what c read
+                }
+    } else {
+        // every column a chain touches is read and written: each chain waits
+        // for the previous toucher of each of its columns
+        std::vector<int> last(n, -1);
+        for (int c = 0; c < nchain; ++c) {
+            for (int b = t.chainPtr[c]; b < t.chainPtr[c + 1]; ++b)
+                for (int e = t.extPtr[b]; e < t.extPtr[b + 1]; ++e) {
+                    const int o = last[t.ext[e]];
+                    if (o >= 0 && o != c) add(c, o);
+                }
+            for (int b = t.chainPtr[c]; b < t.chainPtr[c + 1]; ++b)
+                for (int e = t.extPtr[b]; e < t.extPtr[b + 1]; ++e) last[t.ext[e]] = c;
+        }
+    }
+    std::sort(edge.begin(), edge.end());
+    edge.erase(std::unique(edge.begin(), edge.end()), edge.end());
+    depPtr->assign(nchain + 1, 0);
+    dep->resize(edge.size());
+    for (size_t k = 0; k < edge.size(); ++k) {
+        ++(*depPtr)[edge[k].first + 1];
+        (*dep)[k] = edge[k].second;
+    }
+    for (int c = 0; c < nchain; ++c) (*depPtr)[c + 1] += (*depPtr)[c];
+    return same;
+}
+
+std::vector<int> flowExt(const BundleTable& t, int n) {
+    const int nchain = (int)t.chainPtr.size() - 1;
+    std::vector<int> owner(n, -1);
+    for (int c = 0; c < nchain; ++c)
+        for (int b = t.chainPtr[c]; b < t.chainPtr[c + 1]; ++b)
+            for (int s = t.slot[b]; s < t.slot[b + 1]; ++s) owner[t.rowId[s]] = c;
+    std::vector<int> out(t.ext);
+    for (int c = 0; c < nchain; ++c)
+        for (int b = t.chainPtr[c]; b < t.chainPtr[c + 1]; ++b)
+            for (int e = t.extPtr[b]; e < t.extPtr[b + 1]; ++e) {
+                const int o = owner[t.ext[e]];
+                if (o >= 0 && o <= c) out[e] = (int)((unsigned)t.ext[e] | 0x80000000u);
+            }
+    return out;
 }
 
 }  // namespace race

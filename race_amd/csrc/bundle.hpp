@@ -55,7 +55,7 @@ struct BundleTable {
     long paddedEntries = 0;
     bool ok = true;                 // false if a leaf exceeds the bundle limits
     // per-phase maxima (LDS sizing of each launch)
-    std::vector<int> phRows, phLocal, phWaves;
+    std::vector<int> phRows, phLocal, phWaves, phExt;  // phExt: most outside columns of a bundle
     int numPhases() const { return (int)phaseBundle.size() - 1; }
 };
 
@@ -76,5 +76,23 @@ BundleTable buildBundles(const Schedule& s, int kind, int n, const int* rowPtr, 
 // ids and column ids stay those of their own matrix (each bundle of a batch
 // carries its own vectors).  bundleTable[k] = table of fused bundle k.
 BundleTable concatBundles(const std::vector<const BundleTable*>& tabs, std::vector<int>* bundleTable);
+
+// Dependencies between the chains of one sweep, for a launch that runs all
+// of them in chain order (phase order) with a completion flag per chain
+// instead of one launch per phase.  Chain c waits for the chains
+// dep[depPtr[c] .. depPtr[c+1]), all earlier than c:
+//   GS  : an earlier chain that writes a row c reads (an outside column of
+//         c), or that reads a row c writes;
+//   KACZ: for each column c touches, the previous chain touching it.
+// Returns the number of such pairs inside one phase (0 for a valid
+// colouring: the leaves of a phase are independent).
+long chainDeps(const BundleTable& t, int n, std::vector<int>* depPtr, std::vector<int>* dep);
+
+// Outside columns of a Gauss-Seidel table for the one-launch sweep: t.ext
+// with bit 31 set on every column written earlier in the launch (by an
+// earlier chain, or by an earlier bundle of the same chain).  Only those must
+// be read coherently after the dependency wait; the others still hold the
+// value the launch started with and are read with the matrix.
+std::vector<int> flowExt(const BundleTable& t, int n);
 
 }  // namespace race

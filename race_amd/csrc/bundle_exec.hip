@@ -89,6 +89,9 @@ __device__ __forceinline__ int bundleOfBlock(int nbnd, int xcd) {
     return q < nbnd ? q : -1;
 }
 
+// the lane of a row that reads and writes its x (the first of its lanes)
+__device__ __forceinline__ bool own(bool active, int lane) { return active && lane == 0; }
+
 // Vectors of a bundle: the launch's x, b and n, or, for a batch of matrices
 // (BATCH), the bundle's own (each matrix keeps its vectors).
 template <bool BATCH, typename XT>
@@ -117,7 +120,7 @@ struct BundleVec {
 // a zero high half, not a sign extension that has to wait for the loaded
 // value.  A phase whose chains are single bundles (bundle0 >= 0) skips the
 // chain table.
-template <int BS, int TPR, int E, typename VT, typename XT, bool ONE_RHS, bool BATCH>
+template <int BS, int TPR, int E, typename VT, typename XT, bool ONE_RHS, bool BATCH, int NX = 2>
 __global__ void __launch_bounds__(BS) k_gs_bundle(
     const int* __restrict__ bSlot, const int* __restrict__ bWave, const long* __restrict__ bOff,
     const int* __restrict__ bLen, const int* __restrict__ bExt, const uint16_t* __restrict__ slotWave,
@@ -160,8 +163,11 @@ __global__ void __launch_bounds__(BS) k_gs_bundle(
         // go out first.
         const int sr = active ? s0 + i : 0;
         const unsigned row = (unsigned)rowId[sr];
-        const unsigned ex0 = (unsigned)ext[t < ne ? e0 + t : 0];
-        const unsigned ex1 = (unsigned)ext[t + BS < ne ? e0 + t + BS : 0];
+        // ids of the first NX*BS outside columns, in one batch (columns
+        // beyond are gathered round by round)
+        unsigned ex[NX];
+#pragma unroll
+        for (int k = 0; k < NX; ++k) ex[k] = (unsigned)ext[t + k * BS < ne ? e0 + t + k * BS : 0];
         int li[E];
         typename ValBits<VT>::T lv[E];
 #pragma unroll
@@ -180,14 +186,16 @@ __global__ void __launch_bounds__(BS) k_gs_bundle(
         for (int k = k0; k < k1; ++k) {
             const size_t vo = (size_t)k * n;
             // every gather in flight before the first LDS write
-            const XT g0 = x[vo + (t < ne ? ex0 : 0u)];
-            const XT g1 = x[vo + (t + BS < ne ? ex1 : 0u)];
+            XT g[NX];
+#pragma unroll
+            for (int q = 0; q < NX; ++q) g[q] = x[vo + (t + q * BS < ne ? ex[q] : 0u)];
             const XT gr = x[vo + (own ? row : 0u)];
             const XT br = b[vo + (own ? row : 0u)];
-            if (t < ne) xl[nb + t] = g0;
-            if (t + BS < ne) xl[nb + t + BS] = g1;
+#pragma unroll
+            for (int q = 0; q < NX; ++q)
+                if (t + q * BS < ne) xl[nb + t + q * BS] = g[q];
             if (own) xl[i] = gr;
-            for (int c = t + 2 * BS; c < ne; c += BS) xl[nb + c] = x[vo + ext[e0 + c]];
+            for (int c = t + NX * BS; c < ne; c += BS) xl[nb + c] = x[vo + ext[e0 + c]];
             __syncthreads();
             for (int w = 0; w < nw; ++w) {
                 if (active && myWave == w) {
@@ -216,6 +224,183 @@ __global__ void __launch_bounds__(BS) k_gs_bundle(
             runBundle(bnd);
             __syncthreads();
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// One launch per sweep ("flow"): the chains of every phase are handed out in
+// chain order (phase order) from a ticket counter, and a chain waits only for
+// the earlier chains it conflicts with (chainDeps), through a completion flag
+// per chain, instead of for the whole previous phase at a launch boundary.
+// Tickets go out in an order in which every dependency comes first, and a
+// workgroup takes a ticket only when it can run it, so the chain with the
+// smallest unfinished number always has its dependencies done: the launch
+// cannot deadlock, whatever the number of resident workgroups.
+//
+// Values of x that another workgroup may have written during the launch are
+// read and written with agent-scope relaxed atomics (each XCD has its own
+// L2; these accesses are coherent across them).  A chain publishes its flag
+// after every wave of the workgroup has waited for its stores of x; a waiter
+// reads the flag, then issues its loads of x.
+struct FlowState {
+    int ticket;  // next chain to hand out
+    int done;    // workgroups finished in this launch
+    int epoch;   // launch number: a chain's flag holds the epoch it finished in
+    int abort;   // a wait exceeded its poll budget in this launch
+    int aborts;  // launches that aborted (kept for the host)
+    int pad[3];
+};
+
+__device__ __forceinline__ int ldAgent(const int* p) {
+    return __hip_atomic_load(const_cast<int*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void stAgent(int* p, int v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ T ldCoh(const T* p) {
+    return __hip_atomic_load(const_cast<T*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ void stCoh(T* p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Wait until chain a has finished in this launch.  After pollLimit polls
+// (a broken dependency table), or once another wait has given up, the
+// launch is marked aborted and the wait returns, so every workgroup ends.
+__device__ __forceinline__ void waitChain(const int* flag, int a, int epoch, FlowState* fs, int pollLimit) {
+    int polls = 0;
+    while (ldAgent(&flag[a]) != epoch) {
+        if ((++polls & 255) == 0 && (polls >= pollLimit || ldAgent(&fs->abort))) {
+            stAgent(&fs->abort, 1);
+            return;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+// Gauss-Seidel, one right-hand side, all phases of a sweep in one launch.
+// The bundle body is k_gs_bundle's, with the dependency wait between the
+// prologue's matrix loads and its gathers of x.
+// NX register slots per lane for outside-column ids and values: NX*BS
+// columns are gathered in one batch (bundles of a 27-point stencil at small
+// leaves have about 900-2500 outside columns).
+template <int BS, int TPR, int E, typename VT, typename XT, int NX = 6>
+__global__ void __launch_bounds__(BS) k_gs_flow(
+    const int* __restrict__ bSlot, const int* __restrict__ bWave, const long* __restrict__ bOff,
+    const int* __restrict__ bLen, const int* __restrict__ bExt, const uint16_t* __restrict__ slotWave,
+    const int* __restrict__ rowId, const XT* __restrict__ rdiag, const int* __restrict__ ext,
+    const uint16_t* __restrict__ sidx, const VT* __restrict__ sval, long zero, XT* x, const XT* __restrict__ b,
+    const int* __restrict__ chainPtr, int nchain, const int* __restrict__ depPtr, const int* __restrict__ dep,
+    int* flag, FlowState* fs, int pollLimit) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    XT* xl = reinterpret_cast<XT*>(smem);
+    __shared__ int sTicket;
+    const int t = threadIdx.x, i = t / TPR, l = t % TPR;
+    const int epoch = ldAgent(&fs->epoch);
+    if (t == 0) sTicket = atomicAdd(&fs->ticket, 1);
+    __syncthreads();
+    int cur = sTicket;
+    while (cur < nchain) {
+        int next = 0;
+        const int cLo = chainPtr[cur], cHi = chainPtr[cur + 1];
+        for (int bnd = cLo; bnd < cHi; ++bnd) {
+            const bool first = bnd == cLo;
+            const int d0 = first ? depPtr[cur] : 0, nd = first ? depPtr[cur + 1] - d0 : 0;
+            // branch-free: the dependency ids load with the prologue (every
+            // device array holds at least one element)
+            const int dj = dep[t < nd ? d0 + t : 0];
+            const int s0 = bSlot[bnd], nb = bSlot[bnd + 1] - s0;
+            const long off = bOff[bnd];
+            const int wb = bLen[bnd];
+            const int nw = bWave[bnd + 1] - bWave[bnd];
+            const int e0 = bExt[bnd], ne = bExt[bnd + 1] - e0;
+            const bool active = i < nb;
+            const long lanes = (long)nb * TPR;
+            // branch-free prologue (as in k_gs_bundle)
+            const int sr = active ? s0 + i : 0;
+            const unsigned row = (unsigned)rowId[sr];
+            // ids of the first NX*BS outside columns, in one batch
+            unsigned ex[NX];
+#pragma unroll
+            for (int k = 0; k < NX; ++k) ex[k] = (unsigned)ext[t + k * BS < ne ? e0 + t + k * BS : 0];
+            int li[E];
+            typename ValBits<VT>::T lv[E];
+#pragma unroll
+            for (int u = 0; u < E; ++u) {
+                const long e = (active && u < wb) ? off + (long)u * lanes + t : zero;
+                li[u] = (int)sidx[e];
+                lv[u] = sval[e];
+            }
+            const XT rd = rdiag[sr];
+            const int myWave = (int)slotWave[sr];
+            const XT br = b[own(active, l) ? row : 0u];
+            // Columns no earlier chain of the launch writes (bit 31 clear)
+            // still hold the launch's starting value: they are gathered now,
+            // with the matrix, through the caches.  So is the row's own x
+            // (only this chain writes it).
+            XT g[NX];
+#pragma unroll
+            for (int k = 0; k < NX; ++k) g[k] = x[t + k * BS < ne && !(ex[k] >> 31) ? ex[k] : 0u];
+            const XT gr = x[own(active, l) ? row : 0u];
+            // the next ticket is requested here, behind the prologue's loads
+            // (the wave waits for it where it waits for them), and read after
+            // this chain
+            if (first && t == 0) next = atomicAdd(&fs->ticket, 1);
+            // the chains this one conflicts with, then the columns they wrote
+            if (t < nd) waitChain(flag, dj, epoch, fs, pollLimit);
+            for (int j = t + BS; j < nd; j += BS) waitChain(flag, dep[d0 + j], epoch, fs, pollLimit);
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < NX; ++k)
+                if (t + k * BS < ne && (ex[k] >> 31)) g[k] = ldCoh(&x[ex[k] & 0x7fffffffu]);
+#pragma unroll
+            for (int k = 0; k < NX; ++k)
+                if (t + k * BS < ne) xl[nb + t + k * BS] = g[k];
+            if (own(active, l)) xl[i] = gr;
+            for (int c = t + NX * BS; c < ne; c += BS) {
+                const unsigned ec = (unsigned)ext[e0 + c];
+                xl[nb + c] = (ec >> 31) ? ldCoh(&x[ec & 0x7fffffffu]) : x[ec];
+            }
+            __syncthreads();
+            for (int w = 0; w < nw; ++w) {
+                if (active && myWave == w) {
+                    XT sa = XT(0), sb = XT(0);
+#pragma unroll
+                    for (int u = 0; u + 1 < E; u += 2) {
+                        sa += valOf<XT>(lv[u]) * xl[li[u]];
+                        sb += valOf<XT>(lv[u + 1]) * xl[li[u + 1]];
+                    }
+                    if (E % 2) sa += valOf<XT>(lv[E - 1]) * xl[li[E - 1]];
+                    const XT sm = laneSum<TPR>(sa + sb);
+                    if (l == 0) xl[i] = (br - sm) * rd;
+                }
+                __syncthreads();
+            }
+            if (own(active, l)) stCoh(&x[row], xl[i]);
+            __syncthreads();
+        }
+        // every wave's stores of x are complete before the flag goes out
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (t == 0) {
+            stAgent(&flag[cur], epoch);
+            sTicket = next;
+        }
+        __syncthreads();
+        cur = sTicket;
+    }
+    // the last workgroup to finish prepares the next launch
+    if (t == 0 && atomicAdd(&fs->done, 1) == (int)gridDim.x - 1) {
+        if (ldAgent(&fs->abort)) {
+            atomicAdd(&fs->aborts, 1);
+            stAgent(&fs->abort, 0);
+        }
+        stAgent(&fs->ticket, 0);
+        stAgent(&fs->done, 0);
+        stAgent(&fs->epoch, epoch + 1);
     }
 }
 
@@ -633,6 +818,12 @@ void launchBundles(int kind, int nrhs, const BundleDev<VT, XT>& d, int c0, int n
                            d.slot.p, d.wave.p, d.off.p, d.len.p, d.extPtr.p, d.slotWave.p, d.rowId.p,
                            d.rowNorm.p, d.ext.p, d.sidx.p, d.sval.p, d.zero, x, b, n, nrhs, d.chainPtr.p, c0, nchain, xcd,
                            d.h.phLocal[p], bX, bB, bN);
+    else if (nrhs == 1 && BS == 256 && !BATCH && d.h.phExt[p] > 2 * BS)
+        // many outside columns: gather up to 6*BS of them in one batch
+        hipLaunchKernelGGL((k_gs_bundle<BS, TPR, EE, VT, XT, true, BATCH, 6>), dim3(grid.x), dim3(BS), d.lds(p, BS),
+                           st, d.slot.p, d.wave.p, d.off.p, d.len.p, d.extPtr.p, d.slotWave.p, d.rowId.p, d.diag.p,
+                           d.ext.p, d.sidx.p, d.sval.p, d.zero, x, b, n, nrhs, d.chainPtr.p, c0, nchain, bundle0, xcd, bX,
+                           bB, bN);
     else if (nrhs == 1)
         hipLaunchKernelGGL((k_gs_bundle<BS, TPR, EE, VT, XT, true, BATCH>), dim3(grid.x), dim3(BS), d.lds(p, BS), st,
                            d.slot.p, d.wave.p, d.off.p, d.len.p, d.extPtr.p, d.slotWave.p, d.rowId.p, d.diag.p,
@@ -688,6 +879,80 @@ void launchPhase(int bs, int tpr, int E, int kind, int nrhs, const BundleDev<VT,
 #undef RACE_BUNDLE_CASE
 }
 
+// Device state of the one-launch sweep of one direction.
+struct FlowDev {
+    DevBuf<int> depPtr, dep, flag, ext;
+    DevBuf<FlowState> fs;
+    int nchain = 0, grid = 0;
+    size_t lds = 0;
+    // chains whose conflicts lie inside one phase (0 for a valid colouring)
+    long samePhase = 0;
+    template <typename VT, typename XT>
+    int init(const BundleDev<VT, XT>& d, int n, int bs) {
+        std::vector<int> dp, dd;
+        samePhase = chainDeps(d.h, n, &dp, &dd);
+        nchain = (int)d.h.chainPtr.size() - 1;
+        for (int p = 0; p < d.h.numPhases(); ++p) lds = std::max(lds, d.lds(p, bs));
+        int e;
+        if ((e = upload(depPtr, dp.data(), dp.size()))) return e;
+        if ((e = upload(dep, dd.data(), dd.size()))) return e;
+        const std::vector<int> fx = flowExt(d.h, n);
+        if ((e = upload(ext, fx.data(), fx.size()))) return e;
+        const std::vector<int> zeros(std::max(nchain, 1), 0);
+        if ((e = upload(flag, zeros.data(), zeros.size()))) return e;
+        // flags start below the first epoch
+        FlowState f0{};
+        f0.epoch = 1;
+        if ((e = upload(fs, &f0, 1))) return e;
+        return OK;
+    }
+    int aborts(int* out) const {
+        FlowState f{};
+        RACE_HIP(hipMemcpy(&f, fs.p, sizeof(f), hipMemcpyDeviceToHost));
+        *out = f.aborts;
+        return OK;
+    }
+};
+
+// polls of one dependency flag before a launch gives up (about seconds;
+// a correct dependency table never gets near it)
+constexpr int kFlowPollLimit = 1 << 22;
+
+template <int TPR, int EE, typename VT, typename XT>
+int launchFlow(const BundleDev<VT, XT>& d, FlowDev& f, XT* x, const XT* b, hipStream_t st) {
+    constexpr int BS = 256;
+    auto kern = k_gs_flow<BS, TPR, EE, VT, XT>;
+    if (f.grid == 0) {
+        // as many workgroups as are resident at once (more would only find
+        // the tickets gone)
+        int per = 0, dev = 0, cus = 0;
+        RACE_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void*>(kern), BS, f.lds));
+        RACE_HIP(hipGetDevice(&dev));
+        RACE_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        f.grid = std::max(1, std::min(f.nchain, std::max(per, 1) * cus));
+    }
+    hipLaunchKernelGGL(kern, dim3(f.grid), dim3(BS), f.lds, st, d.slot.p, d.wave.p, d.off.p, d.len.p, d.extPtr.p,
+                       d.slotWave.p, d.rowId.p, d.diag.p, f.ext.p, d.sidx.p, d.sval.p, d.zero, x, b, d.chainPtr.p,
+                       f.nchain, f.depPtr.p, f.dep.p, f.flag.p, f.fs.p, kFlowPollLimit);
+    RACE_HIP(hipGetLastError());
+    return OK;
+}
+
+template <typename VT, typename XT>
+int launchFlowShape(int tpr, int E, const BundleDev<VT, XT>& d, FlowDev& f, XT* x, const XT* b, hipStream_t st) {
+    switch (tpr * 100 + E) {
+        case 104: return launchFlow<1, 4>(d, f, x, b, st);
+        case 108: return launchFlow<1, 8>(d, f, x, b, st);
+        case 116: return launchFlow<1, 16>(d, f, x, b, st);
+        case 208: return launchFlow<2, 8>(d, f, x, b, st);
+        case 213: return launchFlow<2, 13>(d, f, x, b, st);
+        case 216: return launchFlow<2, 16>(d, f, x, b, st);
+        case 408: return launchFlow<4, 8>(d, f, x, b, st);
+        case 416: return launchFlow<4, 16>(d, f, x, b, st);
+        default: return ERR_INTERNAL;
+    }
+}
+
 // lanes per row and register width for the longest row (diagonal excluded
 // for Gauss-Seidel); E < 0 if no kernel instantiation fits.  Rows longer
 // than 64 entries select variable lanes per row (tpr 0, kVarE registers
@@ -715,6 +980,10 @@ class BundleFn : public DeviceFn {
     Dev df, dr;
     int nrhs = 1, n = 0, bs = 256, tpr = 2, E = 16, xcd = 1;
     std::unique_ptr<DeviceFn> fallback;
+    // one launch per sweep (the "flow" option): Gauss-Seidel, one
+    // right-hand side, 256-thread workgroups, uniform lanes per row
+    bool flow = false;
+    FlowDev ff, fr;
 
     int init(const race_kernel_desc& d, const ExecContext& ctx) {
         kind = d.kind;
@@ -747,6 +1016,20 @@ class BundleFn : public DeviceFn {
             if (df.lds(p, bs) > kMaxLds) return fallBack();
         for (int p = 0; p < dr.h.numPhases(); ++p)
             if (dr.lds(p, bs) > kMaxLds) return fallBack();
+        if (ctx.flow && kind == RACE_K_GS && nrhs == 1 && bs == 256 && tpr > 0) {
+            if ((e = ff.init(df, n, bs)) || (e = fr.init(dr, n, bs))) return e;
+            // conflicts inside a phase would serialise its chains: keep the
+            // phase launches then
+            flow = ff.samePhase == 0 && fr.samePhase == 0;
+        }
+        return OK;
+    }
+    int flowAborts(int* out) const override {
+        *out = 0;
+        if (!flow) return OK;
+        int a = 0, c = 0, e;
+        if ((e = ff.aborts(&a)) || (e = fr.aborts(&c))) return e;
+        *out = a + c;
         return OK;
     }
 
@@ -757,6 +1040,7 @@ class BundleFn : public DeviceFn {
     bool graphable() const override { return fallback ? fallback->graphable() : true; }
     int launches(bool rev) const override {
         if (fallback) return fallback->launches(rev);
+        if (flow) return 1;
         const std::vector<int>& pc = (rev ? dr : df).h.phaseChain;
         int k = 0;
         for (size_t p = 0; p + 1 < pc.size(); ++p) k += pc[p + 1] > pc[p];
@@ -769,6 +1053,7 @@ class BundleFn : public DeviceFn {
             return fallback->run(rev, st);
         }
         const Dev& d = rev ? dr : df;
+        if (flow) return launchFlowShape<VT, XT>(tpr, E, d, rev ? fr : ff, (XT*)x, (const XT*)b, st);
         const std::vector<int>& pc = d.h.phaseChain;
         for (int p = 0; p + 1 < (int)pc.size(); ++p) {
             const int c0 = pc[p], nchain = pc[p + 1] - pc[p];

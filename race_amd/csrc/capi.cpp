@@ -108,7 +108,7 @@ struct race_handle_s {
     std::vector<Function> funcs;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int optLayout = 0, optBundleRows = 128, optBundleLocal = 6144, optBundleThreads = 256, optBundleTpr = 0, optXcd = 0;
-    int optBfsBlockMax = 1 << 17, optSymmMode = 0, optGraph = 0;
+    int optBfsBlockMax = 1 << 17, optSymmMode = 0, optGraph = 0, optFlow = 0;
     ~race_handle_s() {
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
@@ -533,6 +533,7 @@ int race_register_device(race_handle h, const race_kernel_desc* d, int* id) {
     ctx.bundleThreads = h->optBundleThreads;
     ctx.bundleTpr = h->optBundleTpr;
     ctx.symmMode = h->optSymmMode;
+    ctx.flow = h->optFlow;
     Function fn;
     int e = makeDeviceFn(*d, ctx, &fn.dev);
     if (e) return e;
@@ -568,6 +569,9 @@ int race_set_option(race_handle h, const char* key, int value) {
     } else if (k == "graph") {
         if (value != 0 && value != 1) return ERR_INVALID_ARG;
         h->optGraph = value;
+    } else if (k == "flow") {
+        if (value != 0 && value != 1) return ERR_INVALID_ARG;
+        h->optFlow = value;
     } else if (k == "bfs_block_max") {
         if (value < 0) return ERR_INVALID_ARG;
         h->optBfsBlockMax = value;
@@ -1052,6 +1056,13 @@ int race_get_launch_count(race_handle h, int id, int rev, int* launches) {
         return ERR_INVALID_ARG;
     *launches = h->funcs[id].dev->launches(rev != 0);
     return OK;
+}
+
+int race_get_flow_aborts(race_handle h, int id, int* aborts) {
+    if (checkHandle(h) || id < 0 || id >= (int)h->funcs.size() || !aborts || !h->funcs[id].dev)
+        return ERR_INVALID_ARG;
+    RACE_HIP(hipStreamSynchronize(h->stream));
+    return h->funcs[id].dev->flowAborts(aborts);
 }
 
 int race_get_wave_count(race_handle h, int id, int rev, int* waves) {

@@ -35,6 +35,12 @@ class DeviceFn {
     virtual bool graphable() const { return false; }
     // kernel launches one execution issues
     virtual int launches(bool /*rev*/) const { return 1; }
+    // one-launch sweeps whose dependency waits gave up (0 unless the
+    // dependency table is broken; synchronises with the device)
+    virtual int flowAborts(int* out) const {
+        *out = 0;
+        return 0;
+    }
     void* x = nullptr;
     void* b = nullptr;
     int power = 1;
@@ -59,6 +65,7 @@ struct ExecContext {
     int symmMode = 0;         // SymmSpMV: 0 auto, 1 colour phases, 2 SELL atomics, 3 expanded
     int sellC = 64;           // batched SELL kernels: rows per chunk (8, 16, 32, 64)
     int sellSigma = 1;        // batched SELL kernels: row-sorting window (1 = unsorted)
+    int flow = 0;             // Gauss-Seidel: one launch per sweep with per-chain dependency flags
 };
 
 int makeDeviceFn(const race_kernel_desc& d, const ExecContext& ctx, std::unique_ptr<DeviceFn>* out);

@@ -345,6 +345,13 @@ class Interface:
         L.check(self._L.race_get_launch_count(self.h, fid, int(rev), ctypes.byref(v)), "launchCount")
         return v.value
 
+    def flowAborts(self, fid):
+        """one-launch sweeps whose dependency waits gave up (0 when the
+        schedule is consistent); synchronises with the device"""
+        v = ctypes.c_int()
+        L.check(self._L.race_get_flow_aborts(self.h, fid, ctypes.byref(v)), "flowAborts")
+        return v.value
+
     def waveCount(self, fid, rev=False):
         v = ctypes.c_int()
         L.check(self._L.race_get_wave_count(self.h, fid, int(rev), ctypes.byref(v)), "waveCount")
