@@ -429,7 +429,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--grid", type=int, default=128)
-    ap.add_argument("--threads", type=int, default=int(os.environ.get("RACE_BENCH_THREADS", "65536")))
+    ap.add_argument("--threads", type=int, default=int(os.environ.get("RACE_BENCH_THREADS", "131072")))
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
@@ -489,7 +489,7 @@ def main():
         rpt = TU.RACE_ROWS_PER_THREAD[int(TU.Tuner.pretrained("race").predict(TU.features(rp, col))[0])]
         race_tuner = {"proposed_rows_per_thread": rpt, "proposed_threads": max(1, n // rpt),
                       "seconds": round(time.time() - t0, 2), "threads_used": args.threads}
-        log(f"tuner proposes {rpt} rows per RACE thread ({n // rpt} threads); measured best 32 (65536)")
+        log(f"tuner proposes {rpt} rows per RACE thread ({n // rpt} threads); measured best 16 (131072)")
     except OSError:
         pass
 

@@ -301,7 +301,21 @@ __global__ void __launch_bounds__(BS) k_gs_flow(
     const int epoch = ldAgent(&fs->epoch);
     if (t == 0) sTicket = atomicAdd(&fs->ticket, 1);
     __syncthreads();
-    int cur = sTicket;
+    // the chain index is uniform: its table entries load through the scalar unit
+    int cur = __builtin_amdgcn_readfirstlane(sTicket);
+    // The previous chain's flag goes out at the next chain's first wait for
+    // memory, which covers the previous chain's stores of x: their
+    // completion overlaps the next prologue.  It is published before that
+    // chain waits for anything, so no wait can depend on it.
+    int pending = -1;
+    auto publish = [&]() {
+        if (pending >= 0) {
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+            if (t == 0) stAgent(&flag[pending], epoch);
+            pending = -1;
+        }
+    };
     while (cur < nchain) {
         int next = 0;
         const int cLo = chainPtr[cur], cHi = chainPtr[cur + 1];
@@ -348,6 +362,7 @@ __global__ void __launch_bounds__(BS) k_gs_flow(
             // (the wave waits for it where it waits for them), and read after
             // this chain
             if (first && t == 0) next = atomicAdd(&fs->ticket, 1);
+            publish();
             // the chains this one conflicts with, then the columns they wrote
             if (t < nd) waitChain(flag, dj, epoch, fs, pollLimit);
             for (int j = t + BS; j < nd; j += BS) waitChain(flag, dep[d0 + j], epoch, fs, pollLimit);
@@ -382,16 +397,13 @@ __global__ void __launch_bounds__(BS) k_gs_flow(
             if (own(active, l)) stCoh(&x[row], xl[i]);
             __syncthreads();
         }
-        // every wave's stores of x are complete before the flag goes out
-        __builtin_amdgcn_s_waitcnt(0);
+        // the flag goes out once every wave's stores of x are complete
+        pending = cur;
+        if (t == 0) sTicket = next;
         __syncthreads();
-        if (t == 0) {
-            stAgent(&flag[cur], epoch);
-            sTicket = next;
-        }
-        __syncthreads();
-        cur = sTicket;
+        cur = __builtin_amdgcn_readfirstlane(sTicket);
     }
+    publish();
     // the last workgroup to finish prepares the next launch
     if (t == 0 && atomicAdd(&fs->done, 1) == (int)gridDim.x - 1) {
         if (ldAgent(&fs->abort)) {
