@@ -310,6 +310,7 @@ def bench_batched(args, R, M):
     kinds = {}
     singles = []
     cpu_sample = []
+    kacz_x = []
     t0 = time.time()
     gen = ((kd, n, mean, rp, col, val) for kd, n, mean, rp, col, val in M.batch_matrices(count, args.batch_seed)
            if args.batch_kinds in ("both", kd))
@@ -328,6 +329,7 @@ def bench_batched(args, R, M):
         dx, db = R.DeviceArray.from_host(np.zeros(nrhs * n, np.float32)), R.DeviceArray.from_host(bvec)
         sx, sb = R.DeviceArray.from_host(xvec), R.DeviceArray.from_host(np.zeros(nrhs * n, np.float32))
         kacz.add(ce, prp, pcol, pval, dx, db)
+        kacz_x.append(dx)
         symm.add(ce, urp, ucol, uval, sx, sb)
         if kept[k] in tune_set:
             tk.add(ce, prp, pcol, pval, dx, db)
@@ -336,7 +338,7 @@ def bench_batched(args, R, M):
             fk.add(ce, prp, pcol, pval, dx, db)
             singles.append((ce, prp, pcol, pval, n))
         if k < 8:
-            cpu_sample.append((ce, prp, pcol, pval, n, bvec.astype(np.float64)))
+            cpu_sample.append((ce, prp, pcol, pval, n, bvec.astype(np.float64), dx))
         bytes_symm += nnzu * 8 + n * (4 + nrhs * 3 * 4)
         bytes_kacz += nnz * 8 + n * (4 + 8 + nrhs * 3 * 4)
         tot_n += n
@@ -348,8 +350,16 @@ def bench_batched(args, R, M):
     tune_n = min(tune_n, count)
     t_prep = time.time() - t0
     t0 = time.time()
-    bk, times_k = tk.autotune(False, reps=2)
-    bs, times_s = ts.autotune(False, reps=3)
+    for bt in (kacz, tk, fk):
+        bt.setOption("flow", args.batch_flow)
+    if args.batch_layouts:
+        bk, bs = (int(v) for v in args.batch_layouts.split(","))
+        times_k, times_s = [], []
+    else:
+        def prog(name):
+            return lambda i, ms: log(f"tuning {name}: candidate {i} {ms:.2f} ms")
+        bk, times_k = tk.autotune(False, reps=2, progress=prog("KACZ"))
+        bs, times_s = ts.autotune(False, reps=3, progress=prog("SymmSpMV"))
     log(f"tuned on {tune_n} matrices in {time.time() - t0:.1f}s: KACZ layout {bk} "
         f"{R.Batch.candidate(R.K_KACZ, bk)}, SymmSpMV layout {bs} {R.Batch.candidate(R.K_SYMMSPMV, bs)}")
     # batched against per-matrix launches on the first tune_n matrices
@@ -377,24 +387,40 @@ def bench_batched(args, R, M):
     t_s = symm.time(False, reps=5)
     t_f = kacz.time(False, reps=3)
     t_b = kacz.time(True, reps=3)
+    launches = (kacz.launchCount(False), kacz.launchCount(True))
+    aborts = kacz.flowAborts() if args.batch_flow else 0
     ms = t_s + sweeps / 2 * (t_f + t_b)
     total = bytes_symm + sweeps * bytes_kacz
     gbs = total / (ms * 1e-3) / 1e9
     log(f"batched step: SymmSpMV {t_s:.3f} ms + {sweeps} x KACZ ({t_f:.3f} / {t_b:.3f} ms) = {ms:.2f} ms, "
         f"{gbs:.1f} GB/s; per-matrix launches on the sample: {t_single:.2f} ms vs batched {t_batch_sample:.2f} ms")
     cpu = None
+    parity = None
     if not args.no_cpu:
         from oracle import race_oracle as O
         O.build()
         cb = ct = 0.0
-        for ce, prp, pcol, pval, n, bv in cpu_sample:
+        # parity: one forward sweep of the batch from x = 0 (FP32 values and
+        # vectors, FP64 row norms) against the oracle's FP64 sweep in RACE
+        # order on the sample matrices
+        for xd in kacz_x:
+            xd.zero()
+        kacz.execute(False)
+        errs = []
+        for ce, prp, pcol, pval, n, bv, dxs in cpu_sample:
             pp, lo, hi = ce.schedule(False)
             calls = np.stack([lo, hi], axis=1).reshape(-1)
             xo = np.zeros(nrhs * n)
             t1 = time.time()
             O.run_calls(O.K_KACZ, prp, pcol, pval, xo, bv, calls, nrhs=nrhs)
             ct += time.time() - t1
+            xg = dxs.download().astype(np.float64)
+            errs.append(float(np.linalg.norm(xg - xo) / np.linalg.norm(xo)))
             cb += int(prp[-1]) * 8 + n * (4 + 8 + nrhs * 3 * 4)
+        parity = {"kacz_fp32_rel_err_vs_oracle_fp64": max(errs), "matrices": len(errs), "tol": 1e-5,
+                  "ok": bool(max(errs) <= 1e-5)}
+        log(f"parity: one forward KACZ sweep, FP32 batch vs FP64 oracle, worst rel err {max(errs):.2e} "
+            f"on {len(errs)} matrices")
         cpu = {"value": round(cb / ct / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
                "sample": f"one forward KACZ sweep, 8 right-hand sides, on the first {len(cpu_sample)} matrices "
                          f"of the batch (oracle C restatement in FP64, 1 thread, {ct * 1e3:.0f} ms)"}
@@ -411,11 +437,13 @@ def bench_batched(args, R, M):
         "roofline": {"bound": "hbm", "achieved": round(gbs, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 5), "traffic": None},
         "cpu_baseline": cpu,
+        "parity": parity,
         "batched": {"symmspmv_ms": round(t_s, 4), "kacz_fwd_ms": round(t_f, 4), "kacz_bwd_ms": round(t_b, 4),
                     "tuned_kacz_layout": list(R.Batch.candidate(R.K_KACZ, bk)),
                     "tuned_symmspmv_layout": list(R.Batch.candidate(R.K_SYMMSPMV, bs)),
                     "tuner_kacz_ms": [round(t, 4) for t in times_k],
                     "tuner_symmspmv_ms": [round(t, 4) for t in times_s],
+                    "flow": args.batch_flow, "kacz_launches_fwd_bwd": list(launches), "flow_aborts": aborts,
                     "sample_matrices": tune_n, "sample_per_matrix_launches_ms": round(t_single, 4),
                     "sample_batched_ms": round(t_batch_sample, 4),
                     "batching_speedup": round(t_single / t_batch_sample, 3),
@@ -445,6 +473,11 @@ def main():
                     help="RACE threads requested per matrix: n / this (4 .. 1024)")
     ap.add_argument("--batch-kinds", choices=["both", "banded", "chung-lu"], default="both",
                     help="keep only one matrix family of the batch")
+    ap.add_argument("--batch-flow", type=int, choices=[0, 1], default=1,
+                    help="Kaczmarz batch: one launch per sweep and shape group, every matrix through its own "
+                         "phases (1), or one fused launch per phase index (0)")
+    ap.add_argument("--batch-layouts", default="",
+                    help="KACZ,SymmSpMV candidate layout indices (skips the autotune)")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the SymmSpMV 64^3 and MPK 256^3 measurements")
     ap.add_argument("--graph", type=int, choices=[0, 1], default=0,
@@ -463,7 +496,11 @@ def main():
     from race_amd import matgen as M
 
     if args.workload == "batched":
-        print(json.dumps(bench_batched(args, R, M)), flush=True)
+        out = bench_batched(args, R, M)
+        print(json.dumps(out), flush=True)
+        if out.get("parity") and not out["parity"]["ok"]:
+            log("PARITY FAILED for the batched Kaczmarz sweep")
+            sys.exit(3)
         return
     if args.workload == "symmspmv":
         r = bench_symmspmv(R, M)

@@ -173,12 +173,13 @@ int race_register_device(race_handle h, const race_kernel_desc *desc, int *id);
  *   "bfs_block_max" ranges of at most this many rows are traversed one
  *                  workgroup each, batched (default 131072; 0 = every range
  *                  level by level over the whole device); set before colouring
- *   "flow"         Gauss-Seidel with one right-hand side: 1 = one launch per
- *                  sweep; the chains of all colour phases are handed out in
- *                  phase order and each waits only for the earlier chains it
- *                  conflicts with (per-chain completion flags) instead of a
- *                  launch boundary per phase (default 0); set before
- *                  registering; results are identical
+ *   "flow"         1 = one launch per sweep (Gauss-Seidel with one
+ *                  right-hand side and 256-thread workgroups; Kaczmarz with
+ *                  256-1024-thread workgroups): the chains of all colour
+ *                  phases are handed out in phase order and each waits only
+ *                  for the earlier chains it conflicts with (per-chain
+ *                  completion flags) instead of a launch boundary per phase
+ *                  (default 0); set before registering; results identical
  * No counterpart in the reference (its executor has no layout choice). */
 int race_set_option(race_handle h, const char *key, int value);
 /* re-point a registered device function at other vectors */
@@ -364,9 +365,18 @@ int race_batch_create(race_batch *out, int kind, int val_precision, int vec_prec
 int race_batch_add(race_batch b, race_handle h, const race_kernel_desc *d);
 /* "bundle_threads", "bundle_rows", "bundle_tpr" (0, 1, 2, 4),
  * "bundle_local" (GS / KACZ); "sell_c" (8, 16, 32, 64), "sell_sigma"
- * (row-sorting window, 1 = none) (SpMV / SymmSpMV) */
+ * (row-sorting window, 1 = none) (SpMV / SymmSpMV); "flow" (Kaczmarz,
+ * 256-1024-thread workgroups): 1 = one launch per sweep and matrix-shape
+ * group, every matrix advancing through its own colour phases with per-chain
+ * dependency flags instead of one fused launch per phase index (default 0;
+ * results identical) */
 int race_batch_set_option(race_batch b, const char *key, int value);
 int race_batch_get_option(race_batch b, const char *key, int *value);
+/* kernel launches of one execution of the built batch; one-launch sweeps
+ * whose dependency waits gave up (always 0 for consistent schedules;
+ * synchronises with the device) */
+int race_batch_get_launch_count(race_batch b, int rev, int *launches);
+int race_batch_get_flow_aborts(race_batch b, int *aborts);
 int race_batch_build(race_batch b);
 int race_batch_execute(race_batch b, int rev);
 int race_batch_execute_async(race_batch b, int rev, void *stream);

@@ -127,6 +127,8 @@ PROTOTYPES = {
     "race_batch_add": (ctypes.c_int, [vp, vp, ctypes.POINTER(KernelDesc)]),
     "race_batch_set_option": (ctypes.c_int, [vp, ctypes.c_char_p, ctypes.c_int]),
     "race_batch_get_option": (ctypes.c_int, [vp, ctypes.c_char_p, i32p]),
+    "race_batch_get_launch_count": (ctypes.c_int, [vp, ctypes.c_int, i32p]),
+    "race_batch_get_flow_aborts": (ctypes.c_int, [vp, i32p]),
     "race_batch_build": (ctypes.c_int, [vp]),
     "race_batch_execute": (ctypes.c_int, [vp, ctypes.c_int]),
     "race_batch_execute_async": (ctypes.c_int, [vp, ctypes.c_int, vp]),

@@ -415,7 +415,20 @@ BundleTable concatBundles(const std::vector<const BundleTable*>& tabs, std::vect
 
 namespace race {
 
-long chainDeps(const BundleTable& t, int n, std::vector<int>* depPtr, std::vector<int>* dep) {
+namespace {
+// key of the row / column id v of bundle b
+inline long keyOf(const std::vector<long>* colBase, int b, int v) { return (colBase ? (*colBase)[b] : 0) + v; }
+// chain of every bundle
+std::vector<int> chainOfBundle(const BundleTable& t) {
+    std::vector<int> cb(t.slot.size() > 0 ? t.slot.size() - 1 : 0, 0);
+    for (int c = 0; c + 1 < (int)t.chainPtr.size(); ++c)
+        for (int b = t.chainPtr[c]; b < t.chainPtr[c + 1]; ++b) cb[b] = c;
+    return cb;
+}
+}  // namespace
+
+long chainDeps(const BundleTable& t, long nkeys, const std::vector<long>* colBase, std::vector<int>* depPtr,
+               std::vector<int>* dep) {
     const int nchain = (int)t.chainPtr.size() - 1;
     std::vector<int> phaseOf(std::max(nchain, 0), 0);
     for (int p = 0; p < t.numPhases(); ++p)
@@ -428,15 +441,15 @@ long chainDeps(const BundleTable& t, int n, std::vector<int>* depPtr, std::vecto
         edge.push_back({waiter, first});
     };
     if (t.kind != RACE_K_KACZ) {
-        // owner[r]: chain that writes row r
-        std::vector<int> owner(n, -1);
+        // owner[key]: chain that writes that row
+        std::vector<int> owner(nkeys, -1);
         for (int c = 0; c < nchain; ++c)
             for (int b = t.chainPtr[c]; b < t.chainPtr[c + 1]; ++b)
-                for (int s = t.slot[b]; s < t.slot[b + 1]; ++s) owner[t.rowId[s]] = c;
+                for (int s = t.slot[b]; s < t.slot[b + 1]; ++s) owner[keyOf(colBase, b, t.rowId[s])] = c;
         for (int c = 0; c < nchain; ++c)
             for (int b = t.chainPtr[c]; b < t.chainPtr[c + 1]; ++b)
                 for (int e = t.extPtr[b]; e < t.extPtr[b + 1]; ++e) {
-                    const int o = owner[t.ext[e]];
+                    const int o = owner[keyOf(colBase, b, t.ext[e])];
                     if (o < 0 || o == c) continue;  // not written in the sweep, or by this chain
                     if (o < c)
                         add(c, o);  // c reads what o wrote
@@ -446,15 +459,15 @@ long chainDeps(const BundleTable& t, int n, std::vector<int>* depPtr, std::vecto
     } else {
         // every column a chain touches is read and written: each chain waits
         // for the previous toucher of each of its columns
-        std::vector<int> last(n, -1);
+        std::vector<int> last(nkeys, -1);
         for (int c = 0; c < nchain; ++c) {
             for (int b = t.chainPtr[c]; b < t.chainPtr[c + 1]; ++b)
                 for (int e = t.extPtr[b]; e < t.extPtr[b + 1]; ++e) {
-                    const int o = last[t.ext[e]];
+                    const int o = last[keyOf(colBase, b, t.ext[e])];
                     if (o >= 0 && o != c) add(c, o);
                 }
             for (int b = t.chainPtr[c]; b < t.chainPtr[c + 1]; ++b)
-                for (int e = t.extPtr[b]; e < t.extPtr[b + 1]; ++e) last[t.ext[e]] = c;
+                for (int e = t.extPtr[b]; e < t.extPtr[b + 1]; ++e) last[keyOf(colBase, b, t.ext[e])] = c;
         }
     }
     std::sort(edge.begin(), edge.end());
@@ -469,19 +482,29 @@ long chainDeps(const BundleTable& t, int n, std::vector<int>* depPtr, std::vecto
     return same;
 }
 
-std::vector<int> flowExt(const BundleTable& t, int n) {
-    const int nchain = (int)t.chainPtr.size() - 1;
-    std::vector<int> owner(n, -1);
-    for (int c = 0; c < nchain; ++c)
-        for (int b = t.chainPtr[c]; b < t.chainPtr[c + 1]; ++b)
-            for (int s = t.slot[b]; s < t.slot[b + 1]; ++s) owner[t.rowId[s]] = c;
+std::vector<int> flowExt(const BundleTable& t, long nkeys, const std::vector<long>* colBase) {
+    const std::vector<int> cb = chainOfBundle(t);
+    const int nb = (int)cb.size();
     std::vector<int> out(t.ext);
-    for (int c = 0; c < nchain; ++c)
-        for (int b = t.chainPtr[c]; b < t.chainPtr[c + 1]; ++b)
+    auto mark = [&](int e) { out[e] = (int)((unsigned)t.ext[e] | 0x80000000u); };
+    if (t.kind != RACE_K_KACZ) {
+        std::vector<int> owner(nkeys, -1);
+        for (int b = 0; b < nb; ++b)
+            for (int s = t.slot[b]; s < t.slot[b + 1]; ++s) owner[keyOf(colBase, b, t.rowId[s])] = cb[b];
+        for (int b = 0; b < nb; ++b)
             for (int e = t.extPtr[b]; e < t.extPtr[b + 1]; ++e) {
-                const int o = owner[t.ext[e]];
-                if (o >= 0 && o <= c) out[e] = (int)((unsigned)t.ext[e] | 0x80000000u);
+                const int o = owner[keyOf(colBase, b, t.ext[e])];
+                if (o >= 0 && o <= cb[b]) mark(e);
             }
+    } else {
+        // bundles in order: a column is fresh once any bundle before has touched it
+        std::vector<char> touched(nkeys, 0);
+        for (int b = 0; b < nb; ++b) {
+            for (int e = t.extPtr[b]; e < t.extPtr[b + 1]; ++e)
+                if (touched[keyOf(colBase, b, t.ext[e])]) mark(e);
+            for (int e = t.extPtr[b]; e < t.extPtr[b + 1]; ++e) touched[keyOf(colBase, b, t.ext[e])] = 1;
+        }
+    }
     return out;
 }
 

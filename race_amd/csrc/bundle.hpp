@@ -84,15 +84,20 @@ BundleTable concatBundles(const std::vector<const BundleTable*>& tabs, std::vect
 //   GS  : an earlier chain that writes a row c reads (an outside column of
 //         c), or that reads a row c writes;
 //   KACZ: for each column c touches, the previous chain touching it.
+// For a fused table of several matrices, colBase[b] offsets the row and
+// column ids of bundle b into one key space of nkeys entries (each matrix
+// its own range); for one matrix colBase is null and nkeys = n.
 // Returns the number of such pairs inside one phase (0 for a valid
 // colouring: the leaves of a phase are independent).
-long chainDeps(const BundleTable& t, int n, std::vector<int>* depPtr, std::vector<int>* dep);
+long chainDeps(const BundleTable& t, long nkeys, const std::vector<long>* colBase, std::vector<int>* depPtr,
+               std::vector<int>* dep);
 
-// Outside columns of a Gauss-Seidel table for the one-launch sweep: t.ext
-// with bit 31 set on every column written earlier in the launch (by an
-// earlier chain, or by an earlier bundle of the same chain).  Only those must
-// be read coherently after the dependency wait; the others still hold the
-// value the launch started with and are read with the matrix.
-std::vector<int> flowExt(const BundleTable& t, int n);
+// Local column ids (t.ext) for the one-launch sweep, with bit 31 set on
+// every column an earlier chain of the launch (or an earlier bundle of the
+// same chain) may have written: GS, a row of such a chain; KACZ, a column
+// such a chain touched.  Only those must be read coherently after the
+// dependency wait; the others still hold the value the launch started with.
+std::vector<int> flowExt(const BundleTable& t, long nkeys, const std::vector<long>* colBase);
+
 
 }  // namespace race

@@ -248,7 +248,8 @@ struct FlowState {
     int epoch;   // launch number: a chain's flag holds the epoch it finished in
     int abort;   // a wait exceeded its poll budget in this launch
     int aborts;  // launches that aborted (kept for the host)
-    int pad[3];
+    int tail;    // readiness-driven sweeps: next free slot of the ready queue
+    int pad[2];
 };
 
 __device__ __forceinline__ int ldAgent(const int* p) {
@@ -292,8 +293,8 @@ __global__ void __launch_bounds__(BS) k_gs_flow(
     const int* __restrict__ bLen, const int* __restrict__ bExt, const uint16_t* __restrict__ slotWave,
     const int* __restrict__ rowId, const XT* __restrict__ rdiag, const int* __restrict__ ext,
     const uint16_t* __restrict__ sidx, const VT* __restrict__ sval, long zero, XT* x, const XT* __restrict__ b,
-    const int* __restrict__ chainPtr, int nchain, const int* __restrict__ depPtr, const int* __restrict__ dep,
-    int* flag, FlowState* fs, int pollLimit) {
+    const int* __restrict__ tLo, const int* __restrict__ tHi, int nchain, const int* __restrict__ depPtr,
+    const int* __restrict__ dep, int* flag, FlowState* fs, int pollLimit) {
     extern __shared__ __align__(16) unsigned char smem[];
     XT* xl = reinterpret_cast<XT*>(smem);
     __shared__ int sTicket;
@@ -318,7 +319,7 @@ __global__ void __launch_bounds__(BS) k_gs_flow(
     };
     while (cur < nchain) {
         int next = 0;
-        const int cLo = chainPtr[cur], cHi = chainPtr[cur + 1];
+        const int cLo = tLo[cur], cHi = tHi[cur];
         for (int bnd = cLo; bnd < cHi; ++bnd) {
             const bool first = bnd == cLo;
             const int d0 = first ? depPtr[cur] : 0, nd = first ? depPtr[cur + 1] - d0 : 0;
@@ -416,8 +417,111 @@ __global__ void __launch_bounds__(BS) k_gs_flow(
     }
 }
 
+// Loads and stores of x in a bundle body.  FLOW: the column id may carry
+// bit 31 (written earlier in the one-launch sweep: read coherently), and
+// every store is coherent; otherwise plain accesses.
+template <bool FLOW, typename XT>
+__device__ __forceinline__ XT ldX(const XT* x, size_t vo, unsigned id) {
+    if constexpr (FLOW) {
+        const XT* p = x + vo + (id & 0x7fffffffu);
+        return (id >> 31) ? ldCoh(p) : *p;
+    } else {
+        return x[vo + id];
+    }
+}
+template <bool FLOW, typename XT>
+__device__ __forceinline__ void stX(XT* x, size_t vo, unsigned id, XT v) {
+    if constexpr (FLOW)
+        stCoh(x + vo + (id & 0x7fffffffu), v);
+    else
+        x[vo + id] = v;
+}
+// nothing to wait for (the phase launches)
+struct NoWait {
+    __device__ __forceinline__ void operator()() const {}
+};
+
 // Kaczmarz: every column the bundle touches lives in LDS; rows of one wave
-// share no column.  The row norms stay FP64.
+// share no column.  The row norms stay FP64.  One bundle for the right-hand
+// sides [k0, k1); wait() runs between the prologue's matrix loads and the
+// gathers of x.
+template <int BS, int TPR, int E, typename VT, typename XT, bool BATCH, bool FLOW, class Wait>
+__device__ __forceinline__ void kaczBundle(
+    const int bnd, const int k0, const int k1, XT* xl, Wait&& wait, const int* __restrict__ bSlot,
+    const int* __restrict__ bWave, const long* __restrict__ bOff, const int* __restrict__ bLen,
+    const int* __restrict__ bExt, const uint16_t* __restrict__ slotWave, const int* __restrict__ rowId,
+    const double* __restrict__ rowNorm, const int* __restrict__ ext, const uint16_t* __restrict__ sidx,
+    const VT* __restrict__ sval, long zero, XT* x0, const XT* __restrict__ b0, int n0, XT* const* __restrict__ bX,
+    const XT* const* __restrict__ bB, const int* __restrict__ bN) {
+    const BundleVec<BATCH, XT> vec(x0, b0, n0, bX, bB, bN, bnd);
+    XT* __restrict__ x = vec.x;
+    const XT* __restrict__ b = vec.b;
+    const int n = vec.n;
+    const int s0 = bSlot[bnd], nb = bSlot[bnd + 1] - s0;
+    const long off = bOff[bnd];
+    const int wb = bLen[bnd];
+    const int nw = bWave[bnd + 1] - bWave[bnd];
+    const int e0 = bExt[bnd], ne = bExt[bnd + 1] - e0;
+    const int t = threadIdx.x, i = t / TPR, l = t % TPR;
+    const bool active = i < nb;
+    const long lanes = (long)nb * TPR;
+    (void)l;
+
+    // branch-free prologue, ids first (as in k_gs_bundle)
+    const int sr = active ? s0 + i : 0;
+    const unsigned row = (unsigned)rowId[sr];
+    const unsigned ex0 = (unsigned)ext[t < ne ? e0 + t : 0];
+    const unsigned ex1 = (unsigned)ext[t + BS < ne ? e0 + t + BS : 0];
+    int li[E];
+    typename ValBits<VT>::T lv[E];
+#pragma unroll
+    for (int u = 0; u < E; ++u) {
+        const long e = (active && u < wb) ? off + (long)u * lanes + t : zero;
+        li[u] = (int)sidx[e];
+        lv[u] = sval[e];
+    }
+    const double rn = rowNorm[sr];
+    const int myWave = (int)slotWave[sr];
+    wait();
+
+    for (int k = k0; k < k1; ++k) {
+        const size_t vo = (size_t)k * n;
+        const XT g0 = ldX<FLOW>(x, vo, t < ne ? ex0 : 0u);
+        const XT g1 = ldX<FLOW>(x, vo, t + BS < ne ? ex1 : 0u);
+        const XT br = b[vo + (active ? row : 0u)];
+        if (t < ne) xl[t] = g0;
+        if (t + BS < ne) xl[t + BS] = g1;
+        for (int c = t + 2 * BS; c < ne; c += BS) xl[c] = ldX<FLOW>(x, vo, (unsigned)ext[e0 + c]);
+        __syncthreads();
+        for (int w = 0; w < nw; ++w) {
+            if (active && myWave == w) {
+                // the row's x entries, read once in one batch (no other
+                // row of this wave writes them)
+                XT cur[E];
+#pragma unroll
+                for (int u = 0; u < E; ++u) cur[u] = xl[li[u]];
+                XT dot = XT(0);
+#pragma unroll
+                for (int u = 0; u < E; ++u) dot += valOf<XT>(lv[u]) * cur[u];
+                dot = laneSum<TPR>(dot);
+                const XT scale = (XT)(((double)dot - (double)br) / rn);
+                // write back the entries with a nonzero value only: a
+                // padding entry may name a column another row updates
+#pragma unroll
+                for (int u = 0; u < E; ++u) {
+                    const XT a = valOf<XT>(lv[u]);
+                    if (a != XT(0)) xl[li[u]] = cur[u] - scale * a;
+                }
+            }
+            __syncthreads();
+        }
+        if (t < ne) stX<FLOW>(x, vo, ex0, xl[t]);
+        if (t + BS < ne) stX<FLOW>(x, vo, ex1, xl[t + BS]);
+        for (int c = t + 2 * BS; c < ne; c += BS) stX<FLOW>(x, vo, (unsigned)ext[e0 + c], xl[c]);
+        __syncthreads();
+    }
+}
+
 template <int BS, int TPR, int E, typename VT, typename XT, bool BATCH>
 __global__ void __launch_bounds__(BS) k_kacz_bundle(
     const int* __restrict__ bSlot, const int* __restrict__ bWave, const long* __restrict__ bOff,
@@ -436,75 +540,13 @@ __global__ void __launch_bounds__(BS) k_kacz_bundle(
     // a chain: one bundle, or the segments of a leaf too large for one bundle,
     // run in order (each segment reads what the previous one wrote)
     const int cLo = chainPtr[chain0 + q], cHi = chainPtr[chain0 + q + 1];
+    // right-hand sides are independent: grid.y splits them over workgroups
+    const int k0 = (int)blockIdx.y;
+    const int k1 = gridDim.y > 1 ? k0 + 1 : nrhs;
     auto runBundle = [&](const int bnd) {
-        const BundleVec<BATCH, XT> vec(x0, b0, n0, bX, bB, bN, bnd);
-        XT* __restrict__ x = vec.x;
-        const XT* __restrict__ b = vec.b;
-        const int n = vec.n;
-        const int s0 = bSlot[bnd], nb = bSlot[bnd + 1] - s0;
-        const long off = bOff[bnd];
-        const int wb = bLen[bnd];
-        const int nw = bWave[bnd + 1] - bWave[bnd];
-        const int e0 = bExt[bnd], ne = bExt[bnd + 1] - e0;
-        const int t = threadIdx.x, i = t / TPR, l = t % TPR;
-        const bool active = i < nb;
-        const long lanes = (long)nb * TPR;
-
-        // branch-free prologue, ids first (as in k_gs_bundle)
-        const int sr = active ? s0 + i : 0;
-        const unsigned row = (unsigned)rowId[sr];
-        const unsigned ex0 = (unsigned)ext[t < ne ? e0 + t : 0];
-        const unsigned ex1 = (unsigned)ext[t + BS < ne ? e0 + t + BS : 0];
-        int li[E];
-        typename ValBits<VT>::T lv[E];
-#pragma unroll
-        for (int u = 0; u < E; ++u) {
-            const long e = (active && u < wb) ? off + (long)u * lanes + t : zero;
-            li[u] = (int)sidx[e];
-            lv[u] = sval[e];
-        }
-        const double rn = rowNorm[sr];
-        const int myWave = (int)slotWave[sr];
-
-        // right-hand sides are independent: grid.y splits them over workgroups
-        const int k0 = (int)blockIdx.y;
-        const int k1 = gridDim.y > 1 ? k0 + 1 : nrhs;
-        for (int k = k0; k < k1; ++k) {
-            const size_t vo = (size_t)k * n;
-            const XT g0 = x[vo + (t < ne ? ex0 : 0u)];
-            const XT g1 = x[vo + (t + BS < ne ? ex1 : 0u)];
-            const XT br = b[vo + (active ? row : 0u)];
-            if (t < ne) xl[t] = g0;
-            if (t + BS < ne) xl[t + BS] = g1;
-            for (int c = t + 2 * BS; c < ne; c += BS) xl[c] = x[vo + ext[e0 + c]];
-            __syncthreads();
-            for (int w = 0; w < nw; ++w) {
-                if (active && myWave == w) {
-                    // the row's x entries, read once in one batch (no other
-                    // row of this wave writes them)
-                    XT cur[E];
-#pragma unroll
-                    for (int u = 0; u < E; ++u) cur[u] = xl[li[u]];
-                    XT dot = XT(0);
-#pragma unroll
-                    for (int u = 0; u < E; ++u) dot += valOf<XT>(lv[u]) * cur[u];
-                    dot = laneSum<TPR>(dot);
-                    const XT scale = (XT)(((double)dot - (double)br) / rn);
-                    // write back the entries with a nonzero value only: a
-                    // padding entry may name a column another row updates
-#pragma unroll
-                    for (int u = 0; u < E; ++u) {
-                        const XT a = valOf<XT>(lv[u]);
-                        if (a != XT(0)) xl[li[u]] = cur[u] - scale * a;
-                    }
-                }
-                __syncthreads();
-            }
-            if (t < ne) x[vo + ex0] = xl[t];
-            if (t + BS < ne) x[vo + ex1] = xl[t + BS];
-            for (int c = t + 2 * BS; c < ne; c += BS) x[vo + ext[e0 + c]] = xl[c];
-            __syncthreads();
-        }
+        kaczBundle<BS, TPR, E, VT, XT, BATCH, false>(bnd, k0, k1, xl, NoWait{}, bSlot, bWave, bOff, bLen, bExt,
+                                                       slotWave, rowId, rowNorm, ext, sidx, sval, zero, x0, b0, n0,
+                                                       bX, bB, bN);
     };
     // the common case, one bundle per workgroup, as its own straight-line copy
     if (cHi == cLo + 1) {
@@ -515,6 +557,111 @@ __global__ void __launch_bounds__(BS) k_kacz_bundle(
             __syncthreads();
         }
     }
+}
+
+// Readiness-driven one-launch sweep.  A workgroup takes a queue position
+// (ticket) and runs the chain stored there: the chains without dependencies
+// occupy the first positions (roots), every other chain is appended by the
+// workgroup that completes its last dependency.  So a workgroup never holds
+// a chain whose dependencies are still running (with tickets in chain order,
+// a matrix of a batch with a few long chains holds the slots of every other
+// matrix's later phases).  No deadlock: a workgroup takes a position only
+// after releasing the successors of its previous chain, and while positions
+// wait for chains, some chain with finished dependencies has been queued.
+// Per right-hand side (blockIdx.y) its own queue, arrival counters and state.
+struct ReadyTables {
+    const int* lo;             // chain -> first bundle
+    const int* hi;             // chain -> end of its bundles
+    const int* depPtr;         // chain -> its dependencies (their count)
+    const int* succPtr;        // chain -> chains depending on it
+    const int* succ;
+    const int* roots;          // chains without dependencies
+    unsigned* arrivals;        // per side: completed dependencies, summed over launches
+    unsigned long long* queue; // per side: (epoch << 32) | chain, from position nroots on
+    FlowState* fs;             // per side
+    int nchain, nroots, pollLimit;
+};
+
+// the chain at queue position h (>= nroots) once it has been written, or
+// -1 when the launch gives up (see waitChain)
+__device__ __forceinline__ int waitQueue(const unsigned long long* q, int h, int epoch, FlowState* fs, int pollLimit) {
+    int polls = 0;
+    for (;;) {
+        const unsigned long long v =
+            __hip_atomic_load(const_cast<unsigned long long*>(&q[h]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if ((int)(v >> 32) == epoch) return (int)(unsigned)v;
+        if ((++polls & 255) == 0 && (polls >= pollLimit || ldAgent(&fs->abort))) {
+            stAgent(&fs->abort, 1);
+            return -1;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+template <int BS, class Body>
+__device__ __forceinline__ void readyLoop(const ReadyTables& r, Body&& body) {
+    __shared__ int sChain;
+    const int t = threadIdx.x;
+    FlowState* fs = r.fs + blockIdx.y;
+    unsigned* arr = r.arrivals + (size_t)blockIdx.y * r.nchain;
+    unsigned long long* q = r.queue + (size_t)blockIdx.y * r.nchain;
+    const int epoch = ldAgent(&fs->epoch);
+    for (;;) {
+        if (t == 0) {
+            const int h = atomicAdd(&fs->ticket, 1);
+            sChain = h < r.nroots ? r.roots[h] : h < r.nchain ? waitQueue(q, h, epoch, fs, r.pollLimit) : -1;
+        }
+        __syncthreads();
+        const int c = __builtin_amdgcn_readfirstlane(sChain);
+        if (c < 0) break;
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        const int cLo = r.lo[c], cHi = r.hi[c];
+        for (int bnd = cLo; bnd < cHi; ++bnd) body(bnd, NoWait{});
+        // every wave's stores of x are complete before any successor is released
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        const int s0 = r.succPtr[c], ns = r.succPtr[c + 1] - s0;
+        for (int j = t; j < ns; j += BS) {
+            const int sc = r.succ[s0 + j];
+            const unsigned need = (unsigned)epoch * (unsigned)(r.depPtr[sc + 1] - r.depPtr[sc]);
+            if (atomicAdd(&arr[sc], 1u) + 1u == need) {
+                const int slot = atomicAdd(&fs->tail, 1);
+                __hip_atomic_store(&q[slot], ((unsigned long long)(unsigned)epoch << 32) | (unsigned)sc,
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+    }
+    // the last workgroup to finish prepares the next launch
+    if (t == 0 && atomicAdd(&fs->done, 1) == (int)gridDim.x - 1) {
+        if (ldAgent(&fs->abort)) {
+            atomicAdd(&fs->aborts, 1);
+            stAgent(&fs->abort, 0);
+        }
+        stAgent(&fs->ticket, 0);
+        stAgent(&fs->tail, r.nroots);
+        stAgent(&fs->done, 0);
+        stAgent(&fs->epoch, epoch + 1);
+    }
+}
+
+// Kaczmarz, all phases of a sweep in one readiness-driven launch; grid.y =
+// right-hand sides.
+template <int BS, int TPR, int E, typename VT, typename XT, bool BATCH>
+__global__ void __launch_bounds__(BS) k_kacz_flow(
+    const int* __restrict__ bSlot, const int* __restrict__ bWave, const long* __restrict__ bOff,
+    const int* __restrict__ bLen, const int* __restrict__ bExt, const uint16_t* __restrict__ slotWave,
+    const int* __restrict__ rowId, const double* __restrict__ rowNorm, const int* __restrict__ ext,
+    const uint16_t* __restrict__ sidx, const VT* __restrict__ sval, long zero, XT* x0, const XT* __restrict__ b0,
+    int n0, XT* const* __restrict__ bX, const XT* const* __restrict__ bB, const int* __restrict__ bN,
+    ReadyTables rt) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    XT* xl = reinterpret_cast<XT*>(smem);
+    const int k = (int)blockIdx.y;
+    readyLoop<BS>(rt, [&](int bnd, auto wait) {
+        kaczBundle<BS, TPR, E, VT, XT, BATCH, true>(bnd, k, k + 1, xl, wait, bSlot, bWave, bOff, bLen, bExt,
+                                                      slotWave, rowId, rowNorm, ext, sidx, sval, zero, x0, b0, n0,
+                                                      bX, bB, bN);
+    });
 }
 
 // ---------------------------------------------------------------------------
@@ -625,6 +772,99 @@ __global__ void __launch_bounds__(BS) k_gs_bundle_vl(
     }
 }
 
+// Variable lanes per row, Kaczmarz (see kaczBundle).  part: BS partial
+// sums, rl0: the first lane of every row of the bundle.
+template <int BS, typename VT, typename XT, bool BATCH, bool FLOW, class Wait>
+__device__ __forceinline__ void kaczBundleVL(
+    const int bnd, const int k0, const int k1, XT* xl, XT* part, int* rl0, Wait&& wait,
+    const int* __restrict__ bSlot, const int* __restrict__ bWave, const long* __restrict__ bOff,
+    const int* __restrict__ bLen, const int* __restrict__ bExt, const uint16_t* __restrict__ slotWave,
+    const int* __restrict__ rowId, const double* __restrict__ rowNorm, const int* __restrict__ ext,
+    const uint16_t* __restrict__ sidx, const VT* __restrict__ sval, long zero, XT* x0, const XT* __restrict__ b0,
+    int n0, const int* __restrict__ lanePtr, const uint16_t* __restrict__ laneSlot, XT* const* __restrict__ bX,
+    const XT* const* __restrict__ bB, const int* __restrict__ bN) {
+    constexpr int E = kVarE;
+    const BundleVec<BATCH, XT> vec(x0, b0, n0, bX, bB, bN, bnd);
+    XT* __restrict__ x = vec.x;
+    const XT* __restrict__ b = vec.b;
+    const int n = vec.n;
+    const int s0 = bSlot[bnd], nb = bSlot[bnd + 1] - s0;
+    const long off = bOff[bnd];
+    const int wb = bLen[bnd];
+    const int nw = bWave[bnd + 1] - bWave[bnd];
+    const int e0 = bExt[bnd], ne = bExt[bnd + 1] - e0;
+    const int lp = lanePtr[bnd], nl = lanePtr[bnd + 1] - lp;
+    const int t = threadIdx.x;
+    const bool active = t < nl;
+    // branch-free prologue, ids first (as in k_gs_bundle_vl)
+    const unsigned ex0 = (unsigned)ext[t < ne ? e0 + t : 0];
+    const unsigned ex1 = (unsigned)ext[t + BS < ne ? e0 + t + BS : 0];
+    const int i = (int)laneSlot[active ? lp + t : 0];  // used by active lanes only
+    const int iPrev = (int)laneSlot[active && t > 0 ? lp + t - 1 : 0];
+    int li[E];
+    typename ValBits<VT>::T lv[E];
+#pragma unroll
+    for (int u = 0; u < E; ++u) {
+        const long e = (active && u < wb) ? off + (long)u * nl + t : zero;
+        li[u] = (int)sidx[e];
+        lv[u] = sval[e];
+    }
+    const int sr = active ? s0 + i : 0;
+    const unsigned row = (unsigned)rowId[sr];
+    const double rn = rowNorm[sr];
+    const int myWave = (int)slotWave[sr];
+    if (active && (t == 0 || iPrev != i)) rl0[i] = t;
+    if (t == 0) rl0[nb] = nl;
+    int rowL0 = 0, rowL1 = 0;
+    wait();
+
+    for (int k = k0; k < k1; ++k) {
+        const size_t vo = (size_t)k * n;
+        const XT g0 = ldX<FLOW>(x, vo, t < ne ? ex0 : 0u);
+        const XT g1 = ldX<FLOW>(x, vo, t + BS < ne ? ex1 : 0u);
+        const XT br = b[vo + (active ? row : 0u)];
+        if (t < ne) xl[t] = g0;
+        if (t + BS < ne) xl[t + BS] = g1;
+        for (int c = t + 2 * BS; c < ne; c += BS) xl[c] = ldX<FLOW>(x, vo, (unsigned)ext[e0 + c]);
+        __syncthreads();
+        if (k == k0 && active) {
+            rowL0 = rl0[i];
+            rowL1 = rl0[i + 1];
+        }
+        for (int w = 0; w < nw; ++w) {
+            // the lane's x entries, read once in one batch and kept
+            // across the barrier (no other row of this wave writes them)
+            XT cur[E];
+            if (active && myWave == w) {
+#pragma unroll
+                for (int u = 0; u < E; ++u) cur[u] = xl[li[u]];
+                XT dot = XT(0);
+#pragma unroll
+                for (int u = 0; u < E; ++u) dot += valOf<XT>(lv[u]) * cur[u];
+                part[t] = dot;
+            }
+            __syncthreads();
+            if (active && myWave == w) {
+                XT rowDot = XT(0);
+                for (int l = rowL0; l < rowL1; ++l) rowDot += part[l];
+                const XT scale = (XT)(((double)rowDot - (double)br) / rn);
+                // nonzero entries only: a padding entry may name a
+                // column another row updates
+#pragma unroll
+                for (int u = 0; u < E; ++u) {
+                    const XT a = valOf<XT>(lv[u]);
+                    if (a != XT(0)) xl[li[u]] = cur[u] - scale * a;
+                }
+            }
+            __syncthreads();
+        }
+        if (t < ne) stX<FLOW>(x, vo, ex0, xl[t]);
+        if (t + BS < ne) stX<FLOW>(x, vo, ex1, xl[t + BS]);
+        for (int c = t + 2 * BS; c < ne; c += BS) stX<FLOW>(x, vo, (unsigned)ext[e0 + c], xl[c]);
+        __syncthreads();
+    }
+}
+
 template <int BS, typename VT, typename XT, bool BATCH>
 __global__ void __launch_bounds__(BS) k_kacz_bundle_vl(
     const int* __restrict__ bSlot, const int* __restrict__ bWave, const long* __restrict__ bOff,
@@ -634,7 +874,6 @@ __global__ void __launch_bounds__(BS) k_kacz_bundle_vl(
     const XT* __restrict__ b0, int n0, int nrhs, const int* __restrict__ chainPtr, int chain0, int nchain, int xcd,
     int ldsLocal, int ldsRows, const int* __restrict__ lanePtr, const uint16_t* __restrict__ laneSlot,
     XT* const* __restrict__ bX, const XT* const* __restrict__ bB, const int* __restrict__ bN) {
-    constexpr int E = kVarE;
     extern __shared__ __align__(16) unsigned char smem[];
     XT* xl = reinterpret_cast<XT*>(smem);
     XT* part = xl + ldsLocal;                               // BS partial sums
@@ -644,87 +883,12 @@ __global__ void __launch_bounds__(BS) k_kacz_bundle_vl(
     const int q = bundleOfBlock(nchain, xcd);
     if (q < 0) return;
     const int cLo = chainPtr[chain0 + q], cHi = chainPtr[chain0 + q + 1];
+    const int k0 = (int)blockIdx.y;
+    const int k1 = gridDim.y > 1 ? k0 + 1 : nrhs;
     auto runBundle = [&](const int bnd) {
-        const BundleVec<BATCH, XT> vec(x0, b0, n0, bX, bB, bN, bnd);
-        XT* __restrict__ x = vec.x;
-        const XT* __restrict__ b = vec.b;
-        const int n = vec.n;
-        const int s0 = bSlot[bnd], nb = bSlot[bnd + 1] - s0;
-        const long off = bOff[bnd];
-        const int wb = bLen[bnd];
-        const int nw = bWave[bnd + 1] - bWave[bnd];
-        const int e0 = bExt[bnd], ne = bExt[bnd + 1] - e0;
-        const int lp = lanePtr[bnd], nl = lanePtr[bnd + 1] - lp;
-        const int t = threadIdx.x;
-        const bool active = t < nl;
-        // branch-free prologue, ids first (as in k_gs_bundle_vl)
-        const unsigned ex0 = (unsigned)ext[t < ne ? e0 + t : 0];
-        const unsigned ex1 = (unsigned)ext[t + BS < ne ? e0 + t + BS : 0];
-        const int i = (int)laneSlot[active ? lp + t : 0];  // used by active lanes only
-        const int iPrev = (int)laneSlot[active && t > 0 ? lp + t - 1 : 0];
-        int li[E];
-        typename ValBits<VT>::T lv[E];
-#pragma unroll
-        for (int u = 0; u < E; ++u) {
-            const long e = (active && u < wb) ? off + (long)u * nl + t : zero;
-            li[u] = (int)sidx[e];
-            lv[u] = sval[e];
-        }
-        const int sr = active ? s0 + i : 0;
-        const unsigned row = (unsigned)rowId[sr];
-        const double rn = rowNorm[sr];
-        const int myWave = (int)slotWave[sr];
-        if (active && (t == 0 || iPrev != i)) rl0[i] = t;
-        if (t == 0) rl0[nb] = nl;
-        int rowL0 = 0, rowL1 = 0;
-
-        const int k0 = (int)blockIdx.y;
-        const int k1 = gridDim.y > 1 ? k0 + 1 : nrhs;
-        for (int k = k0; k < k1; ++k) {
-            const size_t vo = (size_t)k * n;
-            const XT g0 = x[vo + (t < ne ? ex0 : 0u)];
-            const XT g1 = x[vo + (t + BS < ne ? ex1 : 0u)];
-            const XT br = b[vo + (active ? row : 0u)];
-            if (t < ne) xl[t] = g0;
-            if (t + BS < ne) xl[t + BS] = g1;
-            for (int c = t + 2 * BS; c < ne; c += BS) xl[c] = x[vo + ext[e0 + c]];
-            __syncthreads();
-            if (k == k0 && active) {
-                rowL0 = rl0[i];
-                rowL1 = rl0[i + 1];
-            }
-            for (int w = 0; w < nw; ++w) {
-                // the lane's x entries, read once in one batch and kept
-                // across the barrier (no other row of this wave writes them)
-                XT cur[E];
-                if (active && myWave == w) {
-#pragma unroll
-                    for (int u = 0; u < E; ++u) cur[u] = xl[li[u]];
-                    XT dot = XT(0);
-#pragma unroll
-                    for (int u = 0; u < E; ++u) dot += valOf<XT>(lv[u]) * cur[u];
-                    part[t] = dot;
-                }
-                __syncthreads();
-                if (active && myWave == w) {
-                    XT rowDot = XT(0);
-                    for (int l = rowL0; l < rowL1; ++l) rowDot += part[l];
-                    const XT scale = (XT)(((double)rowDot - (double)br) / rn);
-                    // nonzero entries only: a padding entry may name a
-                    // column another row updates
-#pragma unroll
-                    for (int u = 0; u < E; ++u) {
-                        const XT a = valOf<XT>(lv[u]);
-                        if (a != XT(0)) xl[li[u]] = cur[u] - scale * a;
-                    }
-                }
-                __syncthreads();
-            }
-            if (t < ne) x[vo + ex0] = xl[t];
-            if (t + BS < ne) x[vo + ex1] = xl[t + BS];
-            for (int c = t + 2 * BS; c < ne; c += BS) x[vo + ext[e0 + c]] = xl[c];
-            __syncthreads();
-        }
+        kaczBundleVL<BS, VT, XT, BATCH, false>(bnd, k0, k1, xl, part, rl0, NoWait{}, bSlot, bWave, bOff, bLen, bExt,
+                                               slotWave, rowId, rowNorm, ext, sidx, sval, zero, x0, b0, n0, lanePtr,
+                                               laneSlot, bX, bB, bN);
     };
     if (cHi == cLo + 1) {
         runBundle(cLo);
@@ -734,6 +898,28 @@ __global__ void __launch_bounds__(BS) k_kacz_bundle_vl(
             __syncthreads();
         }
     }
+}
+
+// Kaczmarz with variable lanes per row, all phases of a sweep in one launch.
+template <int BS, typename VT, typename XT, bool BATCH>
+__global__ void __launch_bounds__(BS) k_kacz_flow_vl(
+    const int* __restrict__ bSlot, const int* __restrict__ bWave, const long* __restrict__ bOff,
+    const int* __restrict__ bLen, const int* __restrict__ bExt, const uint16_t* __restrict__ slotWave,
+    const int* __restrict__ rowId, const double* __restrict__ rowNorm, const int* __restrict__ ext,
+    const uint16_t* __restrict__ sidx, const VT* __restrict__ sval, long zero, XT* x0, const XT* __restrict__ b0,
+    int n0, int ldsLocal, const int* __restrict__ lanePtr, const uint16_t* __restrict__ laneSlot,
+    XT* const* __restrict__ bX, const XT* const* __restrict__ bB, const int* __restrict__ bN,
+    ReadyTables rt) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    XT* xl = reinterpret_cast<XT*>(smem);
+    XT* part = xl + ldsLocal;
+    int* rl0 = reinterpret_cast<int*>(part + BS);
+    const int k = (int)blockIdx.y;
+    readyLoop<BS>(rt, [&](int bnd, auto wait) {
+        kaczBundleVL<BS, VT, XT, BATCH, true>(bnd, k, k + 1, xl, part, rl0, wait, bSlot, bWave, bOff, bLen, bExt,
+                                              slotWave, rowId, rowNorm, ext, sidx, sval, zero, x0, b0, n0, lanePtr,
+                                              laneSlot, bX, bB, bN);
+    });
 }
 
 // (lanes per row, register width) pairs with a uniform kernel
@@ -891,60 +1077,143 @@ void launchPhase(int bs, int tpr, int E, int kind, int nrhs, const BundleDev<VT,
 #undef RACE_BUNDLE_CASE
 }
 
-// Device state of the one-launch sweep of one direction.
+// polls of one dependency flag or queue position before a launch gives up
+// (about seconds; a correct dependency table never gets near it)
+constexpr int kFlowPollLimit = 1 << 22;
+
+// Device state of the one-launch sweep of one direction: the bundle range
+// and the dependencies of every chain, column ids with the coherent-read
+// bit, and per right-hand side a flag per chain (tickets in chain order) or
+// an arrival counter per chain and a ready queue (readiness-driven), and a
+// ticket/epoch state.
 struct FlowDev {
-    DevBuf<int> depPtr, dep, flag, ext;
+    DevBuf<int> depPtr, dep, flag, ext, lo, hi;
     DevBuf<FlowState> fs;
-    int nchain = 0, grid = 0;
+    // readiness-driven sweeps (Kaczmarz): successors, roots, arrival
+    // counters and ready queues
+    DevBuf<int> succPtr, succ, roots;
+    DevBuf<unsigned> arrivals;
+    DevBuf<unsigned long long> queue;
+    bool readyMode = false;
+    int nroots = 0;
+    int nchain = 0, grid = 0, nrhs = 1, maxLocal = 0, maxRows = 0;
     size_t lds = 0;
     // chains whose conflicts lie inside one phase (0 for a valid colouring)
     long samePhase = 0;
+    // nkeys / colBase: the key space of the rows and columns (chainDeps);
+    // kern: the kernel that will run it, for its occupancy
+    // ready: readiness-driven (queue) instead of tickets in chain order
     template <typename VT, typename XT>
-    int init(const BundleDev<VT, XT>& d, int n, int bs) {
+    int init(const BundleDev<VT, XT>& d, long nkeys, const std::vector<long>* colBase, int bs, int nrhs_,
+             const void* kern, bool ready) {
+        readyMode = ready;
         std::vector<int> dp, dd;
-        samePhase = chainDeps(d.h, n, &dp, &dd);
+        samePhase = chainDeps(d.h, nkeys, colBase, &dp, &dd);
         nchain = (int)d.h.chainPtr.size() - 1;
-        for (int p = 0; p < d.h.numPhases(); ++p) lds = std::max(lds, d.lds(p, bs));
+        nrhs = std::max(1, nrhs_);
+        for (int p = 0; p < d.h.numPhases(); ++p) {
+            maxLocal = std::max(maxLocal, d.h.phLocal[p]);
+            maxRows = std::max(maxRows, d.h.phRows[p]);
+        }
+        // one layout for every phase: the largest local column space, then
+        // (variable lanes) the partial sums and the first lane of every row
+        lds = (size_t)maxLocal * sizeof(XT) + 32 +
+              (d.h.tpr == 0 ? (size_t)bs * sizeof(XT) + (size_t)(maxRows + 1) * sizeof(int) : 0);
+        if (lds > kMaxLds) return OK;  // not used (the caller checks lds)
+        // as many workgroups per right-hand side as are resident at once,
+        // all sides together (more would only find the tickets gone)
+        int per = 0, dev = 0, cus = 0;
+        RACE_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, bs, lds));
+        RACE_HIP(hipGetDevice(&dev));
+        RACE_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        const int slots = std::max(1, std::max(per, 1) * cus / nrhs);
+        grid = std::max(1, std::min(nchain, slots));
+        // tickets in chain order (phase order); tables indexed by ticket
+        // position
+        std::vector<int> order(nchain);
+        for (int c = 0; c < nchain; ++c) order[c] = c;
+        std::vector<int> pos(nchain), hlo(nchain), hhi(nchain), qp(nchain + 1, 0), qd;
+        for (int q = 0; q < nchain; ++q) pos[order[q]] = q;
+        qd.reserve(dd.size());
+        for (int q = 0; q < nchain; ++q) {
+            const int c = order[q];
+            hlo[q] = d.h.chainPtr[c];
+            hhi[q] = d.h.chainPtr[c + 1];
+            for (int j = dp[c]; j < dp[c + 1]; ++j) {
+                if (pos[dd[j]] >= q) return ERR_INTERNAL;  // a dependency after its chain
+                qd.push_back(pos[dd[j]]);
+            }
+            qp[q + 1] = (int)qd.size();
+        }
         int e;
-        if ((e = upload(depPtr, dp.data(), dp.size()))) return e;
-        if ((e = upload(dep, dd.data(), dd.size()))) return e;
-        const std::vector<int> fx = flowExt(d.h, n);
+        if ((e = upload(depPtr, qp.data(), qp.size()))) return e;
+        if ((e = upload(dep, qd.data(), qd.size()))) return e;
+        if ((e = upload(lo, hlo.data(), hlo.size()))) return e;
+        if ((e = upload(hi, hhi.data(), hhi.size()))) return e;
+        const std::vector<int> fx = flowExt(d.h, nkeys, colBase);
         if ((e = upload(ext, fx.data(), fx.size()))) return e;
-        const std::vector<int> zeros(std::max(nchain, 1), 0);
+        const std::vector<int> zeros((size_t)std::max(nchain, 1) * nrhs, 0);
         if ((e = upload(flag, zeros.data(), zeros.size()))) return e;
+        if (ready) {
+            std::vector<int> sp(nchain + 1, 0), sv(qd.size()), rv;
+            for (int c = 0; c < nchain; ++c)
+                for (int j = qp[c]; j < qp[c + 1]; ++j) ++sp[qd[j] + 1];
+            for (int c = 0; c < nchain; ++c) sp[c + 1] += sp[c];
+            std::vector<int> fill(sp.begin(), sp.end() - 1);
+            for (int c = 0; c < nchain; ++c)
+                for (int j = qp[c]; j < qp[c + 1]; ++j) sv[fill[qd[j]]++] = c;
+            for (int c = 0; c < nchain; ++c)
+                if (qp[c + 1] == qp[c]) rv.push_back(c);
+            nroots = (int)rv.size();
+            if ((e = upload(succPtr, sp.data(), sp.size()))) return e;
+            if ((e = upload(succ, sv.data(), sv.size()))) return e;
+            if ((e = upload(roots, rv.data(), rv.size()))) return e;
+            const std::vector<unsigned> az((size_t)std::max(nchain, 1) * nrhs, 0u);
+            if ((e = upload(arrivals, az.data(), az.size()))) return e;
+            // queue entries of epoch 0 never match a launch
+            const std::vector<unsigned long long> qz((size_t)std::max(nchain, 1) * nrhs, 0ull);
+            if ((e = upload(queue, qz.data(), qz.size()))) return e;
+        }
         // flags start below the first epoch
         FlowState f0{};
         f0.epoch = 1;
-        if ((e = upload(fs, &f0, 1))) return e;
+        f0.tail = nroots;
+        const std::vector<FlowState> fv(nrhs, f0);
+        if ((e = upload(fs, fv.data(), fv.size()))) return e;
         return OK;
     }
+    ReadyTables tables() const {
+        ReadyTables r;
+        r.lo = lo.p;
+        r.hi = hi.p;
+        r.depPtr = depPtr.p;
+        r.succPtr = succPtr.p;
+        r.succ = succ.p;
+        r.roots = roots.p;
+        r.arrivals = arrivals.p;
+        r.queue = queue.p;
+        r.fs = fs.p;
+        r.nchain = nchain;
+        r.nroots = nroots;
+        r.pollLimit = kFlowPollLimit;
+        return r;
+    }
     int aborts(int* out) const {
-        FlowState f{};
-        RACE_HIP(hipMemcpy(&f, fs.p, sizeof(f), hipMemcpyDeviceToHost));
-        *out = f.aborts;
+        *out = 0;
+        if (!fs.p) return OK;
+        std::vector<FlowState> f(nrhs);
+        RACE_HIP(hipMemcpy(f.data(), fs.p, sizeof(FlowState) * nrhs, hipMemcpyDeviceToHost));
+        for (const FlowState& q : f) *out += q.aborts;
         return OK;
     }
 };
-
-// polls of one dependency flag before a launch gives up (about seconds;
-// a correct dependency table never gets near it)
-constexpr int kFlowPollLimit = 1 << 22;
 
 template <int TPR, int EE, typename VT, typename XT>
 int launchFlow(const BundleDev<VT, XT>& d, FlowDev& f, XT* x, const XT* b, hipStream_t st) {
     constexpr int BS = 256;
     auto kern = k_gs_flow<BS, TPR, EE, VT, XT>;
-    if (f.grid == 0) {
-        // as many workgroups as are resident at once (more would only find
-        // the tickets gone)
-        int per = 0, dev = 0, cus = 0;
-        RACE_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, reinterpret_cast<const void*>(kern), BS, f.lds));
-        RACE_HIP(hipGetDevice(&dev));
-        RACE_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        f.grid = std::max(1, std::min(f.nchain, std::max(per, 1) * cus));
-    }
     hipLaunchKernelGGL(kern, dim3(f.grid), dim3(BS), f.lds, st, d.slot.p, d.wave.p, d.off.p, d.len.p, d.extPtr.p,
-                       d.slotWave.p, d.rowId.p, d.diag.p, f.ext.p, d.sidx.p, d.sval.p, d.zero, x, b, d.chainPtr.p,
+                       d.slotWave.p, d.rowId.p, d.diag.p, f.ext.p, d.sidx.p, d.sval.p, d.zero, x, b, f.lo.p, f.hi.p,
                        f.nchain, f.depPtr.p, f.dep.p, f.flag.p, f.fs.p, kFlowPollLimit);
     RACE_HIP(hipGetLastError());
     return OK;
@@ -962,6 +1231,91 @@ int launchFlowShape(int tpr, int E, const BundleDev<VT, XT>& d, FlowDev& f, XT* 
         case 408: return launchFlow<4, 8>(d, f, x, b, st);
         case 416: return launchFlow<4, 16>(d, f, x, b, st);
         default: return ERR_INTERNAL;
+    }
+}
+
+// Kaczmarz one-launch sweep: grid (resident workgroups per side, nrhs)
+template <int BS, int TPR, int EE, typename VT, typename XT, bool BATCH>
+int launchKaczFlow(const BundleDev<VT, XT>& d, FlowDev& f, XT* x, const XT* b, int n, XT* const* bX,
+                   const XT* const* bB, const int* bN, hipStream_t st) {
+    if constexpr (TPR == 0) {
+        hipLaunchKernelGGL((k_kacz_flow_vl<BS, VT, XT, BATCH>), dim3(f.grid, f.nrhs), dim3(BS), f.lds, st, d.slot.p,
+                           d.wave.p, d.off.p, d.len.p, d.extPtr.p, d.slotWave.p, d.rowId.p, d.rowNorm.p, f.ext.p,
+                           d.sidx.p, d.sval.p, d.zero, x, b, n, f.maxLocal, d.lanePtr.p, d.laneSlot.p, bX, bB, bN,
+                           f.tables());
+    } else {
+        hipLaunchKernelGGL((k_kacz_flow<BS, TPR, EE, VT, XT, BATCH>), dim3(f.grid, f.nrhs), dim3(BS), f.lds, st,
+                           d.slot.p, d.wave.p, d.off.p, d.len.p, d.extPtr.p, d.slotWave.p, d.rowId.p, d.rowNorm.p,
+                           f.ext.p, d.sidx.p, d.sval.p, d.zero, x, b, n, bX, bB, bN, f.tables());
+    }
+    RACE_HIP(hipGetLastError());
+    return OK;
+}
+
+template <typename VT, typename XT, bool BATCH>
+int launchKaczFlowShape(int bs, int tpr, int E, const BundleDev<VT, XT>& d, FlowDev& f, XT* x, const XT* b, int n,
+                        XT* const* bX, const XT* const* bB, const int* bN, hipStream_t st) {
+#define RACE_KFLOW_CASE(BS_)                                                                                       \
+    switch (tpr * 100 + E) {                                                                                       \
+        case 104: return launchKaczFlow<BS_, 1, 4, VT, XT, BATCH>(d, f, x, b, n, bX, bB, bN, st);                  \
+        case 108: return launchKaczFlow<BS_, 1, 8, VT, XT, BATCH>(d, f, x, b, n, bX, bB, bN, st);                  \
+        case 116: return launchKaczFlow<BS_, 1, 16, VT, XT, BATCH>(d, f, x, b, n, bX, bB, bN, st);                 \
+        case 208: return launchKaczFlow<BS_, 2, 8, VT, XT, BATCH>(d, f, x, b, n, bX, bB, bN, st);                  \
+        case 213: return launchKaczFlow<BS_, 2, 13, VT, XT, BATCH>(d, f, x, b, n, bX, bB, bN, st);                 \
+        case 216: return launchKaczFlow<BS_, 2, 16, VT, XT, BATCH>(d, f, x, b, n, bX, bB, bN, st);                 \
+        case 408: return launchKaczFlow<BS_, 4, 8, VT, XT, BATCH>(d, f, x, b, n, bX, bB, bN, st);                  \
+        case 416: return launchKaczFlow<BS_, 4, 16, VT, XT, BATCH>(d, f, x, b, n, bX, bB, bN, st);                 \
+        case 16: return launchKaczFlow<BS_, 0, kVarE, VT, XT, BATCH>(d, f, x, b, n, bX, bB, bN, st);               \
+        default: return ERR_INTERNAL;                                                                              \
+    }
+    switch (bs) {
+        case 256: RACE_KFLOW_CASE(256)
+        case 512: RACE_KFLOW_CASE(512)
+        case 1024: RACE_KFLOW_CASE(1024)
+        default: return ERR_INTERNAL;
+    }
+#undef RACE_KFLOW_CASE
+}
+// workgroup sizes the one-launch Kaczmarz sweep is built for
+inline bool kaczFlowBs(int bs) { return bs == 256 || bs == 512 || bs == 1024; }
+
+// the kernel a one-launch sweep of this shape runs (for its occupancy)
+template <typename VT, typename XT>
+const void* gsFlowKernel(int tpr, int E) {
+    switch (tpr * 100 + E) {
+        case 104: return reinterpret_cast<const void*>(k_gs_flow<256, 1, 4, VT, XT>);
+        case 108: return reinterpret_cast<const void*>(k_gs_flow<256, 1, 8, VT, XT>);
+        case 116: return reinterpret_cast<const void*>(k_gs_flow<256, 1, 16, VT, XT>);
+        case 208: return reinterpret_cast<const void*>(k_gs_flow<256, 2, 8, VT, XT>);
+        case 213: return reinterpret_cast<const void*>(k_gs_flow<256, 2, 13, VT, XT>);
+        case 216: return reinterpret_cast<const void*>(k_gs_flow<256, 2, 16, VT, XT>);
+        case 408: return reinterpret_cast<const void*>(k_gs_flow<256, 4, 8, VT, XT>);
+        case 416: return reinterpret_cast<const void*>(k_gs_flow<256, 4, 16, VT, XT>);
+        default: return nullptr;
+    }
+}
+template <int BS, typename VT, typename XT, bool BATCH>
+const void* kaczFlowKernelBs(int tpr, int E) {
+    switch (tpr * 100 + E) {
+        case 104: return reinterpret_cast<const void*>(k_kacz_flow<BS, 1, 4, VT, XT, BATCH>);
+        case 108: return reinterpret_cast<const void*>(k_kacz_flow<BS, 1, 8, VT, XT, BATCH>);
+        case 116: return reinterpret_cast<const void*>(k_kacz_flow<BS, 1, 16, VT, XT, BATCH>);
+        case 208: return reinterpret_cast<const void*>(k_kacz_flow<BS, 2, 8, VT, XT, BATCH>);
+        case 213: return reinterpret_cast<const void*>(k_kacz_flow<BS, 2, 13, VT, XT, BATCH>);
+        case 216: return reinterpret_cast<const void*>(k_kacz_flow<BS, 2, 16, VT, XT, BATCH>);
+        case 408: return reinterpret_cast<const void*>(k_kacz_flow<BS, 4, 8, VT, XT, BATCH>);
+        case 416: return reinterpret_cast<const void*>(k_kacz_flow<BS, 4, 16, VT, XT, BATCH>);
+        case 16: return reinterpret_cast<const void*>(k_kacz_flow_vl<BS, VT, XT, BATCH>);
+        default: return nullptr;
+    }
+}
+template <typename VT, typename XT, bool BATCH>
+const void* kaczFlowKernel(int bs, int tpr, int E) {
+    switch (bs) {
+        case 256: return kaczFlowKernelBs<256, VT, XT, BATCH>(tpr, E);
+        case 512: return kaczFlowKernelBs<512, VT, XT, BATCH>(tpr, E);
+        case 1024: return kaczFlowKernelBs<1024, VT, XT, BATCH>(tpr, E);
+        default: return nullptr;
     }
 }
 
@@ -992,8 +1346,9 @@ class BundleFn : public DeviceFn {
     Dev df, dr;
     int nrhs = 1, n = 0, bs = 256, tpr = 2, E = 16, xcd = 1;
     std::unique_ptr<DeviceFn> fallback;
-    // one launch per sweep (the "flow" option): Gauss-Seidel, one
-    // right-hand side, 256-thread workgroups, uniform lanes per row
+    // one launch per sweep (the "flow" option): Gauss-Seidel with one
+    // right-hand side, 256-thread workgroups and uniform lanes per row;
+    // Kaczmarz with 256-1024-thread workgroups
     bool flow = false;
     FlowDev ff, fr;
 
@@ -1028,11 +1383,21 @@ class BundleFn : public DeviceFn {
             if (df.lds(p, bs) > kMaxLds) return fallBack();
         for (int p = 0; p < dr.h.numPhases(); ++p)
             if (dr.lds(p, bs) > kMaxLds) return fallBack();
-        if (ctx.flow && kind == RACE_K_GS && nrhs == 1 && bs == 256 && tpr > 0) {
-            if ((e = ff.init(df, n, bs)) || (e = fr.init(dr, n, bs))) return e;
-            // conflicts inside a phase would serialise its chains: keep the
-            // phase launches then
-            flow = ff.samePhase == 0 && fr.samePhase == 0;
+        const bool gsFlow = kind == RACE_K_GS && nrhs == 1 && bs == 256 && tpr > 0;
+        const bool kaczFlow = kind == RACE_K_KACZ && kaczFlowBs(bs);
+        if (ctx.flow && (gsFlow || kaczFlow)) {
+            const int sides = kind == RACE_K_KACZ ? nrhs : 1;
+            const void* kern =
+                kind == RACE_K_KACZ ? kaczFlowKernel<VT, XT, false>(bs, tpr, E) : gsFlowKernel<VT, XT>(tpr, E);
+            if (!kern) return ERR_INTERNAL;
+            const bool ready = kind == RACE_K_KACZ;
+            if ((e = ff.init(df, n, nullptr, bs, sides, kern, ready)) ||
+                (e = fr.init(dr, n, nullptr, bs, sides, kern, ready)))
+                return e;
+            // conflicts inside a phase would serialise its chains, and one
+            // LDS layout serves every phase: keep the phase launches if
+            // either does not hold
+            flow = ff.samePhase == 0 && fr.samePhase == 0 && ff.lds <= kMaxLds && fr.lds <= kMaxLds;
         }
         return OK;
     }
@@ -1065,6 +1430,9 @@ class BundleFn : public DeviceFn {
             return fallback->run(rev, st);
         }
         const Dev& d = rev ? dr : df;
+        if (flow && kind == RACE_K_KACZ)
+            return launchKaczFlowShape<VT, XT, false>(bs, tpr, E, d, rev ? fr : ff, (XT*)x, (const XT*)b, n, nullptr,
+                                                      nullptr, nullptr, st);
         if (flow) return launchFlowShape<VT, XT>(tpr, E, d, rev ? fr : ff, (XT*)x, (const XT*)b, st);
         const std::vector<int>& pc = d.h.phaseChain;
         for (int p = 0; p + 1 < (int)pc.size(); ++p) {
@@ -1095,10 +1463,16 @@ class BundleBatchFn : public DeviceFn {
         DevBuf<XT*> bx[2];
         DevBuf<const XT*> bb[2];
         DevBuf<int> bn[2];
+        FlowDev flow[2];
     };
     std::vector<std::unique_ptr<Group>> groups;
     int nrhs = 1, bs = 256, xcd = 0, maxPhases[2] = {0, 0};
     long critical[2] = {0, 0};
+    // one launch per group and sweep (the "flow" option, Kaczmarz): every
+    // matrix advances through its own phases, waiting only for its own
+    // earlier chains, instead of every fused phase waiting for the slowest
+    // matrix
+    bool flow = false, flowOk = true;
 
     int init(const std::vector<BatchItem>& items, const ExecContext& ctx) {
         if (items.empty()) return ERR_INVALID_ARG;
@@ -1171,9 +1545,24 @@ class BundleBatchFn : public DeviceFn {
                 for (int p = 0; p < grp->dev[dir].h.numPhases(); ++p)
                     if (grp->dev[dir].lds(p, bs) > kMaxLds) return ERR_INCOMPATIBILITY;
                 maxPhases[dir] = std::max(maxPhases[dir], grp->dev[dir].h.numPhases());
+                if (ctx.flow && kind == RACE_K_KACZ && kaczFlowBs(bs)) {
+                    // every matrix its own range of row / column keys
+                    std::vector<long> base(members.size() + 1, 0);
+                    for (size_t q = 0; q < members.size(); ++q) base[q + 1] = base[q] + items[members[q]].d.nrow;
+                    std::vector<long> colBase(owner.size());
+                    for (size_t k = 0; k < owner.size(); ++k) colBase[k] = base[owner[k]];
+                    const void* kern = kaczFlowKernel<VT, XT, true>(bs, grp->tpr, grp->E);
+                    if (!kern) return ERR_INTERNAL;
+                    if ((e = grp->flow[dir].init(grp->dev[dir], base.back(), &colBase, bs, nrhs, kern, true)))
+                        return e;
+                    // conflicts inside a phase, or one LDS layout for all
+                    // phases too large: the fused phase launches then
+                    if (grp->flow[dir].samePhase != 0 || grp->flow[dir].lds > kMaxLds) flowOk = false;
+                }
             }
             groups.push_back(std::move(grp));
         }
+        flow = flowOk && ctx.flow && kind == RACE_K_KACZ && kaczFlowBs(bs);
         for (int dir = 0; dir < 2; ++dir)
             for (int p = 0; p < maxPhases[dir]; ++p) {
                 int w = 0;
@@ -1184,8 +1573,36 @@ class BundleBatchFn : public DeviceFn {
         return OK;
     }
     int waves(bool rev) const override { return (int)critical[rev ? 1 : 0]; }
+    int launches(bool rev) const override {
+        if (flow) return (int)groups.size();
+        int k = 0;
+        const int dir = rev ? 1 : 0;
+        for (int p = 0; p < maxPhases[dir]; ++p)
+            for (const auto& g : groups)
+                if (p < g->dev[dir].h.numPhases() && g->dev[dir].h.phaseChain[p + 1] > g->dev[dir].h.phaseChain[p]) ++k;
+        return k;
+    }
+    int flowAborts(int* out) const override {
+        *out = 0;
+        if (!flow) return OK;
+        for (const auto& g : groups)
+            for (int dir = 0; dir < 2; ++dir) {
+                int a = 0, e;
+                if ((e = g->flow[dir].aborts(&a))) return e;
+                *out += a;
+            }
+        return OK;
+    }
     int run(bool rev, hipStream_t st) override {
         const int dir = rev ? 1 : 0;
+        if (flow) {
+            for (const auto& g : groups) {
+                int e = launchKaczFlowShape<VT, XT, true>(bs, g->tpr, g->E, g->dev[dir], g->flow[dir], nullptr,
+                                                          nullptr, 0, g->bx[dir].p, g->bb[dir].p, g->bn[dir].p, st);
+                if (e) return e;
+            }
+            return OK;
+        }
         // phase p of every group before phase p + 1 of any (stream order)
         for (int p = 0; p < maxPhases[dir]; ++p)
             for (const auto& g : groups) {

@@ -1193,6 +1193,9 @@ int race_batch_set_option(race_batch b, const char* key, int value) {
     } else if (k == "sell_sigma") {
         if (value < 1) return ERR_INVALID_ARG;
         b->ctx.sellSigma = value;
+    } else if (k == "flow") {
+        if (value != 0 && value != 1) return ERR_INVALID_ARG;
+        b->ctx.flow = value;
     } else {
         return ERR_INVALID_ARG;
     }
@@ -1209,7 +1212,20 @@ int race_batch_get_option(race_batch b, const char* key, int* value) {
     else if (k == "bundle_local") *value = b->ctx.bundleLocal;
     else if (k == "sell_c") *value = b->ctx.sellC;
     else if (k == "sell_sigma") *value = b->ctx.sellSigma;
+    else if (k == "flow") *value = b->ctx.flow;
     else return ERR_INVALID_ARG;
+    return OK;
+}
+
+int race_batch_get_flow_aborts(race_batch b, int* aborts) {
+    if (!b || !aborts || !b->fn) return ERR_INVALID_ARG;
+    RACE_HIP(hipDeviceSynchronize());
+    return b->fn->flowAborts(aborts);
+}
+
+int race_batch_get_launch_count(race_batch b, int rev, int* launches) {
+    if (!b || !launches || !b->fn) return ERR_INVALID_ARG;
+    *launches = b->fn->launches(rev != 0);
     return OK;
 }
 

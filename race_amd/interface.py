@@ -434,6 +434,18 @@ class Batch:
         L.check(self._L.race_batch_get_option(self.h, key.encode(), ctypes.byref(v)), "race_batch_get_option")
         return v.value
 
+    def launchCount(self, rev=False):
+        """kernel launches of one execution of the built batch"""
+        v = ctypes.c_int()
+        L.check(self._L.race_batch_get_launch_count(self.h, int(rev), ctypes.byref(v)), "race_batch_get_launch_count")
+        return v.value
+
+    def flowAborts(self):
+        """one-launch sweeps whose dependency waits gave up (0 when consistent)"""
+        v = ctypes.c_int()
+        L.check(self._L.race_batch_get_flow_aborts(self.h, ctypes.byref(v)), "race_batch_get_flow_aborts")
+        return v.value
+
     def build(self):
         L.check(self._L.race_batch_build(self.h), "race_batch_build")
 
@@ -451,12 +463,35 @@ class Batch:
     def useCandidate(self, i):
         L.check(self._L.race_batch_use_candidate(self.h, int(i)), "race_batch_use_candidate")
 
-    def autotune(self, rev=False, reps=3):
-        best = ctypes.c_int()
-        t = (ctypes.c_float * 12)()
-        L.check(self._L.race_batch_autotune(self.h, int(bool(rev)), int(reps), ctypes.byref(best), t, 12),
-                "race_batch_autotune")
-        return best.value, [float(v) for v in t]
+    def autotune(self, rev=False, reps=3, progress=None):
+        """Time the 12 candidate layouts, keep the fastest; returns (index,
+        times, -1 for a layout these matrices cannot use).  With progress, the
+        candidates are timed one call at a time (race_batch_autotune's loop)
+        and progress(i, ms) is called after each."""
+        if progress is None:
+            best = ctypes.c_int()
+            t = (ctypes.c_float * 12)()
+            L.check(self._L.race_batch_autotune(self.h, int(bool(rev)), int(reps), ctypes.byref(best), t, 12),
+                    "race_batch_autotune")
+            return best.value, [float(v) for v in t]
+        times = []
+        for i in range(12):
+            self.useCandidate(i)
+            st = self._L.race_batch_build(self.h)
+            ms = ctypes.c_float(-1.0)
+            if st == L.SUCCESS:
+                st = self._L.race_batch_time(self.h, int(bool(rev)), int(reps), ctypes.byref(ms))
+            if st == 9:  # RACE_ERR_INTERNAL
+                L.check(st, "race_batch_autotune")
+            times.append(float(ms.value) if st == L.SUCCESS else -1.0)
+            progress(i, times[-1])
+        ok = [i for i in range(12) if times[i] >= 0]
+        if not ok:
+            L.check(6, "race_batch_autotune")
+        best = min(ok, key=lambda i: times[i])
+        self.useCandidate(best)
+        self.build()
+        return best, times
 
     @staticmethod
     def candidate(kind, i):
