@@ -532,10 +532,14 @@ def main():
     try:
         from race_amd import tuner as TU
         t0 = time.time()
-        rpt = TU.RACE_ROWS_PER_THREAD[int(TU.Tuner.pretrained("race").predict(TU.features(rp, col))[0])]
+        feats = TU.features(rp, col)
+        rpt = TU.RACE_ROWS_PER_THREAD[int(TU.Tuner.pretrained("race").predict(feats)[0])]
+        eps = TU.EPS_VALUES[int(TU.Tuner.pretrained("eps").predict(feats)[0])]
         race_tuner = {"proposed_rows_per_thread": rpt, "proposed_threads": max(1, n // rpt),
-                      "seconds": round(time.time() - t0, 2), "threads_used": args.threads}
-        log(f"tuner proposes {rpt} rows per RACE thread ({n // rpt} threads); measured best 16 (131072)")
+                      "proposed_efficiency": eps, "seconds": round(time.time() - t0, 2),
+                      "threads_used": args.threads, "efficiency_used": 40}
+        log(f"tuner proposes {rpt} rows per RACE thread ({n // rpt} threads), efficiency {eps}; "
+            f"measured best 16 (131072) at efficiency 40")
     except OSError:
         pass
 

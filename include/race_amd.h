@@ -400,6 +400,15 @@ int race_batch_destroy(race_batch b);
  * y/t/mask: batch x 12 (host arrays). */
 typedef struct race_mlp_s *race_mlp;
 int race_mlp_dims(int *n_in, int *n_hidden, int *n_out, int *n_params);
+/* The model's 16 input features of a square pattern (host only, no GPU):
+ * log10 n, log10 nnz, mean row length, its coefficient of variation, log10
+ * longest row, shortest row, share of rows longer than twice the mean and
+ * than 32, mean and largest |col - row| / n, share of entries within 64 of
+ * the diagonal, log10 of the number of levels of a breadth-first search
+ * from row 0 (parts restarted at their smallest row), largest level / n,
+ * mean / largest level, Gini coefficient of the row lengths, log10(nnz /
+ * levels). */
+int race_matrix_features(int n, const int *rowPtr, const int *col, double *out16);
 int race_mlp_create(race_mlp *out, unsigned seed);
 int race_mlp_set_params(race_mlp m, const float *params, int count);
 int race_mlp_get_params(race_mlp m, float *params, int count);

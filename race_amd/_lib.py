@@ -142,6 +142,7 @@ PROTOTYPES = {
     "race_device_copy_async": (ctypes.c_int, [vp, vp, ctypes.c_size_t, vp]),
     "race_get_stream": (ctypes.c_int, [vp, ctypes.POINTER(vp)]),
     "race_mlp_dims": (ctypes.c_int, [i32p, i32p, i32p, i32p]),
+    "race_matrix_features": (ctypes.c_int, [ctypes.c_int, i32p, i32p, f64p]),
     "race_mlp_create": (ctypes.c_int, [ctypes.POINTER(vp), ctypes.c_uint]),
     "race_mlp_set_params": (ctypes.c_int, [vp, f32p, ctypes.c_int]),
     "race_mlp_get_params": (ctypes.c_int, [vp, f32p, ctypes.c_int]),

@@ -15,7 +15,7 @@ OBJ = os.path.join(HERE, "_obj")
 LIB = os.path.join(HERE, "librace_amd.so")
 ARCH = os.environ.get("RACE_AMD_ARCH", "gfx950")
 
-SOURCES = ["engine.cpp", "bundle.cpp", "simdify.cpp", "bfs.hip", "exec.hip", "bundle_exec.hip", "sell.hip", "refine.hip", "mlp.hip", "schedule_io.cpp", "rcm.cpp", "capi.cpp"]
+SOURCES = ["engine.cpp", "bundle.cpp", "simdify.cpp", "bfs.hip", "exec.hip", "bundle_exec.hip", "sell.hip", "refine.hip", "mlp.hip", "schedule_io.cpp", "rcm.cpp", "features.cpp", "capi.cpp"]
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 

@@ -14,7 +14,8 @@ budgets MPK_CACHE_MB handed to RACEColor(4, budget): the MPK block depth
 For RACE's own preprocessing (kind "race") the candidates are the rows per
 RACE thread RACE_ROWS_PER_THREAD of a distance-1 colouring for a symmetric
 Gauss-Seidel step: the thread count sets the recursion depth (stages,
-colour phases) and the leaf sizes (dependency waves).
+colour phases) and the leaf sizes (dependency waves).  For RACE's efficiency
+threshold (kind "eps") the candidates are EPS_VALUES at n // 64 threads.
 
     python -m race_amd.tuner sweep  --kind symmspmv --count 1200 --out data.npz
     python -m race_amd.tuner train  --data data.npz --out model.npz
@@ -34,7 +35,7 @@ from .interface import (BF16, FP32, FP64, K_GS, K_KACZ, K_MPK, K_SYMMSPMV, ONE, 
                         Interface)
 
 N_FEATURES, N_CANDIDATES = 16, 12
-KINDS = {"symmspmv": K_SYMMSPMV, "gs": K_GS, "kacz": K_KACZ, "mpk": K_MPK, "race": -1}
+KINDS = {"symmspmv": K_SYMMSPMV, "gs": K_GS, "kacz": K_KACZ, "mpk": K_MPK, "race": -1, "eps": -2}
 # MPK candidates: cache budgets in MB; the last is the Infinity Cache size,
 # the fixed choice the tuner is compared with
 MPK_CACHE_MB = [0.125, 0.25, 0.5, 1, 2, 4, 8, 16, 32, 64, 128, 256]
@@ -42,8 +43,13 @@ MPK_POWER = 4
 # RACE threads = max(1, n // rows per thread); the fixed choice compared
 # with is 256 rows per thread (the batched workload's rule)
 RACE_ROWS_PER_THREAD = [2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096]
+# RACE's efficiency threshold (RACE_EFFICIENCY, read by LevelRecursion at
+# each stage, src/level_recursion.cpp:48,62-70) for a distance-1 colouring
+# with n // EPS_ROWS_PER_THREAD threads; the fixed choice is RACE's default 40
+EPS_VALUES = [5, 10, 20, 30, 40, 50, 60, 70, 80, 85, 90, 95]
+EPS_ROWS_PER_THREAD = 64
 DEFAULT_CANDIDATE = {"symmspmv": 0, "gs": 0, "kacz": 0, "mpk": len(MPK_CACHE_MB) - 1,
-                     "race": RACE_ROWS_PER_THREAD.index(256)}
+                     "race": RACE_ROWS_PER_THREAD.index(256), "eps": EPS_VALUES.index(40)}
 
 
 def _fp(a):
@@ -106,62 +112,37 @@ class MLP:
 
 # ---------------------------------------------------------------------------
 # features
-def bfs_levels(rowPtr, col):
-    """Level sizes of a breadth-first search from row 0 (islands restarted at
-    their smallest row), as numpy frontier expansion."""
-    n = len(rowPtr) - 1
-    rp = np.asarray(rowPtr, np.int64)
-    c = np.asarray(col, np.int64)
-    level = np.full(n, -1, np.int64)
-    sizes = []
-    nxt = 0
-    while True:
-        rest = np.nonzero(level < 0)[0]
-        if rest.size == 0:
-            break
-        front = rest[:1]
-        level[front] = nxt
-        while front.size:
-            sizes.append(front.size)
-            starts, lens = rp[front], rp[front + 1] - rp[front]
-            excl = np.cumsum(lens) - lens  # first position of each row's entries
-            idx = np.arange(lens.sum()) + np.repeat(starts - excl, lens)
-            nb = np.unique(c[idx])
-            nb = nb[level[nb] < 0]
-            nxt += 1
-            level[nb] = nxt
-            front = nb
-    return np.array(sizes, np.int64)
-
-
 def features(rowPtr, col):
-    """16 structural features of a square sparse pattern (CRS)."""
-    rp = np.asarray(rowPtr, np.int64)
-    c = np.asarray(col, np.int64)
-    n = len(rp) - 1
-    nnz = int(rp[-1])
-    rl = np.diff(rp).astype(np.float64)
-    mean = rl.mean()
-    r = np.repeat(np.arange(n), np.diff(rp))
-    dist = np.abs(c - r).astype(np.float64)
-    lv = bfs_levels(rp, c)
-    srt = np.sort(rl)
-    gini = 1.0 - 2.0 * np.sum(np.cumsum(srt)) / (n * srt.sum()) + 1.0 / n
-    return np.array([
-        np.log10(n), np.log10(max(nnz, 1)), mean, rl.std() / mean, np.log10(rl.max()), rl.min(),
-        np.mean(rl > 2 * mean), np.mean(rl > 32), dist.mean() / n, dist.max() / n, np.mean(dist <= 64),
-        np.log10(len(lv)), lv.max() / n, lv.mean() / lv.max(), gini, np.log10(nnz / len(lv)),
-    ], np.float64)
+    """16 structural features of a square sparse pattern (CRS), computed on
+    the host by the library (race_matrix_features): size, row-length
+    moments, column offsets, breadth-first level profile."""
+    rp = np.ascontiguousarray(rowPtr, np.int32)
+    c = np.ascontiguousarray(col, np.int32)
+    out = np.empty(N_FEATURES, np.float64)
+    L.check(L.load().race_matrix_features(len(rp) - 1, rp.ctypes.data_as(L.i32p), c.ctypes.data_as(L.i32p),
+                                          out.ctypes.data_as(L.f64p)), "race_matrix_features")
+    return out
 
 
 # ---------------------------------------------------------------------------
 # sweeps (training data): every candidate timed on one matrix
-def tuner_matrices(count, seed, n_lo=2000, n_hi=100000):
-    """The batched workload's generators plus 3D 7- and 27-point stencils."""
+def tuner_matrices(count, seed, n_lo=2000, n_hi=100000, big=0):
+    """The batched workload's generators plus 3D 7- and 27-point stencils;
+    with big > 0, every (count // big)-th matrix instead a large stencil of
+    1-4.1M rows (7-point 100^3 .. 27-point 160^3)."""
     rng = np.random.default_rng(seed)
     gen = M.batch_matrices(count, seed + 1, n_lo, n_hi)
+    every = count // big if big else 0
     for k in range(count):
-        if k % 8 == 7:
+        if every and k % every == every - 1:
+            q = k // every
+            g = int(100 + 60 * q / max(1, big - 1)) if big > 1 else 160
+            seven = q % 2 == 0
+            rp, col = (M.stencil7 if seven else M.stencil27)(g)
+            val = M.gs_values(rp, col, seed=k)
+            next(gen)
+            yield ("stencil7" if seven else "stencil27"), rp, col, val
+        elif k % 8 == 7:
             g = int(rng.integers(14, 48))
             rp, col = (M.stencil7 if k % 16 == 7 else M.stencil27)(g)
             val = M.gs_values(rp, col, seed=k)
@@ -235,6 +216,46 @@ def sweep_race(rp, col, val, reps=3, stats=None):
     return f, times
 
 
+def sweep_eps(rp, col, val, reps=3, stats=None):
+    """(features, times[12]) of a symmetric Gauss-Seidel step (forward +
+    backward sweep, FP64, default layout) for every efficiency threshold
+    EPS_VALUES (environment variable RACE_EFFICIENCY, as RACE reads it) at
+    n // EPS_ROWS_PER_THREAD threads, colouring at distance 1."""
+    n = len(rp) - 1
+    b = M.uniform_vec(n, 8)
+    times = np.full(N_CANDIDATES, -1.0)
+    saved = os.environ.get("RACE_EFFICIENCY")
+    t0 = time.time()
+    try:
+        for i, eps in enumerate(EPS_VALUES):
+            os.environ["RACE_EFFICIENCY"] = str(eps)
+            ce = Interface(n, max(1, n // EPS_ROWS_PER_THREAD), ONE, rp, col, symm_hint=True)
+            ce.RACEColor()
+            prp, pcol, pval = M.permute(rp, col, val, ce.getPerm(), ce.getInvPerm())
+            prp, pcol, pval = M.make_diag_first(prp, pcol, pval)
+            x, bd = DeviceArray.from_host(np.zeros(n)), DeviceArray.from_host(b)
+            fid = ce.registerDevice(K_GS, prp, pcol, pval, x, bd)
+            ce.executeAsync(fid)
+            ce.executeAsync(fid, True)
+            ce.timerBegin()
+            for _ in range(reps):
+                ce.executeAsync(fid)
+                ce.executeAsync(fid, True)
+            times[i] = ce.timerEnd() / reps
+            ce.close()
+    finally:
+        if saved is None:
+            os.environ.pop("RACE_EFFICIENCY", None)
+        else:
+            os.environ["RACE_EFFICIENCY"] = saved
+    t1 = time.time()
+    f = features(rp, col)
+    if stats is not None:
+        stats["autotune"] = t1 - t0
+        stats["features"] = time.time() - t1
+    return f, times
+
+
 def sweep_one(kind, rp, col, val, reps=3, stats=None):
     """(features, times[12]) of one matrix; times -1 where a layout does not
     fit.  Matrix coloured by RACE with max(4, n / 256) threads.  stats, if
@@ -244,6 +265,8 @@ def sweep_one(kind, rp, col, val, reps=3, stats=None):
         return sweep_mpk(rp, col, val, reps, stats)
     if kind == -1:
         return sweep_race(rp, col, val, reps, stats)
+    if kind == -2:
+        return sweep_eps(rp, col, val, reps, stats)
     n = len(rp) - 1
     T = max(4, min(1024, n // 256))
     dist = ONE if kind == K_GS else TWO
@@ -313,7 +336,7 @@ class Tuner:
     @classmethod
     def pretrained(cls, kind):
         """The model shipped in race_amd/tuner_data for "symmspmv", "gs",
-        "mpk" or "race" (trained on the sweeps stored beside it)."""
+        "mpk", "race" or "eps" (trained on the sweeps stored beside it)."""
         path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuner_data", f"{kind}_model.npz")
         return cls(0).load(path)
 
@@ -345,11 +368,21 @@ def _main(argv=None):
     s.add_argument("--count", type=int, default=1200)
     s.add_argument("--seed", type=int, default=11)
     s.add_argument("--n-hi", type=int, default=100000)
+    s.add_argument("--big", type=int, default=0, help="large stencils (1-4.1M rows) among the count matrices")
+    s.add_argument("--start", type=int, default=0, help="sweep only matrices [start, stop) of the sequence")
+    s.add_argument("--stop", type=int, default=None)
     s.add_argument("--out", required=True)
+    mg = sub.add_parser("merge")
+    mg.add_argument("--parts", nargs="+", required=True)
+    mg.add_argument("--out", required=True)
     t = sub.add_parser("train")
     t.add_argument("--data", required=True)
     t.add_argument("--holdout", type=int, default=200)
     t.add_argument("--epochs", type=int, default=3000)
+    t.add_argument("--epochs-grid", default="",
+                   help="comma-separated epoch counts: pick the one scoring best on the last --val training "
+                        "matrices (trained on the rest), then train on all training matrices with it")
+    t.add_argument("--val", type=int, default=0)
     t.add_argument("--penalty", type=float, default=None,
                    help="target for candidates that do not fit (default: masked)")
     t.add_argument("--out", required=True)
@@ -362,7 +395,10 @@ def _main(argv=None):
         kind = KINDS[a.kind]
         F, T, names, secs, fsecs = [], [], [], [], []
         t0 = time.time()
-        for k, (name, rp, col, val) in enumerate(tuner_matrices(a.count, a.seed, n_hi=a.n_hi)):
+        stop = a.count if a.stop is None else a.stop
+        for k, (name, rp, col, val) in enumerate(tuner_matrices(a.count, a.seed, n_hi=a.n_hi, big=a.big)):
+            if k < a.start or k >= stop:
+                continue
             st = {}
             f, tm = sweep_one(kind, rp, col, val, stats=st)
             secs.append(st["autotune"])
@@ -370,17 +406,36 @@ def _main(argv=None):
             F.append(f)
             T.append(tm)
             names.append(name)
-            if (k + 1) % 50 == 0:
+            if (k + 1) % 25 == 0 or len(rp) > 1000000:
                 print(f"sweep {a.kind}: {k + 1}/{a.count} ({time.time() - t0:.0f}s)", flush=True)
         np.savez(a.out, features=np.array(F), times=np.array(T), names=np.array(names), kind=a.kind,
                  sweep_seconds=np.array(secs), feature_seconds=np.array(fsecs))
+    elif a.cmd == "merge":
+        zs = [np.load(p) for p in a.parts]
+        np.savez(a.out, **{k: np.concatenate([z[k] for z in zs]) for k in
+                           ("features", "times", "names", "sweep_seconds", "feature_seconds")},
+                 kind=zs[0]["kind"])
     elif a.cmd == "train":
         z = np.load(a.data)
         F, T = z["features"], z["times"]
         tr = slice(0, len(F) - a.holdout)
+        epochs = a.epochs
+        if a.epochs_grid and a.val > 0:
+            ntr = len(F) - a.holdout
+            fit_part, val_part = slice(0, ntr - a.val), slice(ntr - a.val, ntr)
+            best = None
+            for ep in (int(v) for v in a.epochs_grid.split(",")):
+                tv = Tuner(0)
+                tv.fit(F[fit_part], T[fit_part], epochs=ep, penalty=a.penalty)
+                sc = score(T[val_part], tv.predict(F[val_part]))
+                print(f"epochs {ep}: validation score {sc:.4f} on {a.val} matrices", flush=True)
+                if best is None or sc > best[0]:
+                    best = (sc, ep)
+            epochs = best[1]
+            print(f"chosen epochs {epochs}", flush=True)
         tu = Tuner(0)
         t0 = time.time()
-        loss = tu.fit(F[tr], T[tr], epochs=a.epochs, penalty=a.penalty)
+        loss = tu.fit(F[tr], T[tr], epochs=epochs, penalty=a.penalty)
         print(f"trained on {len(F) - a.holdout} matrices, loss {loss:.4f}, {time.time() - t0:.1f}s", flush=True)
         tu.save(a.out)
     else:

@@ -43,6 +43,66 @@ def test_oracle_mlp_forward_closed_form():
     assert np.allclose(O.mlp_forward(p, X), np.maximum(X, 0)[:, :12])
 
 
+def _bfs_levels_reference(rowPtr, col):
+    """Level sizes of a breadth-first search from row 0 (islands restarted at
+    their smallest row), as numpy frontier expansion."""
+    n = len(rowPtr) - 1
+    rp = np.asarray(rowPtr, np.int64)
+    c = np.asarray(col, np.int64)
+    level = np.full(n, -1, np.int64)
+    sizes = []
+    nxt = 0
+    while True:
+        rest = np.nonzero(level < 0)[0]
+        if rest.size == 0:
+            break
+        front = rest[:1]
+        level[front] = nxt
+        while front.size:
+            sizes.append(front.size)
+            starts, lens = rp[front], rp[front + 1] - rp[front]
+            excl = np.cumsum(lens) - lens
+            idx = np.arange(lens.sum()) + np.repeat(starts - excl, lens)
+            nb = np.unique(c[idx])
+            nb = nb[level[nb] < 0]
+            nxt += 1
+            level[nb] = nxt
+            front = nb
+    return np.array(sizes, np.int64)
+
+
+def _features_reference(rowPtr, col):
+    """The 16 features written out in numpy (the library computes them in C++)."""
+    rp = np.asarray(rowPtr, np.int64)
+    c = np.asarray(col, np.int64)
+    n = len(rp) - 1
+    nnz = int(rp[-1])
+    rl = np.diff(rp).astype(np.float64)
+    mean = rl.mean()
+    r = np.repeat(np.arange(n), np.diff(rp))
+    dist = np.abs(c - r).astype(np.float64)
+    lv = _bfs_levels_reference(rp, c)
+    srt = np.sort(rl)
+    gini = 1.0 - 2.0 * np.sum(np.cumsum(srt)) / (n * srt.sum()) + 1.0 / n
+    return np.array([
+        np.log10(n), np.log10(max(nnz, 1)), mean, rl.std() / mean, np.log10(rl.max()), rl.min(),
+        np.mean(rl > 2 * mean), np.mean(rl > 32), dist.mean() / n, dist.max() / n, np.mean(dist <= 64),
+        np.log10(len(lv)), lv.max() / n, lv.mean() / lv.max(), gini, np.log10(nnz / len(lv)),
+    ], np.float64)
+
+
+def test_library_features_equal_numpy_reference():
+    """race_matrix_features against the numpy restatement on the tuner's own
+    matrix families (banded, Chung-Lu with hubs, stencils, islands)."""
+    mats = [(rp, col) for _, rp, col, _ in TU.tuner_matrices(24, 5, n_lo=300, n_hi=4000)]
+    rp1, c1 = M.stencil27(4)
+    n1 = len(rp1) - 1
+    mats.append((np.concatenate([rp1, rp1[1:] + rp1[-1]]).astype(np.int32),
+                 np.concatenate([c1, c1 + n1]).astype(np.int32)))
+    for rp, col in mats:
+        assert np.allclose(TU.features(rp, col), _features_reference(rp, col), rtol=1e-10, atol=1e-12)
+
+
 def test_features_closed_form_7pt():
     g = 10
     rp, col = M.stencil7(g)
@@ -68,8 +128,10 @@ def test_features_levels_with_islands():
     n1 = len(rp1) - 1
     rp = np.concatenate([rp1, rp1[1:] + rp1[-1]]).astype(np.int32)
     col = np.concatenate([c1, c1 + n1]).astype(np.int32)
-    lv = TU.bfs_levels(rp, col)
+    lv = _bfs_levels_reference(rp, col)
     assert lv.tolist() == [1, 7, 19, 37] * 2  # 3L^2 + 3L + 1
+    f = TU.features(rp, col)
+    assert np.isclose(f[11], np.log10(8)) and np.isclose(f[12], 37 / (2 * n1))
 
 
 def test_targets():
