@@ -172,6 +172,9 @@ class Interface {
         commSimpleFns_.push_back(std::make_unique<CommSimpleThunk>(CommSimpleThunk{std::move(f), args}));
         return race_attach_communication_simple(h_, funcId, &CommSimpleThunk::call, commSimpleFns_.back().get());
     }
+    // engine options of race_set_option ("layout", bundle shape, "graph",
+    // "flow", ...), set before registering (no reference counterpart)
+    int setOption(const char* key, int value) { return race_set_option(h_, key, value); }
     // GPU kernels: the typed device registration (no reference counterpart)
     int registerDevice(const race_kernel_desc& d) {
         int id = -1;

@@ -272,6 +272,7 @@ int main() {
     ce.executeFunction(hid, true);
 
     double *dx, *db;
+    const std::vector<double> xstart(xg.begin(), xg.begin() + n);
     if (hipMalloc(&dx, n * sizeof(double)) != hipSuccess) return 3;
     if (hipMalloc(&db, n * sizeof(double)) != hipSuccess) return 3;
     if (hipMemcpy(dx, xg.data(), n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) return 3;
@@ -299,6 +300,25 @@ int main() {
     std::printf("threads=%d eff=%.3f relerr=%.3e\n", ce.getNumThreads(), ce.getEfficiency(),
                 std::sqrt(num / den));
     const bool colorOk = std::sqrt(num / den) <= 1e-12;
+    // the same device sweeps, one launch per sweep (engine option "flow"):
+    // bit for bit the phase launches
+    {
+        CHECK(ce.setOption("flow", 1) == RACE_SUCCESS);
+        double* dx2;
+        if (hipMalloc(&dx2, n * sizeof(double)) != hipSuccess) return 3;
+        if (hipMemcpy(dx2, xstart.data(), n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) return 3;
+        race_kernel_desc d2 = d;
+        d2.x = dx2;
+        const int fid = ce.registerDevice(d2);
+        CHECK(fid >= 0);
+        ce.executeFunction(fid);
+        ce.executeFunction(fid, true);
+        std::vector<double> xf(n);
+        if (hipMemcpy(xf.data(), dx2, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) return 3;
+        for (int r = 0; r < n; ++r) CHECK(xf[r] == xg[r]);
+        CHECK(ce.setOption("flow", 0) == RACE_SUCCESS);
+        (void)hipFree(dx2);
+    }
     // the same sweeps on one thread (setSerial) give the same x
     std::vector<double> xs(n, 0.0);
     GsArgs gs1{&P, xs.data(), b.data()};
