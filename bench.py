@@ -311,6 +311,9 @@ def bench_batched(args, R, M):
     singles = []
     cpu_sample = []
     kacz_x = []
+    mlp_feats = []
+    if args.batch_layouts == "mlp":
+        from race_amd import tuner as TU
     t0 = time.time()
     gen = ((kd, n, mean, rp, col, val) for kd, n, mean, rp, col, val in M.batch_matrices(count, args.batch_seed)
            if args.batch_kinds in ("both", kd))
@@ -330,6 +333,8 @@ def bench_batched(args, R, M):
         sx, sb = R.DeviceArray.from_host(xvec), R.DeviceArray.from_host(np.zeros(nrhs * n, np.float32))
         kacz.add(ce, prp, pcol, pval, dx, db)
         kacz_x.append(dx)
+        if args.batch_layouts == "mlp":
+            mlp_feats.append(TU.features(rp, col))
         symm.add(ce, urp, ucol, uval, sx, sb)
         if kept[k] in tune_set:
             tk.add(ce, prp, pcol, pval, dx, db)
@@ -352,7 +357,24 @@ def bench_batched(args, R, M):
     t0 = time.time()
     for bt in (kacz, tk, fk):
         bt.setOption("flow", args.batch_flow)
-    if args.batch_layouts:
+    if args.batch_layouts == "mlp":
+        # One layout serves the whole batch.  The tuner ranks the candidates
+        # by their mean predicted log-time over the matrices; its models were
+        # trained on single matrices, and in a batch the matrices also run
+        # concurrently, so the three best ranked Kaczmarz layouts are timed
+        # on the tuning sample (instead of all twelve)
+        F = np.array(mlp_feats)
+        rank_k = np.argsort(TU.Tuner.pretrained("kacz").scores(F).mean(axis=0))
+        bs = int(np.argmin(TU.Tuner.pretrained("symmspmv").scores(F).mean(axis=0)))
+        times_k = []
+        for c in rank_k[:3]:
+            tk.useCandidate(int(c))
+            tk.build()
+            times_k.append(tk.time(False, reps=2))
+            log(f"tuner-ranked KACZ candidate {int(c)}: {times_k[-1]:.2f} ms")
+        bk = int(rank_k[int(np.argmin(times_k))])
+        times_s = []
+    elif args.batch_layouts:
         bk, bs = (int(v) for v in args.batch_layouts.split(","))
         times_k, times_s = [], []
     else:
@@ -480,7 +502,10 @@ def main():
                     help="Kaczmarz batch: one launch per sweep and shape group, every matrix through its own "
                          "phases (1), or one fused launch per phase index (0)")
     ap.add_argument("--batch-layouts", default="",
-                    help="KACZ,SymmSpMV candidate layout indices (skips the autotune)")
+                    help="KACZ,SymmSpMV candidate layout indices, or mlp (the tuner's SymmSpMV layout and the "
+                         "fastest of its three best ranked KACZ layouts); either skips the full autotune")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the batched workload (512 matrices, tuner layouts) in other_kernels")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the SymmSpMV 64^3 and MPK 256^3 measurements")
     ap.add_argument("--graph", type=int, choices=[0, 1], default=0,
@@ -627,6 +652,11 @@ def main():
             Ox.build()
         extra = {"symmspmv": bench_symmspmv(R, M, O=Ox), "mpk": bench_mpk(R, M, O=Ox),
                  "spmtv": bench_spmtv(R, M)}
+        if not args.no_batched:
+            # the batched workload with the tuner's layouts (no autotune)
+            bargs = argparse.Namespace(**vars(args))
+            bargs.batch_layouts, bargs.batch_flow = "mlp", 1
+            extra["batched"] = bench_batched(bargs, R, M)
 
     # CPU oracle beside it, and the parity of the timed GPU state: the oracle
     # builds its own RACE colouring of the same matrix (LevelRecursion in the
@@ -719,7 +749,8 @@ def main():
     }
     print(json.dumps(out), flush=True)
     bad = [k for k, v in (("gs", parity), ("symmspmv", (extra or {}).get("symmspmv", {}).get("parity")),
-                          ("mpk", (extra or {}).get("mpk", {}).get("parity"))) if v and not v.get("ok")]
+                          ("mpk", (extra or {}).get("mpk", {}).get("parity")),
+                          ("batched", (extra or {}).get("batched", {}).get("parity"))) if v and not v.get("ok")]
     if bad:
         log(f"PARITY FAILED for {bad}: the GPU result differs from the oracle")
         sys.exit(3)

@@ -194,6 +194,8 @@ def sweep_race(rp, col, val, reps=3, stats=None):
     times = np.full(N_CANDIDATES, -1.0)
     t0 = time.time()
     for i, rpt in enumerate(RACE_ROWS_PER_THREAD):
+        if n > 200000:
+            print(f"  race candidate {i} ({rpt} rows per thread) of n = {n}", flush=True)
         ce = Interface(n, max(1, n // rpt), ONE, rp, col, symm_hint=True)
         ce.RACEColor()
         prp, pcol, pval = M.permute(rp, col, val, ce.getPerm(), ce.getInvPerm())
@@ -260,7 +262,9 @@ def sweep_one(kind, rp, col, val, reps=3, stats=None):
     """(features, times[12]) of one matrix; times -1 where a layout does not
     fit.  Matrix coloured by RACE with max(4, n / 256) threads.  stats, if
     given, receives the seconds spent on features and on timing layouts.
-    kind K_MPK sweeps cache budgets, -1 RACE thread counts."""
+    kind K_MPK sweeps cache budgets, -1 RACE thread counts, -2 efficiency
+    thresholds; K_KACZ times the batched workload's Kaczmarz sweeps (FP32,
+    8 right-hand sides, one-launch sweeps), the others FP64 with one."""
     if kind == K_MPK:
         return sweep_mpk(rp, col, val, reps, stats)
     if kind == -1:
@@ -277,9 +281,17 @@ def sweep_one(kind, rp, col, val, reps=3, stats=None):
     prp, pcol, pval = M.make_diag_first(prp, pcol, pval)
     if kind == K_SYMMSPMV:
         prp, pcol, pval = M.upper_triangle(prp, pcol, pval)
-    x = DeviceArray.from_host(M.uniform_vec(n, 7))
-    b = DeviceArray.from_host(M.uniform_vec(n, 8) if kind != K_SYMMSPMV else np.zeros(n))
-    bt = Batch(kind, FP64, FP64, 1)
+    if kind == K_KACZ:
+        # the batched workload's arithmetic: FP32 values and vectors, FP64
+        # row norms, 8 right-hand sides, readiness-driven one-launch sweeps
+        x = DeviceArray.from_host(np.zeros(8 * n, np.float32))
+        b = DeviceArray.from_host(M.uniform_vec(n, 8, nrhs=8).astype(np.float32))
+        bt = Batch(kind, FP32, FP32, 8)
+        bt.setOption("flow", 1)
+    else:
+        x = DeviceArray.from_host(M.uniform_vec(n, 7))
+        b = DeviceArray.from_host(M.uniform_vec(n, 8) if kind != K_SYMMSPMV else np.zeros(n))
+        bt = Batch(kind, FP64, FP64, 1)
     bt.add(ce, prp, pcol, pval, x, b)
     times = np.full(N_CANDIDATES, -1.0)
     t0 = time.time()
@@ -330,13 +342,17 @@ class Tuner:
         return self.mlp.train(self._norm(F), y, m, epochs, lr)
 
     def predict(self, F, precision=BF16):
-        y = self.mlp.forward(self._norm(np.atleast_2d(F)), precision)
-        return np.argmin(y, axis=1)
+        return np.argmin(self.scores(F, precision), axis=1)
+
+    def scores(self, F, precision=BF16):
+        """predicted log(t_i / t_best) of every candidate, per matrix"""
+        return self.mlp.forward(self._norm(np.atleast_2d(F)), precision)
 
     @classmethod
     def pretrained(cls, kind):
         """The model shipped in race_amd/tuner_data for "symmspmv", "gs",
-        "mpk", "race" or "eps" (trained on the sweeps stored beside it)."""
+        "kacz", "mpk", "race" or "eps" (trained on the sweeps stored beside
+        it)."""
         path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuner_data", f"{kind}_model.npz")
         return cls(0).load(path)
 
@@ -369,6 +385,8 @@ def _main(argv=None):
     s.add_argument("--seed", type=int, default=11)
     s.add_argument("--n-hi", type=int, default=100000)
     s.add_argument("--big", type=int, default=0, help="large stencils (1-4.1M rows) among the count matrices")
+    s.add_argument("--stencils", default="",
+                   help="sweep only these 3D stencils instead, e.g. 7:60,27:128 (points:grid size)")
     s.add_argument("--start", type=int, default=0, help="sweep only matrices [start, stop) of the sequence")
     s.add_argument("--stop", type=int, default=None)
     s.add_argument("--out", required=True)
@@ -396,7 +414,17 @@ def _main(argv=None):
         F, T, names, secs, fsecs = [], [], [], [], []
         t0 = time.time()
         stop = a.count if a.stop is None else a.stop
-        for k, (name, rp, col, val) in enumerate(tuner_matrices(a.count, a.seed, n_hi=a.n_hi, big=a.big)):
+        seq = tuner_matrices(a.count, a.seed, n_hi=a.n_hi, big=a.big)
+        if a.stencils:
+            def stencil_seq(spec):
+                for q, item in enumerate(spec.split(",")):
+                    pts, g = (int(v) for v in item.split(":"))
+                    rp, col = (M.stencil7 if pts == 7 else M.stencil27)(g)
+                    yield f"stencil{pts}", rp, col, M.gs_values(rp, col, seed=1000 + q)
+            seq = stencil_seq(a.stencils)
+            stop = len(a.stencils.split(","))
+            a.count = stop
+        for k, (name, rp, col, val) in enumerate(seq):
             if k < a.start or k >= stop:
                 continue
             st = {}

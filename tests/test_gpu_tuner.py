@@ -105,13 +105,15 @@ def test_pretrained_race_threads_tuner_scores_on_held_out():
     assert s_mlp >= 0.85 and s_mlp > s_fixed + 0.1
 
 
-@pytest.mark.parametrize("kind,holdout,floor", [("symmspmv", 200, 0.95), ("gs", 200, 0.95), ("eps", 60, 0.9)])
+@pytest.mark.parametrize("kind,holdout,floor", [("symmspmv", 200, 0.95), ("gs", 200, 0.95), ("kacz", 100, 0.95),
+                                                ("eps", 60, 0.9)])
 def test_pretrained_heads_beat_the_fixed_choice_on_held_out(kind, holdout, floor):
     """The shipped SymmSpMV / GS layout models (2000 training matrices, the
-    held-out 200 include stencils of 3.8M and 4.1M rows) and the efficiency
-    model on the held-out tails of their sweeps: at least the floor of the
-    autotuned best, and better than the fixed choice (layout 0, efficiency
-    40)."""
+    held-out 200 include stencils of 3.8M and 4.1M rows), the Kaczmarz layout
+    model of the batched workload (FP32, 8 right-hand sides, one-launch
+    sweeps) and the efficiency model on the held-out tails of their sweeps:
+    at least the floor of the autotuned best, and better than the fixed
+    choice (layout 0, efficiency 40)."""
     import os
     z = np.load(os.path.join(os.path.dirname(TU.__file__), "tuner_data", f"{kind}_sweep.npz"))
     F, T = z["features"], z["times"]
