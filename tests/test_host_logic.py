@@ -216,3 +216,24 @@ def test_schedule_decoder_under_sanitizers(tmp_path):
     out = subprocess.run([exe], capture_output=True, text=True, timeout=300)
     assert out.returncode == 0, out.stderr[-2000:]
     assert "accepted" in out.stdout
+
+
+def test_flow_dependency_tables_under_sanitizers(tmp_path):
+    """tests/cpp/flow_deps.cpp: the one-launch sweeps' dependency tables and
+    coherent-read bits (chainDeps, flowExt) against conflicts computed
+    directly from a 5-point grid coloured by lines, GS and KACZ, forward and
+    backward, every line a chain of several bundles; with AddressSanitizer
+    and UBSan (g++ only, no device code)."""
+    import shutil
+    if not shutil.which("g++"):
+        pytest.skip("g++ not available")
+    csrc = os.path.join(ROOT, "race_amd", "csrc")
+    exe = str(tmp_path / "flow_deps")
+    subprocess.check_call(["g++", "-O1", "-g", "-std=c++17", "-fsanitize=address,undefined",
+                           "-fno-sanitize-recover=all", "-I", csrc, "-I", os.path.join(ROOT, "include"),
+                           "-I", "/opt/rocm/include", os.path.join(ROOT, "tests", "cpp", "flow_deps.cpp"),
+                           os.path.join(csrc, "bundle.cpp"), os.path.join(csrc, "engine.cpp"), "-o", exe],
+                          timeout=300)
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout[-2000:] + out.stderr[-2000:]
+    assert "flow deps ok" in out.stdout
