@@ -510,6 +510,10 @@ def main():
                     help="skip the SymmSpMV 64^3 and MPK 256^3 measurements")
     ap.add_argument("--graph", type=int, choices=[0, 1], default=0,
                     help="replay each sweep's phase launches as a HIP graph")
+    ap.add_argument("--xcd-order", type=int, choices=[0, 1], default=0,
+                    help="GS bundles: consecutive bundles of a phase on the same XCD")
+    ap.add_argument("--bundle-rows", type=int, default=0, help="GS bundles: rows per bundle (0 = default)")
+    ap.add_argument("--bundle-threads", type=int, default=0, help="GS bundles: workgroup size (0 = default)")
     ap.add_argument("--flow", type=int, choices=[0, 1], default=0,
                     help="one launch per sweep with per-chain dependency flags instead of one per colour phase")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -574,6 +578,11 @@ def main():
     t_color = time.time() - t0
     ce.setOption("graph", args.graph)
     ce.setOption("flow", args.flow)
+    ce.setOption("xcd_order", args.xcd_order)
+    if args.bundle_rows:
+        ce.setOption("bundle_rows", args.bundle_rows)
+    if args.bundle_threads:
+        ce.setOption("bundle_threads", args.bundle_threads)
     bfs_ms, ntrav = ce.preprocessStats()
     perm, inv = ce.getPerm(), ce.getInvPerm()
     log(f"RACEColor T={args.threads}: {t_color:.2f}s (GPU BFS {bfs_ms:.0f} ms in {ntrav} traversals), "

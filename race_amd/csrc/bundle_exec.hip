@@ -1016,9 +1016,11 @@ void launchBundles(int kind, int nrhs, const BundleDev<VT, XT>& d, int c0, int n
                            d.slot.p, d.wave.p, d.off.p, d.len.p, d.extPtr.p, d.slotWave.p, d.rowId.p,
                            d.rowNorm.p, d.ext.p, d.sidx.p, d.sval.p, d.zero, x, b, n, nrhs, d.chainPtr.p, c0, nchain, xcd,
                            d.h.phLocal[p], bX, bB, bN);
-    else if (nrhs == 1 && BS == 256 && !BATCH && d.h.phExt[p] > 2 * BS)
-        // many outside columns: gather up to 6*BS of them in one batch
-        hipLaunchKernelGGL((k_gs_bundle<BS, TPR, EE, VT, XT, true, BATCH, 6>), dim3(grid.x), dim3(BS), d.lds(p, BS),
+    else if (nrhs == 1 && (BS == 256 || BS == 512) && !BATCH && d.h.phExt[p] > 2 * BS)
+        // many outside columns: gather up to 1536 (256 threads) or 2048
+        // (512 threads) of them in one batch
+        hipLaunchKernelGGL((k_gs_bundle<BS, TPR, EE, VT, XT, true, BATCH, (BS == 256 ? 6 : 4)>), dim3(grid.x), dim3(BS),
+                           d.lds(p, BS),
                            st, d.slot.p, d.wave.p, d.off.p, d.len.p, d.extPtr.p, d.slotWave.p, d.rowId.p, d.diag.p,
                            d.ext.p, d.sidx.p, d.sval.p, d.zero, x, b, n, nrhs, d.chainPtr.p, c0, nchain, bundle0, xcd, bX,
                            bB, bN);

@@ -538,3 +538,26 @@ def test_sell_c_sigmize_and_simdify(oracle, T, C, w):
     oracle.sell_c_sigmize(calls, w, C, oc, cs, rl, clp)
     assert ok == oracle.simdify_resolve(w, C, n, oc, cs, clp, ov, True)
     assert np.array_equal(pc, oc) and np.array_equal(pv, ov)
+
+
+@pytest.mark.parametrize("bs", [256, 512])
+def test_gs_bundles_with_wide_halos(oracle, bs):
+    """Bundles of many small leaves (27-point 24^3 at 2048 RACE threads):
+    more than 2 x workgroup-size outside columns per bundle, gathered in one
+    batch (6 x 256 or 4 x 512 id slots, the rest round by round), against
+    the oracle in RACE order."""
+    rp, col = M.stencil27(24)
+    val = M.gs_values(rp, col, seed=17)
+    n = len(rp) - 1
+    ce, (prp, pcol, pval), zt = _prepared(rp, col, val, 2048, R.ONE, oracle)
+    ce.setOption("bundle_threads", bs)
+    b = M.uniform_vec(n, 3)
+    x0 = M.uniform_vec(n, 4)
+    dx, db = R.DeviceArray.from_host(x0), R.DeviceArray.from_host(b)
+    fid = ce.registerDevice(R.K_GS, prp, pcol, pval, dx, db)
+    xo = x0.copy()
+    for rev in (False, True, False):
+        print("wide halo GS", bs, rev, flush=True)
+        ce.executeFunction(fid, rev)
+        oracle.run_calls(oracle.K_GS, prp, pcol, pval, xo, b, oracle.leaf_calls(zt, rev))
+        assert relerr(dx.download(), xo) <= TOL
