@@ -368,8 +368,10 @@ int race_batch_add(race_batch b, race_handle h, const race_kernel_desc *d);
  * (row-sorting window, 1 = none) (SpMV / SymmSpMV); "flow" (Kaczmarz,
  * 256-1024-thread workgroups): 1 = one launch per sweep and matrix-shape
  * group, every matrix advancing through its own colour phases with per-chain
- * dependency flags instead of one fused launch per phase index (default 0;
- * results identical) */
+ * dependency flags instead of one fused launch per phase index; the groups'
+ * launches run concurrently (the group with the longest dependency path on
+ * the given stream, the others on streams of the batch, joined back into it
+ * by events before the call's work ends) (default 0; results identical) */
 int race_batch_set_option(race_batch b, const char *key, int value);
 int race_batch_get_option(race_batch b, const char *key, int *value);
 /* kernel launches of one execution of the built batch; one-launch sweeps

@@ -1083,6 +1083,10 @@ void launchPhase(int bs, int tpr, int E, int kind, int nrhs, const BundleDev<VT,
 // (about seconds; a correct dependency table never gets near it)
 constexpr int kFlowPollLimit = 1 << 22;
 
+// a bundle's prologue, gathers and stores, in dependency waves (weight of a
+// bundle in the longest-path estimate of a readiness-driven sweep)
+constexpr int kBundleFixedWaves = 4;
+
 // Device state of the one-launch sweep of one direction: the bundle range
 // and the dependencies of every chain, column ids with the coherent-read
 // bit, and per right-hand side a flag per chain (tickets in chain order) or
@@ -1098,6 +1102,9 @@ struct FlowDev {
     DevBuf<unsigned long long> queue;
     bool readyMode = false;
     int nroots = 0;
+    // readiness-driven: the longest dependency path of the sweep, in
+    // dependency waves plus kBundleFixedWaves per bundle
+    long longestPath = 0;
     int nchain = 0, grid = 0, nrhs = 1, maxLocal = 0, maxRows = 0;
     size_t lds = 0;
     // chains whose conflicts lie inside one phase (0 for a valid colouring)
@@ -1167,6 +1174,18 @@ struct FlowDev {
             for (int c = 0; c < nchain; ++c)
                 if (qp[c + 1] == qp[c]) rv.push_back(c);
             nroots = (int)rv.size();
+            // longest path from each chain to the end of the sweep (its
+            // successors are later chains)
+            std::vector<long> path(nchain, 0);
+            for (int c = nchain - 1; c >= 0; --c) {
+                long cost = 0;
+                for (int bnd = hlo[c]; bnd < hhi[c]; ++bnd)
+                    cost += d.h.wave[bnd + 1] - d.h.wave[bnd] + kBundleFixedWaves;
+                long tail = 0;
+                for (int j = sp[c]; j < sp[c + 1]; ++j) tail = std::max(tail, path[sv[j]]);
+                path[c] = cost + tail;
+                longestPath = std::max(longestPath, path[c]);
+            }
             if ((e = upload(succPtr, sp.data(), sp.size()))) return e;
             if ((e = upload(succ, sv.data(), sv.size()))) return e;
             if ((e = upload(roots, rv.data(), rv.size()))) return e;
@@ -1475,6 +1494,21 @@ class BundleBatchFn : public DeviceFn {
     // earlier chains, instead of every fused phase waiting for the slowest
     // matrix
     bool flow = false, flowOk = true;
+    // flow: the groups hold different matrices, so their launches run
+    // concurrently, the group with the longest dependency path first (on
+    // the caller's stream), so that its long chains start at once and the
+    // other groups (each on a stream of its own, forked from and joined back
+    // into the caller's stream by events) fill the remaining workgroup slots
+    std::vector<hipStream_t> side;
+    std::vector<hipEvent_t> joinEv;
+    hipEvent_t forkEv = nullptr;
+    std::vector<int> launchOrder[2];
+
+    ~BundleBatchFn() override {
+        for (hipStream_t s : side) (void)hipStreamDestroy(s);
+        for (hipEvent_t e : joinEv) (void)hipEventDestroy(e);
+        if (forkEv) (void)hipEventDestroy(forkEv);
+    }
 
     int init(const std::vector<BatchItem>& items, const ExecContext& ctx) {
         if (items.empty()) return ERR_INVALID_ARG;
@@ -1565,6 +1599,24 @@ class BundleBatchFn : public DeviceFn {
             groups.push_back(std::move(grp));
         }
         flow = flowOk && ctx.flow && kind == RACE_K_KACZ && kaczFlowBs(bs);
+        for (int dir = 0; dir < 2; ++dir) {
+            launchOrder[dir].resize(groups.size());
+            for (size_t g = 0; g < groups.size(); ++g) launchOrder[dir][g] = (int)g;
+            std::stable_sort(launchOrder[dir].begin(), launchOrder[dir].end(), [&](int a, int b) {
+                return groups[a]->flow[dir].longestPath > groups[b]->flow[dir].longestPath;
+            });
+        }
+        if (flow && groups.size() > 1) {
+            RACE_HIP(hipEventCreateWithFlags(&forkEv, hipEventDisableTiming));
+            for (size_t g = 1; g < groups.size(); ++g) {
+                hipStream_t s = nullptr;
+                hipEvent_t e = nullptr;
+                RACE_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+                side.push_back(s);
+                RACE_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+                joinEv.push_back(e);
+            }
+        }
         for (int dir = 0; dir < 2; ++dir)
             for (int p = 0; p < maxPhases[dir]; ++p) {
                 int w = 0;
@@ -1598,11 +1650,17 @@ class BundleBatchFn : public DeviceFn {
     int run(bool rev, hipStream_t st) override {
         const int dir = rev ? 1 : 0;
         if (flow) {
-            for (const auto& g : groups) {
-                int e = launchKaczFlowShape<VT, XT, true>(bs, g->tpr, g->E, g->dev[dir], g->flow[dir], nullptr,
-                                                          nullptr, 0, g->bx[dir].p, g->bb[dir].p, g->bn[dir].p, st);
+            if (!side.empty()) RACE_HIP(hipEventRecord(forkEv, st));
+            for (size_t q = 0; q < groups.size(); ++q) {
+                Group& g = *groups[launchOrder[dir][q]];
+                hipStream_t s = q == 0 ? st : side[q - 1];
+                if (q > 0) RACE_HIP(hipStreamWaitEvent(s, forkEv, 0));
+                int e = launchKaczFlowShape<VT, XT, true>(bs, g.tpr, g.E, g.dev[dir], g.flow[dir], nullptr, nullptr,
+                                                          0, g.bx[dir].p, g.bb[dir].p, g.bn[dir].p, s);
                 if (e) return e;
+                if (q > 0) RACE_HIP(hipEventRecord(joinEv[q - 1], s));
             }
+            for (hipEvent_t e : joinEv) RACE_HIP(hipStreamWaitEvent(st, e, 0));
             return OK;
         }
         // phase p of every group before phase p + 1 of any (stream order)
