@@ -206,3 +206,44 @@ def test_flow_batch_kacz_equals_fused_phases(bs):
         for ce, (xa, xb) in keep:
             assert np.array_equal(xa.download(), xb.download()), s
     assert batches[1].flowAborts() == 0
+
+
+def test_flow_batch_async_sweeps_in_order():
+    """Several shape groups, so each sweep forks the groups' launches onto
+    streams of the batch and joins them back into the batch's stream: four
+    sweeps queued back to back without synchronisation (each sweep's launches
+    must start after the previous sweep's joins) equal four synchronous
+    sweeps bit for bit, and the last synchronous call waits for every
+    group."""
+    mats = list(M.batch_matrices(8, 11, 500, 6000))
+    batches = [R.Batch(R.K_KACZ, R.FP32, R.FP32, nrhs=8) for _ in range(2)]
+    keep = []
+    for k, (name, n, mean, rp, col, val) in enumerate(mats):
+        ce = R.Interface(n, max(4, n // 256), R.TWO, rp, col, symm_hint=True)
+        ce.RACEColor()
+        perm, inv = ce.getPerm(), ce.getInvPerm()
+        prp, pcol, pval = M.permute(rp, col, val, perm, inv)
+        prp, pcol, pval = M.make_diag_first(prp, pcol, pval)
+        b = M.uniform_vec(n, 90 + k, nrhs=8).astype(np.float32)
+        pair = []
+        for bt in batches:
+            dx = R.DeviceArray.from_host(np.zeros(8 * n, np.float32))
+            db = R.DeviceArray.from_host(b)
+            bt.add(ce, prp, pcol, pval, dx, db)
+            pair.append(dx)
+        keep.append((ce, pair))
+    for bt in batches:
+        bt.setOption("bundle_threads", 512)
+        bt.setOption("bundle_rows", 256)
+        bt.setOption("flow", 1)
+        bt.build()
+    assert batches[0].launchCount() > 1, "the case needs several shape groups"
+    print("flow batch async: 4 queued sweeps", flush=True)
+    for s in range(3):
+        batches[0].executeAsync(bool(s % 2))
+    batches[0].execute(True)
+    for s in range(4):
+        batches[1].execute(bool(s % 2))
+    for ce, (xa, xb) in keep:
+        assert np.array_equal(xa.download(), xb.download())
+    assert batches[0].flowAborts() == 0 and batches[1].flowAborts() == 0
