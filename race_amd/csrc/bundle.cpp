@@ -508,4 +508,23 @@ std::vector<int> flowExt(const BundleTable& t, long nkeys, const std::vector<lon
     return out;
 }
 
+std::vector<long> chainLongestPaths(const BundleTable& t, const std::vector<int>& depPtr,
+                                    const std::vector<int>& dep, int fixedPerBundle) {
+    const int nc = (int)t.chainPtr.size() - 1;
+    std::vector<long> path(std::max(nc, 0), 0);
+    // successors of every chain, all later than it: a backward pass sees
+    // them finished
+    std::vector<std::vector<int>> succ(std::max(nc, 0));
+    for (int c = 0; c < nc; ++c)
+        for (int j = depPtr[c]; j < depPtr[c + 1]; ++j) succ[dep[j]].push_back(c);
+    for (int c = nc - 1; c >= 0; --c) {
+        long cost = 0;
+        for (int b = t.chainPtr[c]; b < t.chainPtr[c + 1]; ++b) cost += t.wave[b + 1] - t.wave[b] + fixedPerBundle;
+        long tail = 0;
+        for (int s : succ[c]) tail = std::max(tail, path[s]);
+        path[c] = cost + tail;
+    }
+    return path;
+}
+
 }  // namespace race

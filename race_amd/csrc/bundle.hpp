@@ -99,5 +99,13 @@ long chainDeps(const BundleTable& t, long nkeys, const std::vector<long>* colBas
 // dependency wait; the others still hold the value the launch started with.
 std::vector<int> flowExt(const BundleTable& t, long nkeys, const std::vector<long>* colBase);
 
+// Longest dependency path from every chain to the end of the sweep, for
+// dependencies as chainDeps returns them (each an earlier chain): a chain
+// costs its bundles' dependency waves plus fixedPerBundle per bundle, and
+// its path is that cost plus the longest path among the chains that depend
+// on it.
+std::vector<long> chainLongestPaths(const BundleTable& t, const std::vector<int>& depPtr,
+                                    const std::vector<int>& dep, int fixedPerBundle);
+
 
 }  // namespace race

@@ -1174,18 +1174,8 @@ struct FlowDev {
             for (int c = 0; c < nchain; ++c)
                 if (qp[c + 1] == qp[c]) rv.push_back(c);
             nroots = (int)rv.size();
-            // longest path from each chain to the end of the sweep (its
-            // successors are later chains)
-            std::vector<long> path(nchain, 0);
-            for (int c = nchain - 1; c >= 0; --c) {
-                long cost = 0;
-                for (int bnd = hlo[c]; bnd < hhi[c]; ++bnd)
-                    cost += d.h.wave[bnd + 1] - d.h.wave[bnd] + kBundleFixedWaves;
-                long tail = 0;
-                for (int j = sp[c]; j < sp[c + 1]; ++j) tail = std::max(tail, path[sv[j]]);
-                path[c] = cost + tail;
-                longestPath = std::max(longestPath, path[c]);
-            }
+            // queue positions are chain numbers (order is the identity)
+            for (long p : chainLongestPaths(d.h, dp, dd, kBundleFixedWaves)) longestPath = std::max(longestPath, p);
             if ((e = upload(succPtr, sp.data(), sp.size()))) return e;
             if ((e = upload(succ, sv.data(), sv.size()))) return e;
             if ((e = upload(roots, rv.data(), rv.size()))) return e;

@@ -10,7 +10,11 @@
 //  * the coherent-read bit marks exactly the columns an earlier chain, or an
 //    earlier bundle of the same chain, writes (GS) or touches (KACZ);
 //  * with one dependency removed the check reports a violation (the check
-//    is not vacuous).
+//    is not vacuous);
+//  * the longest dependency paths (chainLongestPaths) equal a fixpoint of
+//    path[a] >= cost[a] + path[c] over every dependency a of c, relaxed in
+//    no particular order, and a chain no other depends on costs its own
+//    bundles only.
 // Prints "flow deps ok" and exits 0 on success.
 #include <algorithm>
 #include <cstdio>
@@ -150,6 +154,35 @@ void check(const Grid& g, int kind, bool rev) {
             if (!(c == victim && j == dp[c])) dd2.push_back(dd[j]);
     for (int c = victim; c < nc; ++c) --dp2[c + 1];
     EXPECT(unordered(g, t, kind, dp2, dd2) > 0, "kind %d: a removed dependency went unnoticed", kind);
+    // longest paths against a fixpoint relaxation (no use of chain order)
+    const int fixed = 4;
+    const std::vector<long> lp = chainLongestPaths(t, dp, dd, fixed);
+    EXPECT((int)lp.size() == nc, "longest paths: %d for %d chains", (int)lp.size(), nc);
+    std::vector<long> cost(nc, 0), fp(nc, 0);
+    std::vector<char> hasSucc(nc, 0);
+    for (int c = 0; c < nc; ++c) {
+        for (int b = t.chainPtr[c]; b < t.chainPtr[c + 1]; ++b) cost[c] += t.wave[b + 1] - t.wave[b] + fixed;
+        fp[c] = cost[c];
+    }
+    for (bool changed = true; changed;) {
+        changed = false;
+        for (int c = nc - 1; c >= 0; --c)  // against the chain order on purpose
+            for (int j = dp[c]; j < dp[c + 1]; ++j) {
+                const int a = dd[j];
+                hasSucc[a] = 1;
+                if (cost[a] + fp[c] > fp[a]) {
+                    fp[a] = cost[a] + fp[c];
+                    changed = true;
+                }
+            }
+    }
+    long longest = 0;
+    for (int c = 0; c < nc && c < (int)lp.size(); ++c) {
+        EXPECT(lp[c] == fp[c], "kind %d rev %d: chain %d path %ld, fixpoint %ld", kind, (int)rev, c, lp[c], fp[c]);
+        if (!hasSucc[c]) EXPECT(lp[c] == cost[c], "chain %d without successors: path %ld, cost %ld", c, lp[c], cost[c]);
+        longest = std::max(longest, lp[c]);
+    }
+    EXPECT(longest > cost[0], "kind %d rev %d: no path longer than one chain", kind, (int)rev);
     // coherent-read bits
     const std::vector<int> fx = flowExt(t, g.n, nullptr);
     std::vector<int> chainOf(t.slot.size() - 1), owner(g.n, -1);
